@@ -1,0 +1,183 @@
+"""Benchmark: converged states/s of the ITP loop on one MI355X.
+
+A step is one complete ITP solve of the workload (seeded initial states,
+propagate -> orthonormalize -> energies until the convergence tests pass),
+with all inputs resident in HBM and the per-solve setup (tables, allocations,
+uploads) done before the timed region, as the reference starts its timer at
+the first step (itpsystem.cpp:416-421). value = converged states of all timed
+solves / wall time of the timed solves.
+
+Default workload: the 256x256 Dirichlet grid, N = 500, oscillator + 40
+Gaussian impurities, B = 1, 4th-order multi-product split, eps = 0.05, on the
+plain FP64 GPU path.
+
+Prints one JSON line. Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "converged states/s @1e-8 rel eigenvalue err, mixed vs FP64 GPU path; %HBM"
+HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TFLOPS = 78.6
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def sweep_transfers(cfg):
+    """Field transfers (N*M*s bytes each) per propagate call and per energies call."""
+    h = cfg["halforder"]
+    p_prop = 2 * h * h + 5 * h - 1  # 2t+1 sweeps of 2 transfers per member, plus h-1 accumulator reads
+    if cfg["B"] != 0:
+        p_en = 10   # per H: row (2) + column with add and dot (3); twice, the last without a write
+        n_en = 4
+    else:
+        p_en = 14   # per H: row (2) + column (2) + row with V*psi and dot (3); twice
+        n_en = 6
+    return p_prop, h * h + 2 * h, p_en, n_en
+
+
+def cpu_baseline(cfg, V, per_solve_iters, per_solve_converged, sample_steps):
+    from oracle import pyoracle as O
+    threads = O.num_threads()
+    sysc = O.System(cfg, V)
+    t0 = time.perf_counter()
+    for _ in range(sample_steps):
+        sysc.step()
+    dt = time.perf_counter() - t0
+    s_per_iter = dt / sample_steps
+    value = per_solve_converged / (per_solve_iters * s_per_iter)
+    return {"value": value, "unit": "converged states/s", "cores": threads, "kind": "port",
+            "sample": f"{sample_steps} ITP iteration(s) of the same workload with the FP64 C++ oracle "
+                      f"({s_per_iter:.2f} s/iteration, OMP threads {threads}), scaled by the GPU solve's "
+                      f"{per_solve_iters} iterations and {per_solve_converged} converged states"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2_dirichlet256")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--cpu-sample-steps", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the single-GPU throughput is the reported value)", file=sys.stderr)
+        sys.exit(2)
+
+    import itp2d_amd as I
+    from itp2d_amd import workloads as W
+
+    cfg, V = W.config(args.workload)
+    cfg = dict(cfg, precision=I.FP32 if args.precision == "fp32" else I.FP64)
+    N, M = cfg["N"], cfg["sx"] * cfg["sy"]
+    esz = 8 if args.precision == "fp32" else 16
+    ctx = I.Context(0)
+    log(f"workload {args.workload}: {cfg['sx']}x{cfg['sy']}, N={N}, h={cfg['halforder']}, B={cfg['B']}, "
+        f"{args.precision}; CUs={ctx.num_cus}")
+
+    for w in range(args.warmup):
+        s = I.ITPSystem(ctx, cfg, V)
+        rc = s.run()
+        log(f"warmup solve {w}: rc={rc} steps={s.total_steps} converged={s.num_converged} "
+            f"error={s.error_flag} {s.total_time:.2f}s")
+        del s
+
+    systems = [I.ITPSystem(ctx, cfg, V) for _ in range(args.steps)]
+    ctx.enable_timing(True)
+    ctx.reset_phase_times()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for k, s in enumerate(systems):
+        s.run()
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    phases = ctx.phase_times()
+    ctx.enable_timing(False)
+
+    iters = [s.total_steps for s in systems]
+    conv = [s.num_converged for s in systems]
+    errs = [s.error_flag for s in systems]
+    for k, s in enumerate(systems):
+        log(f"timed solve {k}: steps={iters[k]} converged={conv[k]} error={errs[k]} {s.total_time:.3f}s")
+    value = sum(conv) / wall
+    E0 = systems[0].sorted()[0]
+
+    # roofline of the dominant kernel group, from the HIP events of the timed region
+    p_prop, n_prop, p_en, n_en = sweep_transfers(cfg)
+    t_prop, c_prop = phases["propagate"]
+    t_en, c_en = phases["energies"]
+    t_ov, c_ov = phases["overlap"]
+    t_eig, c_eig = phases["eigensolve"]
+    t_lc, c_lc = phases["lincomb"]
+    field = N * M * esz
+    sweep_bytes = c_prop * p_prop * field + c_en * p_en * field
+    sweep_ms = t_prop + t_en
+    sweep_launches = c_prop * n_prop + c_en * n_en
+    gram_flops = 8.0 * M * N * (N + 1) / 2 * c_ov
+    rot_flops = 8.0 * M * N * N * c_lc
+    groups = {
+        "sweep_kernel": dict(bound="hbm", ms=sweep_ms, achieved=sweep_bytes / (sweep_ms * 1e-3) / 1e9,
+                             peak=HBM_PEAK_GBS, unit="GB/s", launches=sweep_launches,
+                             per_launch_bytes=sweep_bytes / max(sweep_launches, 1)),
+        "gram_kernel": dict(bound="mfma", ms=t_ov, achieved=gram_flops / (t_ov * 1e-3) / 1e12 if t_ov else 0,
+                            peak=FP64_MFMA_PEAK_TFLOPS, unit="TFLOP/s", launches=c_ov),
+        "rotate_kernel": dict(bound="mfma", ms=t_lc, achieved=rot_flops / (t_lc * 1e-3) / 1e12 if t_lc else 0,
+                              peak=FP64_MFMA_PEAK_TFLOPS, unit="TFLOP/s", launches=c_lc),
+    }
+    dom = max(groups, key=lambda k: groups[k]["ms"])
+    g = groups[dom]
+    roofline = {"kernel": dom, "bound": g["bound"], "achieved": round(g["achieved"], 3), "peak": g["peak"],
+                "unit": g["unit"], "frac": round(g["achieved"] / g["peak"], 4), "traffic": None,
+                "avg_launch_ms": round(g["ms"] / max(g["launches"], 1), 5)}
+    others = {k: {"achieved": round(v["achieved"], 3), "unit": v["unit"], "frac": round(v["achieved"] / v["peak"], 4),
+                  "ms": round(v["ms"], 3)} for k, v in groups.items() if k != dom}
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "converged states/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "c32-fields/f64-dense",
+        "data": "synthetic (seeded uniform-noise initial states, seeded impurity potential)",
+        "config": {
+            "workload": args.workload,
+            "grid": f"{cfg['sx']}x{cfg['sy']}", "boundary": "dirichlet" if cfg["boundary"] else "periodic",
+            "N": N, "n_wanted": cfg["n_wanted"], "halforder": cfg["halforder"], "B": cfg["B"],
+            "eps": cfg["eps_values"], "tests": [list(cfg["timestep_test"]), list(cfg["final_test"])],
+            "path": args.precision,
+            "itp_iterations_per_solve": iters[0], "converged_per_solve": conv[0],
+            "itp_iterations_per_s": round(sum(iters) / wall, 3),
+            "E_lowest": [round(x, 10) for x in E0[:4].tolist()],
+            "phase_ms": {k: round(v[0], 3) for k, v in phases.items()},
+            "phase_calls": {k: v[1] for k, v in phases.items()},
+        },
+        "roofline": roofline,
+        "roofline_other": others,
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline sample ...")
+        line["cpu_baseline"] = cpu_baseline(W.config(args.workload)[0], V, iters[0], conv[0], args.cpu_sample_steps)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,172 @@
+/*
+ * itp2d.h -- C ABI of the MI355X (gfx950) imaginary-time-propagation engine.
+ *
+ * Plain C: opaque handles, pointers and sizes, int status codes. No C++
+ * exceptions cross this boundary; the C++ facade (itp2d_amd/host/itp2d.hpp)
+ * maps the codes back to the reference's exception types.
+ *
+ * Each entry point names the reference interface it replaces (paths under
+ * src/ of the reference). All work is queued on the context's one HIP stream;
+ * calls that return host-visible results synchronise that stream.
+ *
+ * Field layout in HBM: [N][sy][sx] interleaved complex (re, im), the same
+ * order as the reference's StateArray (statearray.cpp:21-75); double for
+ * ITP_FP64 sets, float for ITP_FP32 sets. Host-side buffers passed to this API
+ * are always interleaved complex double.
+ */
+#ifndef ITP2D_H
+#define ITP2D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. The three eigen errors keep the reference's semantics:
+ * NonPositiveEigenvalue / NonNormalEigenvalue (stateset.cpp:164-173,
+ * exceptions.hpp:55-86) and EigensolverError (eigensolver.hpp:80-84). */
+enum itp_status {
+  ITP_OK = 0,
+  ITP_E_INVALID = 1,
+  ITP_E_NONPOSITIVE_EIG = 2,
+  ITP_E_NONNORMAL_EIG = 3,
+  ITP_E_EIGENSOLVER = 4,
+  ITP_E_HIP = 5,
+  ITP_E_UNSUPPORTED = 6,
+  ITP_E_NOMEM = 7
+};
+
+/* BoundaryType (itp2d_common.hpp:50) */
+enum itp_boundary { ITP_PERIODIC = 0, ITP_DIRICHLET = 1 };
+
+/* enum Transform, same order (transformer.hpp:27-28) */
+enum itp_transform_kind {
+  ITP_FFT = 0, ITP_iFFT, ITP_FFTx, ITP_iFFTx, ITP_FFTy, ITP_iFFTy,
+  ITP_DST, ITP_iDST, ITP_DSTx, ITP_iDSTx, ITP_DSTy, ITP_iDSTy,
+  ITP_DCT, ITP_iDCT, ITP_DCTx, ITP_iDCTx, ITP_DCTy, ITP_iDCTy
+};
+
+enum itp_precision { ITP_FP64 = 0, ITP_FP32 = 1 };
+
+typedef struct itp_ctx itp_ctx;
+typedef struct itp_grid itp_grid;
+typedef struct itp_potential itp_potential;
+typedef struct itp_stateset itp_stateset;
+typedef struct itp_split itp_split;
+typedef struct itp_hamiltonian itp_hamiltonian;
+
+/* ---- context: one device, one stream (replaces the process-wide OpenMP
+ * setup, itpsystem.cpp:51) ---- */
+int itp_ctx_create(int device, itp_ctx** out);
+void itp_ctx_destroy(itp_ctx* ctx);
+const char* itp_last_error(const itp_ctx* ctx);
+int itp_ctx_synchronize(itp_ctx* ctx);
+/* Device properties used by measurement code: number of CUs. */
+int itp_ctx_num_cus(const itp_ctx* ctx);
+
+/* ---- grid: DataLayout (datalayout.hpp:30-54, datalayout.cpp:21-36) plus the
+ * Transformer's k tables and normalisation factors (transformer.cpp:66-90).
+ * dx = lenx/sx as Parameters::get_grid_delta (parameters.hpp:97). ---- */
+int itp_grid_create(itp_ctx* ctx, size_t sx, size_t sy, double dx, int boundary, itp_grid** out);
+void itp_grid_destroy(itp_grid* g);
+/* Transformer::fft_kx/fft_ky/dsct_kx/dsct_ky (transformer.hpp:36-45); any pointer may be NULL. */
+int itp_grid_kvalues(const itp_grid* g, double* fft_kx, double* fft_ky, double* dsct_kx, double* dsct_ky);
+/* Transformer::normalization_factor(Transform) (transformer.hpp:94-130) */
+double itp_grid_normalization_factor(const itp_grid* g, int kind);
+/* DataLayout::get_posx/get_posy (datalayout.hpp:56-61) */
+int itp_grid_positions(const itp_grid* g, double* posx, double* posy);
+
+/* Transformer::transform(comp*, Transform) (transformer.hpp:132-158) applied to
+ * nfields fields already on the device (interleaved complex double). */
+int itp_transform_device(itp_grid* g, void* fields, size_t nfields, int kind);
+/* Same on host buffers (upload, transform, download). */
+int itp_transform_host(itp_grid* g, double* fields, size_t nfields, int kind);
+
+/* ---- potential: Potential (potential.hpp:31-58). V is [sy][sx] doubles;
+ * V == NULL is the null potential (Potential::is_null). ---- */
+int itp_potential_create(itp_grid* g, const double* V, itp_potential** out);
+void itp_potential_destroy(itp_potential* p);
+
+/* ---- state set: StateSet storage (stateset.cpp:26-48); ITP_FP64 or ITP_FP32
+ * fields. Host buffers are [N][sy][sx] interleaved complex double. ---- */
+int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out);
+void itp_stateset_destroy(itp_stateset* s);
+int itp_stateset_upload(itp_stateset* s, const double* interleaved);
+int itp_stateset_download(itp_stateset* s, double* interleaved);
+/* Device pointer of the current field buffer (changes after orthonormalize and propagate). */
+void* itp_stateset_device_ptr(itp_stateset* s);
+size_t itp_stateset_size(const itp_stateset* s);
+
+/* ---- evolution operator: MultiProductSplit (multiproductsplit.cpp:23-58) built
+ * from SecondOrderSplit members (secondordersplit.cpp:21-69), ExpKinetic
+ * (expkinetic.cpp:33-197) and ExpPotential (exppotential.cpp:28-63). ---- */
+int itp_split_create(itp_potential* pot, int halforder, double eps, double B, itp_split** out);
+void itp_split_destroy(itp_split* sp);
+/* EvolutionOperator::set_time_step (operators.hpp:65-69; multiproductsplit.cpp:43-47) */
+int itp_split_set_time_step(itp_split* sp, double eps);
+/* Effective half order (1 for a null potential, multiproductsplit.cpp:24). */
+int itp_split_halforder(const itp_split* sp);
+/* ITPSystem::propagate: T applied to every state (itpsystem.cpp:243-256);
+ * Operator::operate (operators.hpp:33-52) batched over the whole set. */
+int itp_propagate(itp_split* sp, itp_stateset* s);
+
+/* ---- StateSet::orthonormalize (stateset.cpp:136-232): overlap matrix,
+ * Hermitian eigendecomposition, eigenvalue checks, Psi <- Psi U Lambda^(-1/2).
+ * On ITP_E_NONPOSITIVE_EIG / ITP_E_NONNORMAL_EIG the states are left as they
+ * were and *bad_n, *bad_lambda (nullable) name the offending eigenvalue in
+ * ascending order, like NonPositiveEigenvalue(n, eval, ...). ---- */
+int itp_orthonormalize(itp_stateset* s, size_t* bad_n, double* bad_lambda);
+/* Overlap matrix S_ij = <psi_i|psi_j> as computed by itp_orthonormalize
+ * (stateset.cpp:145-154); out is N*N complex, column-major, full Hermitian. */
+int itp_overlap(itp_stateset* s, double* out);
+
+/* ---- Hamiltonian H = Kinetic + Potential (itpsystem.cpp:117-119;
+ * kinetic.cpp:25-142) and Operator::mean_and_standard_deviation
+ * (operators.cpp:40-50) for every state, H applied twice literally.
+ * E[n] = Re<psi_n|H psi_n>, sigma[n] = Re sqrt(<psi_n|H H psi_n> - <H>^2), by
+ * slot n (unsorted, like the tuples before the sort at itpsystem.cpp:496). ---- */
+int itp_hamiltonian_create(itp_potential* pot, double B, itp_hamiltonian** out);
+void itp_hamiltonian_destroy(itp_hamiltonian* h);
+int itp_energies(itp_hamiltonian* h, itp_stateset* s, double* E, double* sigma);
+/* Complex <H> and <H^2> per slot (interleaved re, im; each nullable). */
+int itp_energies_complex(itp_hamiltonian* h, itp_stateset* s, double* mean, double* meansq);
+
+/* ---- single-operator hooks (Operator::operate on host fields, for parity
+ * checks; fields are [n][sy][sx] interleaved complex double, in place) ---- */
+/* ExpKinetic(e, B, tr, bt, coeff, prefactor) (expkinetic.hpp:38-39) */
+int itp_apply_expkinetic_host(itp_grid* g, double B, double e, double coeff, double prefactor,
+                              double* fields, size_t n);
+/* ExpPotential(pot, e, coeff, prefactor) (exppotential.hpp:33) */
+int itp_apply_exppotential_host(itp_potential* p, double e, double coeff, double prefactor,
+                                double* fields, size_t n);
+/* SecondOrderSplit(pot, e, B, tr, bt, prefactor, exponent) (secondordersplit.cpp:21) */
+int itp_apply_secondorder_host(itp_potential* p, double B, double e, double prefactor, int exponent,
+                               double* fields, size_t n);
+/* H = T + V applied once (OperatorSum::operate, operatorsum.cpp:61-86) */
+int itp_apply_hamiltonian_host(itp_hamiltonian* h, double* fields, size_t n);
+/* EigenSolver::solve (eigensolver.hpp:80-84) on the GPU: A is n*n complex
+ * column-major (upper triangle read); on return A holds eigenvectors in
+ * columns and w the eigenvalues in ascending order, like zheev('V','U'). */
+int itp_eigh_host(itp_ctx* ctx, size_t n, double* A, double* w);
+
+/* Seeded initial states: re, im ~ U[-1,1] from a splitmix64 stream, each state
+ * normalised (StateSet::init_to_gaussian_noise, stateset.cpp:124-131, with
+ * uniform noise). Host-side setup, not part of the timed loop. */
+int itp_init_states(uint64_t* rng_state, size_t sx, size_t sy, double dx, size_t N, double* out);
+
+/* ---- timing of the last calls on the context's stream (milliseconds, from
+ * HIP events recorded around each kernel group; 0 if not recorded). ---- */
+enum itp_phase { ITP_PHASE_PROPAGATE = 0, ITP_PHASE_OVERLAP, ITP_PHASE_EIGENSOLVE,
+                 ITP_PHASE_LINCOMB, ITP_PHASE_ENERGIES, ITP_NUM_PHASES };
+int itp_ctx_enable_timing(itp_ctx* ctx, int enable);
+/* Accumulated event time per phase since the last reset (ms). */
+int itp_ctx_phase_times(itp_ctx* ctx, double* ms, long long* counts);
+int itp_ctx_reset_phase_times(itp_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ITP2D_H */

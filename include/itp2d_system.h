@@ -1,0 +1,79 @@
+/*
+ * itp2d_system.h -- C ABI of the ITP driver (the reference's ITPSystem,
+ * itpsystem.hpp:55-156) running on the gfx950 engine of itp2d.h.
+ *
+ * The loop semantics follow itpsystem.cpp:202-511 and convergence.cpp:75-200:
+ * sorted (E, sigma, slot) tuples, sticky time-step convergence flags, the eps
+ * schedule (list, then division), the max-steps and min-eps bail-outs and the
+ * optional recovery after a non-positive overlap eigenvalue.
+ */
+#ifndef ITP2D_SYSTEM_H
+#define ITP2D_SYSTEM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "itp2d.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ConvergenceTest kinds (convergence.cpp:28-49) */
+enum itps_test_kind { ITPS_TEST_NONE = 0, ITPS_TEST_ONESTEP, ITPS_TEST_ABSCHANGE,
+                      ITPS_TEST_RELCHANGE, ITPS_TEST_ABSSTDEV, ITPS_TEST_RELSTDEV };
+
+typedef struct itps_test { int kind; double p0, p1; } itps_test;
+
+/* The Parameters fields the loop consumes (parameters.hpp, defaults at
+ * parameters.cpp:25-55). dx = lenx/sx (parameters.hpp:97). */
+typedef struct itps_params {
+  size_t sx, sy;
+  double lenx;
+  int boundary;
+  size_t N;
+  size_t n_wanted;
+  size_t ignore_lowest;
+  int halforder;
+  double B;
+  const double* eps_values; size_t n_eps;
+  double eps_divisor;
+  int exhaust_eps;
+  int max_steps;
+  double min_time_step;
+  int recover;
+  uint64_t seed;            /* initial-state noise seed, continued on recovery */
+  itps_test timestep_test, final_test;
+  int precision;            /* ITP_FP64 or ITP_FP32 fields */
+} itps_params;
+
+typedef struct itps_system itps_system;
+
+/* V may be NULL (null potential); init_states ([N][sy][sx] complex double) may
+ * be NULL, in which case seeded noise is used (itp_init_states). */
+int itps_create(itp_ctx* ctx, const itps_params* p, const double* V, const double* init_states, itps_system** out);
+void itps_destroy(itps_system* s);
+/* ITPSystem::step (itpsystem.cpp:407-470) */
+int itps_step(itps_system* s);
+/* while (!finished) step(); returns the status of the last failing call or ITP_OK */
+int itps_run(itps_system* s);
+int itps_finished(const itps_system* s);
+int itps_error_flag(const itps_system* s);
+int itps_total_steps(const itps_system* s);
+double itps_eps(const itps_system* s);
+size_t itps_num_converged(const itps_system* s);
+size_t itps_num_timestep_converged(const itps_system* s);
+/* Sorted energies, deviations and slot indices of the latest step (length N each, nullable). */
+int itps_sorted(const itps_system* s, double* E, double* sigma, size_t* index);
+/* Final states, by slot ([N][sy][sx] complex double). */
+int itps_states(itps_system* s, double* out);
+/* Wall time from the first step to finish (s), as the reference's total timer (itpsystem.cpp:416-421,533). */
+double itps_total_time(const itps_system* s);
+/* Number of orthonormalization failures recovered from. */
+int itps_recoveries(const itps_system* s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ITP2D_SYSTEM_H */

@@ -1,0 +1,462 @@
+"""ctypes binding of libitp2d_amd.so (C ABI: include/itp2d.h, include/itp2d_system.h).
+
+Host arrays are numpy complex128 of shape [N, sy, sx] (the reference's
+StateArray order, statearray.cpp:21-75); potentials are float64 [sy, sx].
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+library_path = os.path.join(HERE, "lib", "libitp2d_amd.so")
+
+TRANSFORMS = ["FFT", "iFFT", "FFTx", "iFFTx", "FFTy", "iFFTy", "DST", "iDST", "DSTx",
+              "iDSTx", "DSTy", "iDSTy", "DCT", "iDCT", "DCTx", "iDCTx", "DCTy", "iDCTy"]
+PERIODIC, DIRICHLET = 0, 1
+FP64, FP32 = 0, 1
+TESTS = {"none": 0, "onestep": 1, "abschange": 2, "relchange": 3, "absstdev": 4, "relstdev": 5}
+PHASES = ["propagate", "overlap", "eigensolve", "lincomb", "energies"]
+
+ITP_OK, ITP_E_INVALID, ITP_E_NONPOSITIVE_EIG, ITP_E_NONNORMAL_EIG, ITP_E_EIGENSOLVER = 0, 1, 2, 3, 4
+
+
+class ITPError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class NonPositiveEigenvalue(ITPError):
+    """exceptions.hpp:55-71: the overlap matrix has an eigenvalue <= 0."""
+
+    def __init__(self, code, msg, n, value):
+        super().__init__(code, msg)
+        self.n = n
+        self.value = value
+
+
+class NonNormalEigenvalue(NonPositiveEigenvalue):
+    """exceptions.hpp:73-86"""
+
+
+class EigensolverError(ITPError):
+    """exceptions.hpp:237-244"""
+
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+_sz = C.c_size_t
+_vp = C.c_void_p
+
+
+class _Test(C.Structure):
+    _fields_ = [("kind", C.c_int), ("p0", C.c_double), ("p1", C.c_double)]
+
+
+class _Params(C.Structure):
+    _fields_ = [("sx", _sz), ("sy", _sz), ("lenx", C.c_double), ("boundary", C.c_int), ("N", _sz),
+                ("n_wanted", _sz), ("ignore_lowest", _sz), ("halforder", C.c_int), ("B", C.c_double),
+                ("eps_values", _dp), ("n_eps", _sz), ("eps_divisor", C.c_double),
+                ("exhaust_eps", C.c_int), ("max_steps", C.c_int), ("min_time_step", C.c_double),
+                ("recover", C.c_int), ("seed", C.c_uint64), ("timestep_test", _Test),
+                ("final_test", _Test), ("precision", C.c_int)]
+
+
+def lib():
+    """Loads the gfx950 library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(library_path):
+            raise ImportError(f"{library_path} is missing: run python itp2d_amd/build.py "
+                              "(there is no CPU fallback)")
+        L = C.CDLL(library_path)
+        sig = {
+            "itp_ctx_create": [C.c_int, C.POINTER(_vp)],
+            "itp_ctx_destroy": [_vp],
+            "itp_last_error": [_vp],
+            "itp_ctx_synchronize": [_vp],
+            "itp_ctx_num_cus": [_vp],
+            "itp_ctx_enable_timing": [_vp, C.c_int],
+            "itp_ctx_phase_times": [_vp, _dp, C.POINTER(C.c_longlong)],
+            "itp_ctx_reset_phase_times": [_vp],
+            "itp_grid_create": [_vp, _sz, _sz, C.c_double, C.c_int, C.POINTER(_vp)],
+            "itp_grid_destroy": [_vp],
+            "itp_grid_kvalues": [_vp, _dp, _dp, _dp, _dp],
+            "itp_grid_positions": [_vp, _dp, _dp],
+            "itp_grid_normalization_factor": [_vp, C.c_int],
+            "itp_transform_host": [_vp, _dp, _sz, C.c_int],
+            "itp_transform_device": [_vp, _vp, _sz, C.c_int],
+            "itp_potential_create": [_vp, _dp, C.POINTER(_vp)],
+            "itp_potential_destroy": [_vp],
+            "itp_stateset_create": [_vp, _sz, C.c_int, C.POINTER(_vp)],
+            "itp_stateset_destroy": [_vp],
+            "itp_stateset_upload": [_vp, _dp],
+            "itp_stateset_download": [_vp, _dp],
+            "itp_stateset_device_ptr": [_vp],
+            "itp_stateset_size": [_vp],
+            "itp_split_create": [_vp, C.c_int, C.c_double, C.c_double, C.POINTER(_vp)],
+            "itp_split_destroy": [_vp],
+            "itp_split_set_time_step": [_vp, C.c_double],
+            "itp_split_halforder": [_vp],
+            "itp_propagate": [_vp, _vp],
+            "itp_orthonormalize": [_vp, C.POINTER(_sz), _dp],
+            "itp_overlap": [_vp, _dp],
+            "itp_hamiltonian_create": [_vp, C.c_double, C.POINTER(_vp)],
+            "itp_hamiltonian_destroy": [_vp],
+            "itp_energies": [_vp, _vp, _dp, _dp],
+            "itp_energies_complex": [_vp, _vp, _dp, _dp],
+            "itp_apply_expkinetic_host": [_vp, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _sz],
+            "itp_apply_exppotential_host": [_vp, C.c_double, C.c_double, C.c_double, _dp, _sz],
+            "itp_apply_secondorder_host": [_vp, C.c_double, C.c_double, C.c_double, C.c_int, _dp, _sz],
+            "itp_apply_hamiltonian_host": [_vp, _dp, _sz],
+            "itp_eigh_host": [_vp, _sz, _dp, _dp],
+            "itp_init_states": [C.POINTER(C.c_uint64), _sz, _sz, C.c_double, _sz, _dp],
+            "itps_create": [_vp, C.POINTER(_Params), _dp, _dp, C.POINTER(_vp)],
+            "itps_destroy": [_vp],
+            "itps_step": [_vp],
+            "itps_run": [_vp],
+            "itps_finished": [_vp],
+            "itps_error_flag": [_vp],
+            "itps_total_steps": [_vp],
+            "itps_eps": [_vp],
+            "itps_num_converged": [_vp],
+            "itps_num_timestep_converged": [_vp],
+            "itps_sorted": [_vp, _dp, _dp, C.POINTER(_sz)],
+            "itps_states": [_vp, _dp],
+            "itps_total_time": [_vp],
+            "itps_recoveries": [_vp],
+        }
+        for name, args in sig.items():
+            getattr(L, name).argtypes = args
+        L.itp_last_error.restype = C.c_char_p
+        L.itp_grid_normalization_factor.restype = C.c_double
+        L.itp_stateset_device_ptr.restype = _vp
+        L.itp_stateset_size.restype = _sz
+        L.itps_eps.restype = C.c_double
+        L.itps_total_time.restype = C.c_double
+        L.itps_num_converged.restype = _sz
+        L.itps_num_timestep_converged.restype = _sz
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def _c128(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+class Context:
+    """One device and one stream."""
+
+    def __init__(self, device=0):
+        self.h = _vp()
+        rc = lib().itp_ctx_create(device, C.byref(self.h))
+        if rc:
+            raise ITPError(rc, f"itp_ctx_create({device}) failed (no GPU?)")
+
+    def error(self):
+        return lib().itp_last_error(self.h).decode()
+
+    def check(self, rc, what="", bad=None):
+        if rc == ITP_OK:
+            return
+        msg = f"{what}: {self.error()}"
+        if rc == ITP_E_NONPOSITIVE_EIG:
+            raise NonPositiveEigenvalue(rc, msg, *(bad or (0, 0.0)))
+        if rc == ITP_E_NONNORMAL_EIG:
+            raise NonNormalEigenvalue(rc, msg, *(bad or (0, 0.0)))
+        if rc == ITP_E_EIGENSOLVER:
+            raise EigensolverError(rc, msg)
+        raise ITPError(rc, msg)
+
+    def synchronize(self):
+        self.check(lib().itp_ctx_synchronize(self.h), "synchronize")
+
+    @property
+    def num_cus(self):
+        return lib().itp_ctx_num_cus(self.h)
+
+    def enable_timing(self, on=True):
+        lib().itp_ctx_enable_timing(self.h, int(on))
+
+    def phase_times(self):
+        ms = (C.c_double * 5)()
+        cnt = (C.c_longlong * 5)()
+        lib().itp_ctx_phase_times(self.h, ms, cnt)
+        return {PHASES[i]: (ms[i], cnt[i]) for i in range(5)}
+
+    def reset_phase_times(self):
+        lib().itp_ctx_reset_phase_times(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.itp_ctx_destroy(self.h)
+            self.h = None
+
+
+class Grid:
+    """DataLayout + Transformer (datalayout.hpp:30-54, transformer.hpp:30-63)."""
+
+    def __init__(self, ctx, sx, sy, dx, boundary=PERIODIC):
+        self.ctx, self.sx, self.sy, self.dx, self.boundary = ctx, sx, sy, dx, boundary
+        self.h = _vp()
+        ctx.check(lib().itp_grid_create(ctx.h, sx, sy, dx, boundary, C.byref(self.h)), "itp_grid_create")
+
+    def kvalues(self):
+        a = [np.zeros(n) for n in (self.sx, self.sy, self.sx, self.sy)]
+        lib().itp_grid_kvalues(self.h, *[_p(x) for x in a])
+        return dict(zip(["fft_kx", "fft_ky", "dsct_kx", "dsct_ky"], a))
+
+    def positions(self):
+        px, py = np.zeros(self.sx), np.zeros(self.sy)
+        lib().itp_grid_positions(self.h, _p(px), _p(py))
+        return px, py
+
+    def normalization_factor(self, kind):
+        return lib().itp_grid_normalization_factor(self.h, TRANSFORMS.index(kind) if isinstance(kind, str) else kind)
+
+    def transform(self, fields, kind):
+        f = _c128(fields).copy()
+        squeeze = f.ndim == 2
+        if squeeze:
+            f = f[None]
+        k = TRANSFORMS.index(kind) if isinstance(kind, str) else kind
+        self.ctx.check(lib().itp_transform_host(self.h, _p(f.view(np.float64)), f.shape[0], k), "transform")
+        return f[0] if squeeze else f
+
+    def apply_expkinetic(self, fields, B, e, coeff=-1.0, prefactor=1.0):
+        f = _c128(fields).copy()
+        self.ctx.check(lib().itp_apply_expkinetic_host(self.h, B, e, coeff, prefactor, _p(f.view(np.float64)),
+                                                        f.shape[0]), "expkinetic")
+        return f
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.itp_grid_destroy(self.h)
+            self.h = None
+
+
+class Potential:
+    """Potential (potential.hpp:31-58); V=None is the null potential."""
+
+    def __init__(self, grid, V=None):
+        self.grid = grid
+        self.V = None if V is None else np.ascontiguousarray(V, dtype=np.float64)
+        self.h = _vp()
+        grid.ctx.check(lib().itp_potential_create(grid.h, _p(self.V), C.byref(self.h)), "itp_potential_create")
+
+    def apply_exppotential(self, fields, e, coeff=-0.5, prefactor=1.0):
+        f = _c128(fields).copy()
+        self.grid.ctx.check(lib().itp_apply_exppotential_host(self.h, e, coeff, prefactor,
+                                                               _p(f.view(np.float64)), f.shape[0]), "exppotential")
+        return f
+
+    def apply_secondorder(self, fields, B, e, prefactor=1.0, exponent=1):
+        f = _c128(fields).copy()
+        self.grid.ctx.check(lib().itp_apply_secondorder_host(self.h, B, e, prefactor, exponent,
+                                                              _p(f.view(np.float64)), f.shape[0]), "secondorder")
+        return f
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.itp_potential_destroy(self.h)
+            self.h = None
+
+
+class StateSet:
+    """StateSet (stateset.hpp): N fields on the device, FP64 or FP32."""
+
+    def __init__(self, grid, N, precision=FP64):
+        self.grid, self.N, self.precision = grid, N, precision
+        self.h = _vp()
+        grid.ctx.check(lib().itp_stateset_create(grid.h, N, precision, C.byref(self.h)), "itp_stateset_create")
+
+    def upload(self, states):
+        s = _c128(states)
+        assert s.shape == (self.N, self.grid.sy, self.grid.sx)
+        self.grid.ctx.check(lib().itp_stateset_upload(self.h, _p(s.view(np.float64))), "upload")
+
+    def download(self):
+        out = np.zeros((self.N, self.grid.sy, self.grid.sx), dtype=np.complex128)
+        self.grid.ctx.check(lib().itp_stateset_download(self.h, _p(out.view(np.float64))), "download")
+        return out
+
+    def orthonormalize(self):
+        n = _sz(0)
+        v = C.c_double(0)
+        rc = lib().itp_orthonormalize(self.h, C.byref(n), C.byref(v))
+        self.grid.ctx.check(rc, "orthonormalize", bad=(n.value, v.value))
+
+    def overlap(self):
+        out = np.zeros(self.N * self.N, dtype=np.complex128)
+        self.grid.ctx.check(lib().itp_overlap(self.h, _p(out.view(np.float64))), "overlap")
+        return out.reshape(self.N, self.N).T  # column-major -> S[i, j]
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.itp_stateset_destroy(self.h)
+            self.h = None
+
+
+class MultiProductSplit:
+    """The evolution operator T (multiproductsplit.cpp:23-58)."""
+
+    def __init__(self, potential, halforder, eps, B=0.0):
+        self.pot = potential
+        self.h = _vp()
+        potential.grid.ctx.check(lib().itp_split_create(potential.h, halforder, eps, B, C.byref(self.h)),
+                                 "itp_split_create")
+
+    @property
+    def halforder(self):
+        return lib().itp_split_halforder(self.h)
+
+    def set_time_step(self, eps):
+        self.pot.grid.ctx.check(lib().itp_split_set_time_step(self.h, eps), "set_time_step")
+
+    def propagate(self, states):
+        self.pot.grid.ctx.check(lib().itp_propagate(self.h, states.h), "propagate")
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.itp_split_destroy(self.h)
+            self.h = None
+
+
+class Hamiltonian:
+    """H = T + V (itpsystem.cpp:117-119)."""
+
+    def __init__(self, potential, B=0.0):
+        self.pot = potential
+        self.h = _vp()
+        potential.grid.ctx.check(lib().itp_hamiltonian_create(potential.h, B, C.byref(self.h)),
+                                 "itp_hamiltonian_create")
+
+    def energies(self, states):
+        E = np.zeros(states.N)
+        sd = np.zeros(states.N)
+        self.pot.grid.ctx.check(lib().itp_energies(self.h, states.h, _p(E), _p(sd)), "energies")
+        return E, sd
+
+    def energies_complex(self, states):
+        m = np.zeros(2 * states.N)
+        q = np.zeros(2 * states.N)
+        self.pot.grid.ctx.check(lib().itp_energies_complex(self.h, states.h, _p(m), _p(q)), "energies")
+        return m.view(np.complex128), q.view(np.complex128)
+
+    def apply(self, fields):
+        f = _c128(fields).copy()
+        self.pot.grid.ctx.check(lib().itp_apply_hamiltonian_host(self.h, _p(f.view(np.float64)), f.shape[0]),
+                                "hamiltonian")
+        return f
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.itp_hamiltonian_destroy(self.h)
+            self.h = None
+
+
+def eigh(ctx, A):
+    """EigenSolver::solve on the GPU: ascending w, eigenvectors in columns."""
+    A = np.asarray(A, dtype=np.complex128)
+    n = A.shape[0]
+    a = np.ascontiguousarray(A.T).ravel().copy()  # column-major
+    w = np.zeros(n)
+    ctx.check(lib().itp_eigh_host(ctx.h, n, _p(a.view(np.float64)), _p(w)), "eigh")
+    return w, np.ascontiguousarray(a.reshape(n, n).T)
+
+
+def init_states(seed, N, sx, sy, dx):
+    """Seeded uniform-noise initial states, each normalised (host-side setup)."""
+    st = C.c_uint64(seed)
+    out = np.zeros((N, sy, sx), dtype=np.complex128)
+    lib().itp_init_states(C.byref(st), sx, sy, dx, N, _p(out.view(np.float64)))
+    return out
+
+
+def _params(cfg):
+    p = _Params()
+    eps = np.ascontiguousarray(cfg.get("eps_values", []), dtype=np.float64)
+    for k in ["sx", "sy", "lenx", "boundary", "N", "n_wanted", "halforder", "B", "eps_divisor",
+              "max_steps", "min_time_step", "seed"]:
+        setattr(p, k, cfg[k])
+    p.ignore_lowest = cfg.get("ignore_lowest", 0)
+    p.exhaust_eps = int(cfg.get("exhaust_eps", 0))
+    p.recover = int(cfg.get("recover", 0))
+    p.precision = int(cfg.get("precision", FP64))
+    p.eps_values = _p(eps) if eps.size else None
+    p.n_eps = eps.size
+    for name in ["timestep_test", "final_test"]:
+        t = cfg[name]
+        setattr(p, name, _Test(TESTS[t[0]], t[1] if len(t) > 1 else 0.0, t[2] if len(t) > 2 else 0.0))
+    return p, eps
+
+
+class ITPSystem:
+    """The ITP driver (itpsystem.cpp) on the GPU engine."""
+
+    def __init__(self, ctx, cfg, V=None, init_states=None):
+        self.ctx, self.cfg = ctx, cfg
+        self._params, self._eps = _params(cfg)
+        self._V = None if V is None else np.ascontiguousarray(V, dtype=np.float64)
+        self._init = None if init_states is None else _c128(init_states)
+        self.h = _vp()
+        rc = lib().itps_create(ctx.h, C.byref(self._params), _p(self._V),
+                               None if self._init is None else _p(self._init.view(np.float64)), C.byref(self.h))
+        ctx.check(rc, "itps_create")
+
+    def step(self):
+        return lib().itps_step(self.h)
+
+    def run(self):
+        return lib().itps_run(self.h)
+
+    @property
+    def finished(self):
+        return bool(lib().itps_finished(self.h))
+
+    @property
+    def error_flag(self):
+        return bool(lib().itps_error_flag(self.h))
+
+    @property
+    def total_steps(self):
+        return lib().itps_total_steps(self.h)
+
+    @property
+    def eps(self):
+        return lib().itps_eps(self.h)
+
+    @property
+    def num_converged(self):
+        return lib().itps_num_converged(self.h)
+
+    @property
+    def total_time(self):
+        return lib().itps_total_time(self.h)
+
+    @property
+    def recoveries(self):
+        return lib().itps_recoveries(self.h)
+
+    def sorted(self):
+        N = self.cfg["N"]
+        E, sd = np.zeros(N), np.zeros(N)
+        idx = (_sz * N)()
+        lib().itps_sorted(self.h, _p(E), _p(sd), idx)
+        return E, sd, np.array(list(idx))
+
+    def states(self):
+        N, sy, sx = self.cfg["N"], self.cfg["sy"], self.cfg["sx"]
+        out = np.zeros((N, sy, sx), dtype=np.complex128)
+        self.ctx.check(lib().itps_states(self.h, _p(out.view(np.float64))), "itps_states")
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.itps_destroy(self.h)
+            self.h = None

@@ -1,0 +1,983 @@
+// C ABI of the gfx950 ITP engine (declared in include/itp2d.h).
+//
+// Objects own device memory; all kernels go to the context's one stream.
+// Table construction (host, FP64, the reference's formulas) and the sweep
+// sequences come from engine.hpp; kernels from kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/itp2d.h"
+#include "engine.hpp"
+#include "gpu.hpp"
+
+using namespace itp;
+
+struct itp_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int num_cus = 0;
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+  bool timing = false;
+  // event pairs recorded around phases without synchronising; resolved when
+  // the times are read or the pool is full
+  std::vector<hipEvent_t> evpool;
+  std::vector<int> pair_phase;
+  size_t ev_used = 0;
+  double ms[ITP_NUM_PHASES] = {};
+  long long cnt[ITP_NUM_PHASES] = {};
+};
+
+struct itp_grid {
+  itp_ctx* ctx = nullptr;
+  GridInfo g;
+  LineGeom lg[2];  // [0] FP64 fields, [1] FP32 fields
+  cplx<double>* d_tw = nullptr;
+};
+
+struct itp_potential {
+  itp_grid* grid = nullptr;
+  bool null = true;
+  std::vector<double> V;
+  double* dV = nullptr;
+};
+
+struct itp_stateset {
+  itp_grid* grid = nullptr;
+  size_t N = 0;
+  int prec = ITP_FP64;
+  size_t esz = 16;
+  void* buf[2] = {nullptr, nullptr};
+  int cur = 0;
+  // orthonormalisation workspace
+  int np = 0, nc = 0, nchunk = 1;
+  long long kchunk = 0;
+  cplx<double>* part = nullptr;
+  cplx<double>* A[2] = {nullptr, nullptr};
+  cplx<double>* V[2] = {nullptr, nullptr};
+  cplx<double>* Vinit = nullptr;
+  cplx<double>* C = nullptr;
+  double* lam = nullptr;
+  int* perm = nullptr;
+  double* info = nullptr;
+  int* bad = nullptr;
+  // energies
+  cplx<double>* epart = nullptr;  // [2][N][max(sx,sy)]
+  double* dE = nullptr;           // E[N], sigma[N]
+  cplx<double>* dmean = nullptr;  // mean[N], msq[N]
+};
+
+struct MemberTabs {
+  ExpKinTabs host;
+  std::vector<double> vh, vf, vp;
+  double* dev = nullptr;  // one allocation for all tables of the member
+  MemberDev d;
+};
+
+struct itp_split {
+  itp_potential* pot = nullptr;
+  int halforder = 1;
+  double B = 0, eps = 0;
+  std::vector<double> coefs;
+  std::vector<std::unique_ptr<MemberTabs>> members;
+};
+
+struct itp_hamiltonian {
+  itp_potential* pot = nullptr;
+  double B = 0;
+  KinTabs host;
+  double* dev = nullptr;
+  KinHDev d;
+};
+
+namespace {
+
+int set_err(itp_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCK(ctx, expr)                                                                   \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return set_err((ctx), ITP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class F>
+int guarded(itp_ctx* c, F f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    return set_err(c, ITP_E_NOMEM, "host allocation failed");
+  } catch (const std::exception& ex) {
+    return set_err(c, ITP_E_INVALID, ex.what());
+  }
+}
+
+int dalloc(itp_ctx* c, void** p, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return set_err(c, ITP_E_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+  }
+  return ITP_OK;
+}
+
+int ensure_ws(itp_ctx* c, size_t bytes) {
+  if (bytes <= c->ws_bytes) return ITP_OK;
+  if (c->ws) {
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    HIPCK(c, hipFree(c->ws));
+    c->ws = nullptr;
+    c->ws_bytes = 0;
+  }
+  int rc = dalloc(c, &c->ws, bytes);
+  if (rc) return rc;
+  c->ws_bytes = bytes;
+  return ITP_OK;
+}
+
+// Accumulate the recorded event pairs into the per-phase totals.
+void flush_events(itp_ctx* c) {
+  if (c->ev_used == 0) return;
+  (void)hipEventSynchronize(c->evpool[c->ev_used - 1]);
+  for (size_t p = 0; p + 1 < c->ev_used; p += 2) {
+    float t = 0;
+    if (hipEventElapsedTime(&t, c->evpool[p], c->evpool[p + 1]) == hipSuccess) {
+      c->ms[c->pair_phase[p / 2]] += t;
+      c->cnt[c->pair_phase[p / 2]]++;
+    }
+  }
+  c->ev_used = 0;
+  c->pair_phase.clear();
+}
+
+// HIP-event timing of one phase on the context's stream (only when enabled).
+struct Phase {
+  itp_ctx* c;
+  int ph;
+  bool on;
+  Phase(itp_ctx* c_, int ph_) : c(c_), ph(ph_), on(c_->timing) {
+    if (!on) return;
+    if (c->ev_used + 2 > c->evpool.size()) flush_events(c);
+    (void)hipEventRecord(c->evpool[c->ev_used++], c->stream);
+  }
+  void stop() {
+    if (!on) return;
+    (void)hipEventRecord(c->evpool[c->ev_used++], c->stream);
+    c->pair_phase.push_back(ph);
+    on = false;
+  }
+};
+
+LineGeom make_geom(const GridInfo& g, const cplx<double>* d_tw, size_t esz) {
+  LineGeom lg;
+  make_plan(g.sx, lg.plan_x);
+  make_plan(g.sy, lg.plan_y);
+  lg.tw_x = d_tw;
+  lg.mk_x = d_tw + g.sx;
+  lg.tw_y = d_tw + 2 * g.sx;
+  lg.mk_y = d_tw + 2 * g.sx + g.sy;
+  // lines per workgroup: the most that keep a sweep within 64 KiB of LDS
+  // (two workgroups per CU), at least 1, at most 8
+  auto pick = [&](int n, int lines) {
+    int best = 1;
+    for (int L = 8; L >= 1; L /= 2) {
+      if (lines % L) continue;
+      const size_t bytes = (size_t)3 * L * (n + 1) * esz + (size_t)2 * n * esz + 272 * 16;
+      if (bytes <= 64 * 1024) { best = L; break; }
+    }
+    return best;
+  };
+  lg.L_row = std::min(4, pick(g.sx, g.sy));
+  lg.L_col = std::min(4, pick(g.sy, g.sx));
+  return lg;
+}
+
+int upload_tables(itp_ctx* c, const std::vector<const std::vector<double>*>& tabs, double** dev,
+                  std::vector<const double*>& ptrs) {
+  size_t total = 0;
+  for (auto t : tabs) total += t->size();
+  int rc = dalloc(c, (void**)dev, total * sizeof(double));
+  if (rc) return rc;
+  size_t off = 0;
+  ptrs.clear();
+  for (auto t : tabs) {
+    ptrs.push_back(t->empty() ? nullptr : *dev + off);
+    if (!t->empty()) HIPCK(c, hipMemcpy(*dev + off, t->data(), t->size() * sizeof(double), hipMemcpyHostToDevice));
+    off += t->size();
+  }
+  return ITP_OK;
+}
+
+void free_member(MemberTabs* m) {
+  if (m && m->dev) { (void)hipFree(m->dev); m->dev = nullptr; }
+}
+
+// (Re)build the device tables of every MultiProductSplit member for time step eps
+// (multiproductsplit.cpp:23-47, secondordersplit.cpp:21-69).
+int build_split_tables(itp_split* sp) {
+  itp_ctx* c = sp->pot->grid->ctx;
+  const GridInfo& g = sp->pot->grid->g;
+  const bool has_v = !sp->pot->null;
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  for (auto& m : sp->members) free_member(m.get());
+  sp->members.clear();
+  for (int t = 1; t <= sp->halforder; t++) {
+    std::unique_ptr<MemberTabs> m(new MemberTabs);
+    const double h = sp->eps / t;
+    const double coef = sp->coefs[t - 1];
+    if (has_v) {
+      expkin_tables(g, sp->B, h, -1.0, 1.0, m->host);
+      exppot_values(g, sp->pot->V.data(), h, -0.5, 1.0, m->vh);
+      if (t >= 2) exppot_values(g, sp->pot->V.data(), h, -1.0, 1.0, m->vf);
+      exppot_values(g, sp->pot->V.data(), h, -0.5, coef, m->vp);
+    } else {
+      expkin_tables(g, sp->B, h, -1.0 * t, coef, m->host);
+    }
+    std::vector<const double*> p;
+    int rc = upload_tables(c, {&m->host.mult, &m->host.xm, &m->host.xm2, &m->host.ym, &m->vh, &m->vf, &m->vp},
+                           &m->dev, p);
+    if (rc) return rc;
+    m->d.kin = KinDev{p[0], p[1], p[2], p[3]};
+    m->d.vhalf = p[4];
+    m->d.vfull = p[5];
+    m->d.vpref = p[6];
+    sp->members.push_back(std::move(m));
+  }
+  return ITP_OK;
+}
+
+template <class R>
+int propagate_impl(itp_split* sp, itp_stateset* s) {
+  itp_grid* gr = s->grid;
+  itp_ctx* c = gr->ctx;
+  const GridInfo& g = gr->g;
+  const size_t fbytes = s->N * (size_t)g.M * sizeof(cplx<R>);
+  int rc = ensure_ws(c, fbytes);
+  if (rc) return rc;
+  const Branch br = branch_of(sp->B, g.bt);
+  const bool has_v = !sp->pot->null;
+  const cplx<R>* in = (const cplx<R>*)s->buf[s->cur];
+  cplx<R>* out = (cplx<R>*)s->buf[s->cur ^ 1];
+  cplx<R>* work = (cplx<R>*)c->ws;
+  std::vector<Sweep<R>> prog;
+  for (int t = 1; t <= sp->halforder; t++)
+    member_program<R>(br, sp->members[t - 1]->d, has_v ? t : 1, has_v, in, work, out, t > 1, prog);
+  Phase ph(c, ITP_PHASE_PROPAGATE);
+  GpuBackend be;
+  be.st = c->stream;
+  run_program<R>(be, g, gr->lg[sizeof(R) == 4], prog, (int)s->N);
+  if (be.err != hipSuccess) return set_err(c, ITP_E_HIP, std::string("propagate: ") + hipGetErrorString(be.err));
+  ph.stop();
+  s->cur ^= 1;
+  return ITP_OK;
+}
+
+template <class R>
+int energies_impl(itp_hamiltonian* h, itp_stateset* s, double* E, double* sigma, double* mean, double* msq) {
+  itp_grid* gr = s->grid;
+  itp_ctx* c = gr->ctx;
+  const GridInfo& g = gr->g;
+  const size_t fbytes = s->N * (size_t)g.M * sizeof(cplx<R>);
+  int rc = ensure_ws(c, 2 * fbytes);
+  if (rc) return rc;
+  const Branch br = branch_of(h->B, g.bt);
+  const double* V = h->pot->null ? nullptr : h->pot->dV;
+  const cplx<R>* psi = (const cplx<R>*)s->buf[s->cur];
+  cplx<R>* w1 = (cplx<R>*)c->ws;
+  cplx<R>* w2 = (cplx<R>*)((char*)c->ws + fbytes);
+  const size_t pstride = s->N * (size_t)std::max(g.sx, g.sy);
+  cplx<double>* p1 = s->epart;
+  cplx<double>* p2 = s->epart + pstride;
+  const LineGeom& lg = gr->lg[sizeof(R) == 4];
+  std::vector<Sweep<R>> prog;
+  Phase ph(c, ITP_PHASE_ENERGIES);
+  const int nb = hamiltonian_program<R>(g, lg, br, h->d, V, psi, w1, w2, psi, p1, prog);
+  hamiltonian_program<R>(g, lg, br, h->d, V, w2, w1, nullptr, psi, p2, prog);
+  GpuBackend be;
+  be.st = c->stream;
+  run_program<R>(be, g, lg, prog, (int)s->N);
+  if (be.err != hipSuccess) return set_err(c, ITP_E_HIP, std::string("energies: ") + hipGetErrorString(be.err));
+  HIPCK(c, launch_energy_reduce(p1, p2, nb, (int)s->N, g.dx, s->dE, s->dE + s->N, s->dmean, s->dmean + s->N,
+                                c->stream));
+  ph.stop();
+  if (E) HIPCK(c, hipMemcpyAsync(E, s->dE, s->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (sigma) HIPCK(c, hipMemcpyAsync(sigma, s->dE + s->N, s->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (mean) HIPCK(c, hipMemcpyAsync(mean, s->dmean, s->N * 16, hipMemcpyDeviceToHost, c->stream));
+  if (msq) HIPCK(c, hipMemcpyAsync(msq, s->dmean + s->N, s->N * 16, hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  return ITP_OK;
+}
+
+// Overlap + eigendecomposition + coefficients; returns after the host has
+// checked the eigenvalues (stateset.cpp:145-177).
+int eigen_stage(itp_stateset* s, size_t* bad_n, double* bad_lambda, bool scale) {
+  itp_ctx* c = s->grid->ctx;
+  const int N = (int)s->N;
+  Phase pe(c, ITP_PHASE_EIGENSOLVE);
+  HIPCK(c, hipMemcpyAsync(s->V[0], s->Vinit, (size_t)s->np * s->np * 16, hipMemcpyDeviceToDevice, c->stream));
+  HIPCK(c, launch_jacobi(s->A[0], s->A[1], s->V[0], s->V[1], s->np, 60, 1e-14, s->info, c->num_cus, c->stream));
+  HIPCK(c, launch_eig_finish(s->A[0], s->V[0], s->np, N, s->lam, s->perm, s->C, s->nc, scale ? 1 : 0, c->stream));
+  double info[4];
+  int bad = 0;
+  std::vector<double> lam(N);
+  HIPCK(c, hipMemcpyAsync(info, s->info, sizeof(info), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipMemcpyAsync(&bad, s->bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipMemcpyAsync(lam.data(), s->lam, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  pe.stop();
+  if (bad) return set_err(c, ITP_E_EIGENSOLVER, "overlap matrix has non-finite entries");
+  if (info[1] != 1.0)
+    return set_err(c, ITP_E_EIGENSOLVER, "Jacobi eigensolver did not converge in " + std::to_string((int)info[0]) + " sweeps");
+  for (int n = 0; n < N; n++) {
+    const double ev = lam[n];
+    if (ev <= 0) {
+      if (bad_n) *bad_n = n;
+      if (bad_lambda) *bad_lambda = ev;
+      return set_err(c, ITP_E_NONPOSITIVE_EIG, "non-positive overlap eigenvalue " + std::to_string(ev) +
+                                                   " at n = " + std::to_string(n));
+    }
+    if (std::fpclassify(ev) != FP_NORMAL) {
+      if (bad_n) *bad_n = n;
+      if (bad_lambda) *bad_lambda = ev;
+      return set_err(c, ITP_E_NONNORMAL_EIG, "non-normal overlap eigenvalue at n = " + std::to_string(n));
+    }
+  }
+  return ITP_OK;
+}
+
+template <class R>
+int gram_impl(itp_stateset* s) {
+  itp_ctx* c = s->grid->ctx;
+  const GridInfo& g = s->grid->g;
+  Phase po(c, ITP_PHASE_OVERLAP);
+  HIPCK(c, hipMemsetAsync(s->bad, 0, sizeof(int), c->stream));
+  HIPCK(c, launch_gram<R>((const cplx<R>*)s->buf[s->cur], g.M, (int)s->N, s->nc, s->part, s->nchunk, s->kchunk, g.dx,
+                          s->A[0], s->np, s->bad, c->stream));
+  po.stop();
+  return ITP_OK;
+}
+
+template <class R>
+int ortho_impl(itp_stateset* s, size_t* bad_n, double* bad_lambda) {
+  itp_ctx* c = s->grid->ctx;
+  const GridInfo& g = s->grid->g;
+  int rc = gram_impl<R>(s);
+  if (rc) return rc;
+  rc = eigen_stage(s, bad_n, bad_lambda, true);
+  if (rc) return rc;
+  Phase pl(c, ITP_PHASE_LINCOMB);
+  HIPCK(c, launch_rotate<R>((const cplx<R>*)s->buf[s->cur], (cplx<R>*)s->buf[s->cur ^ 1], s->C, s->nc, g.M,
+                            (int)s->N, c->stream));
+  pl.stop();
+  s->cur ^= 1;
+  return ITP_OK;
+}
+
+uint64_t splitmix64(uint64_t& st) {
+  uint64_t z = (st += 0x9E3779B97F4A7C15ULL);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+// temporary device copy of host fields for the single-operator hooks
+struct DevFields {
+  itp_ctx* c = nullptr;
+  cplx<double>* p = nullptr;
+  cplx<double>* w = nullptr;
+  ~DevFields() {
+    if (p) (void)hipFree(p);
+    if (w) (void)hipFree(w);
+  }
+};
+
+int hook_begin(itp_ctx* c, const double* host, size_t nelem, DevFields& df, bool need_work) {
+  df.c = c;
+  int rc = dalloc(c, (void**)&df.p, nelem * 16);
+  if (rc) return rc;
+  if (need_work) {
+    rc = dalloc(c, (void**)&df.w, nelem * 16);
+    if (rc) return rc;
+  }
+  HIPCK(c, hipMemcpy(df.p, host, nelem * 16, hipMemcpyHostToDevice));
+  return ITP_OK;
+}
+
+int hook_end(itp_ctx* c, double* host, size_t nelem, DevFields& df, hipError_t err) {
+  if (err != hipSuccess) return set_err(c, ITP_E_HIP, hipGetErrorString(err));
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  HIPCK(c, hipMemcpy(host, df.p, nelem * 16, hipMemcpyDeviceToHost));
+  return ITP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int itp_ctx_create(int device, itp_ctx** out) {
+  if (!out) return ITP_E_INVALID;
+  *out = nullptr;
+  std::unique_ptr<itp_ctx> c(new itp_ctx);
+  c->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) return ITP_E_HIP;
+  if (hipStreamCreate(&c->stream) != hipSuccess) return ITP_E_HIP;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return ITP_E_HIP;
+  c->num_cus = prop.multiProcessorCount;
+  c->evpool.resize(2048);
+  for (auto& e : c->evpool)
+    if (hipEventCreate(&e) != hipSuccess) return ITP_E_HIP;
+  *out = c.release();
+  return ITP_OK;
+}
+
+void itp_ctx_destroy(itp_ctx* c) {
+  if (!c) return;
+  (void)hipStreamSynchronize(c->stream);
+  if (c->ws) (void)hipFree(c->ws);
+  for (auto& e : c->evpool)
+    if (e) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* itp_last_error(const itp_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+int itp_ctx_synchronize(itp_ctx* c) {
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  return ITP_OK;
+}
+
+int itp_ctx_num_cus(const itp_ctx* c) { return c ? c->num_cus : 0; }
+
+int itp_ctx_enable_timing(itp_ctx* c, int enable) {
+  if (!c) return ITP_E_INVALID;
+  c->timing = enable != 0;
+  return ITP_OK;
+}
+int itp_ctx_phase_times(itp_ctx* c, double* ms, long long* counts) {
+  if (!c) return ITP_E_INVALID;
+  flush_events(c);
+  for (int i = 0; i < ITP_NUM_PHASES; i++) {
+    if (ms) ms[i] = c->ms[i];
+    if (counts) counts[i] = c->cnt[i];
+  }
+  return ITP_OK;
+}
+int itp_ctx_reset_phase_times(itp_ctx* c) {
+  if (!c) return ITP_E_INVALID;
+  flush_events(c);
+  for (int i = 0; i < ITP_NUM_PHASES; i++) { c->ms[i] = 0; c->cnt[i] = 0; }
+  return ITP_OK;
+}
+
+int itp_grid_create(itp_ctx* c, size_t sx, size_t sy, double dx, int boundary, itp_grid** out) {
+  if (!c || !out || sx < 2 || sy < 2 || !(dx > 0) || (boundary != ITP_PERIODIC && boundary != ITP_DIRICHLET))
+    return set_err(c, ITP_E_INVALID, "itp_grid_create: bad arguments");
+  if (sx > 4096 || sy > 4096) return set_err(c, ITP_E_UNSUPPORTED, "grid sides above 4096 are not supported");
+  return guarded(c, [&]() -> int {
+    std::unique_ptr<itp_grid> gr(new itp_grid);
+    gr->ctx = c;
+    gr->g = make_grid((int)sx, (int)sy, dx, boundary);
+    LinePlan px, py;
+    if (!make_plan((int)sx, px) || !make_plan((int)sy, py))
+      return set_err(c, ITP_E_UNSUPPORTED, "grid sides must be 2^a 3^b 5^c with an even radix-stage count");
+    std::vector<cplx<double>> twx, mkx, twy, mky;
+    twiddle_tables((int)sx, twx, mkx);
+    twiddle_tables((int)sy, twy, mky);
+    std::vector<cplx<double>> all;
+    all.insert(all.end(), twx.begin(), twx.end());
+    all.insert(all.end(), mkx.begin(), mkx.end());
+    all.insert(all.end(), twy.begin(), twy.end());
+    all.insert(all.end(), mky.begin(), mky.end());
+    int rc = dalloc(c, (void**)&gr->d_tw, all.size() * 16);
+    if (rc) return rc;
+    HIPCK(c, hipMemcpy(gr->d_tw, all.data(), all.size() * 16, hipMemcpyHostToDevice));
+    gr->lg[0] = make_geom(gr->g, gr->d_tw, 16);
+    gr->lg[1] = make_geom(gr->g, gr->d_tw, 8);
+    *out = gr.release();
+    return ITP_OK;
+  });
+}
+
+void itp_grid_destroy(itp_grid* g) {
+  if (!g) return;
+  (void)hipStreamSynchronize(g->ctx->stream);
+  if (g->d_tw) (void)hipFree(g->d_tw);
+  delete g;
+}
+
+int itp_grid_kvalues(const itp_grid* g, double* fkx, double* fky, double* dkx, double* dky) {
+  if (!g) return ITP_E_INVALID;
+  if (fkx) std::copy(g->g.fft_kx.begin(), g->g.fft_kx.end(), fkx);
+  if (fky) std::copy(g->g.fft_ky.begin(), g->g.fft_ky.end(), fky);
+  if (dkx) std::copy(g->g.dsct_kx.begin(), g->g.dsct_kx.end(), dkx);
+  if (dky) std::copy(g->g.dsct_ky.begin(), g->g.dsct_ky.end(), dky);
+  return ITP_OK;
+}
+
+int itp_grid_positions(const itp_grid* g, double* posx, double* posy) {
+  if (!g) return ITP_E_INVALID;
+  if (posx) std::copy(g->g.posx.begin(), g->g.posx.end(), posx);
+  if (posy) std::copy(g->g.posy.begin(), g->g.posy.end(), posy);
+  return ITP_OK;
+}
+
+double itp_grid_normalization_factor(const itp_grid* g, int kind) {
+  if (!g || kind < 0 || kind > ITP_iDCTy) return NAN;
+  const GridInfo& gi = g->g;
+  const int k = kind / 2;  // pairs (FFT,iFFT), (FFTx,iFFTx), ...
+  switch (k) {
+    case 0: return gi.nFFT;
+    case 1: return gi.nFFTx;
+    case 2: return gi.nFFTy;
+    case 3: case 6: return gi.nDSCT;
+    case 4: case 7: return gi.nDSCTx;
+    default: return gi.nDSCTy;
+  }
+}
+
+int itp_transform_device(itp_grid* g, void* fields, size_t nfields, int kind) {
+  if (!g || !fields) return ITP_E_INVALID;
+  itp_ctx* c = g->ctx;
+  return guarded(c, [&]() -> int {
+    std::vector<Sweep<double>> prog;
+    if (!transform_program<double>(kind, (cplx<double>*)fields, prog)) return set_err(c, ITP_E_INVALID, "transform kind");
+    GpuBackend be;
+    be.st = c->stream;
+    run_program<double>(be, g->g, g->lg[0], prog, (int)nfields);
+    if (be.err != hipSuccess) return set_err(c, ITP_E_HIP, hipGetErrorString(be.err));
+    return ITP_OK;
+  });
+}
+
+int itp_transform_host(itp_grid* g, double* fields, size_t nfields, int kind) {
+  if (!g || !fields) return ITP_E_INVALID;
+  itp_ctx* c = g->ctx;
+  DevFields df;
+  const size_t n = nfields * (size_t)g->g.M;
+  int rc = hook_begin(c, fields, n, df, false);
+  if (rc) return rc;
+  rc = itp_transform_device(g, df.p, nfields, kind);
+  if (rc) return rc;
+  return hook_end(c, fields, n, df, hipSuccess);
+}
+
+int itp_potential_create(itp_grid* g, const double* V, itp_potential** out) {
+  if (!g || !out) return ITP_E_INVALID;
+  itp_ctx* c = g->ctx;
+  return guarded(c, [&]() -> int {
+    std::unique_ptr<itp_potential> p(new itp_potential);
+    p->grid = g;
+    p->null = (V == nullptr);
+    if (V) {
+      p->V.assign(V, V + g->g.M);
+      int rc = dalloc(c, (void**)&p->dV, g->g.M * sizeof(double));
+      if (rc) return rc;
+      HIPCK(c, hipMemcpy(p->dV, V, g->g.M * sizeof(double), hipMemcpyHostToDevice));
+    }
+    *out = p.release();
+    return ITP_OK;
+  });
+}
+
+void itp_potential_destroy(itp_potential* p) {
+  if (!p) return;
+  (void)hipStreamSynchronize(p->grid->ctx->stream);
+  if (p->dV) (void)hipFree(p->dV);
+  delete p;
+}
+
+int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out) {
+  if (!g || !out || N < 1 || (precision != ITP_FP64 && precision != ITP_FP32)) return ITP_E_INVALID;
+  itp_ctx* c = g->ctx;
+  if (N > 4096) return set_err(c, ITP_E_UNSUPPORTED, "more than 4096 states per set is not supported");
+  if (N > 65535) return set_err(c, ITP_E_UNSUPPORTED, "too many states");
+  return guarded(c, [&]() -> int {
+    std::unique_ptr<itp_stateset> s(new itp_stateset);
+    s->grid = g;
+    s->N = N;
+    s->prec = precision;
+    s->esz = precision == ITP_FP64 ? 16 : 8;
+    const size_t M = g->g.M;
+    int rc;
+    for (int b = 0; b < 2; b++) {
+      if ((rc = dalloc(c, &s->buf[b], N * M * s->esz))) return rc;
+      HIPCK(c, hipMemset(s->buf[b], 0, N * M * s->esz));
+    }
+    s->np = (int)(N + (N & 1));
+    s->nc = (int)((N + 63) / 64 * 64);
+    const int nb = s->nc / 64;
+    const int npairs = nb * (nb + 1) / 2;
+    long long nchunk = (2048 + npairs - 1) / npairs;
+    nchunk = std::min<long long>(nchunk, std::max<long long>(1, (long long)M / 256));
+    const size_t per_chunk = (size_t)s->nc * s->nc * 16;
+    nchunk = std::max<long long>(1, std::min<long long>(nchunk, (long long)((512ull << 20) / per_chunk)));
+    s->kchunk = ((long long)M + nchunk - 1) / nchunk;
+    s->kchunk = (s->kchunk + 15) / 16 * 16;
+    s->nchunk = (int)(((long long)M + s->kchunk - 1) / s->kchunk);
+    const size_t np2 = (size_t)s->np * s->np * 16;
+    if ((rc = dalloc(c, (void**)&s->part, (size_t)s->nchunk * per_chunk))) return rc;
+    for (int b = 0; b < 2; b++) {
+      if ((rc = dalloc(c, (void**)&s->A[b], np2))) return rc;
+      if ((rc = dalloc(c, (void**)&s->V[b], np2))) return rc;
+      HIPCK(c, hipMemset(s->A[b], 0, np2));
+    }
+    if ((rc = dalloc(c, (void**)&s->Vinit, np2))) return rc;
+    {
+      std::vector<cplx<double>> I((size_t)s->np * s->np, mk<double>(0.0, 0.0));
+      for (int i = 0; i < s->np; i++) I[(size_t)i * s->np + i] = mk<double>(1.0, 0.0);
+      HIPCK(c, hipMemcpy(s->Vinit, I.data(), np2, hipMemcpyHostToDevice));
+    }
+    if ((rc = dalloc(c, (void**)&s->C, (size_t)s->nc * s->nc * 16))) return rc;
+    HIPCK(c, hipMemset(s->C, 0, (size_t)s->nc * s->nc * 16));
+    if ((rc = dalloc(c, (void**)&s->lam, N * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, (void**)&s->perm, N * sizeof(int)))) return rc;
+    if ((rc = dalloc(c, (void**)&s->info, 8 * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, (void**)&s->bad, sizeof(int)))) return rc;
+    const size_t pstride = N * (size_t)std::max(g->g.sx, g->g.sy);
+    if ((rc = dalloc(c, (void**)&s->epart, 2 * pstride * 16))) return rc;
+    if ((rc = dalloc(c, (void**)&s->dE, 2 * N * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, (void**)&s->dmean, 2 * N * 16))) return rc;
+    *out = s.release();
+    return ITP_OK;
+  });
+}
+
+void itp_stateset_destroy(itp_stateset* s) {
+  if (!s) return;
+  (void)hipStreamSynchronize(s->grid->ctx->stream);
+  void* ptrs[] = {s->buf[0], s->buf[1], s->part, s->A[0], s->A[1], s->V[0], s->V[1], s->Vinit, s->C,
+                  s->lam, s->perm, s->info, s->bad, s->epart, s->dE, s->dmean};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete s;
+}
+
+int itp_stateset_upload(itp_stateset* s, const double* host) {
+  if (!s || !host) return ITP_E_INVALID;
+  itp_ctx* c = s->grid->ctx;
+  const size_t n = s->N * (size_t)s->grid->g.M;
+  if (s->prec == ITP_FP64) {
+    HIPCK(c, hipMemcpyAsync(s->buf[s->cur], host, n * 16, hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    return ITP_OK;
+  }
+  // stage through the other buffer's space is too small (FP32); use the workspace
+  int rc = ensure_ws(c, n * 16);
+  if (rc) return rc;
+  HIPCK(c, hipMemcpyAsync(c->ws, host, n * 16, hipMemcpyHostToDevice, c->stream));
+  HIPCK(c, launch_convert(c->ws, s->buf[s->cur], (long long)n, 1, c->stream));
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  return ITP_OK;
+}
+
+int itp_stateset_download(itp_stateset* s, double* host) {
+  if (!s || !host) return ITP_E_INVALID;
+  itp_ctx* c = s->grid->ctx;
+  const size_t n = s->N * (size_t)s->grid->g.M;
+  if (s->prec == ITP_FP64) {
+    HIPCK(c, hipMemcpyAsync(host, s->buf[s->cur], n * 16, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    return ITP_OK;
+  }
+  int rc = ensure_ws(c, n * 16);
+  if (rc) return rc;
+  HIPCK(c, launch_convert(s->buf[s->cur], c->ws, (long long)n, 0, c->stream));
+  HIPCK(c, hipMemcpyAsync(host, c->ws, n * 16, hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  return ITP_OK;
+}
+
+void* itp_stateset_device_ptr(itp_stateset* s) { return s ? s->buf[s->cur] : nullptr; }
+size_t itp_stateset_size(const itp_stateset* s) { return s ? s->N : 0; }
+
+int itp_split_create(itp_potential* pot, int halforder, double eps, double B, itp_split** out) {
+  if (!pot || !out || halforder < 1 || halforder > 12 || !(eps > 0) || !std::isfinite(B)) return ITP_E_INVALID;
+  itp_ctx* c = pot->grid->ctx;
+  return guarded(c, [&]() -> int {
+    std::unique_ptr<itp_split> sp(new itp_split);
+    sp->pot = pot;
+    sp->halforder = pot->null ? 1 : halforder;  // multiproductsplit.cpp:24
+    sp->B = B;
+    sp->eps = eps;
+    sp->coefs = mps_coefficients(sp->halforder);
+    int rc = build_split_tables(sp.get());
+    if (rc) {
+      for (auto& m : sp->members) free_member(m.get());
+      return rc;
+    }
+    *out = sp.release();
+    return ITP_OK;
+  });
+}
+
+void itp_split_destroy(itp_split* sp) {
+  if (!sp) return;
+  (void)hipStreamSynchronize(sp->pot->grid->ctx->stream);
+  for (auto& m : sp->members) free_member(m.get());
+  delete sp;
+}
+
+int itp_split_set_time_step(itp_split* sp, double eps) {
+  if (!sp || !(eps > 0)) return ITP_E_INVALID;
+  itp_ctx* c = sp->pot->grid->ctx;
+  return guarded(c, [&]() -> int {
+    sp->eps = eps;
+    return build_split_tables(sp);
+  });
+}
+
+int itp_split_halforder(const itp_split* sp) { return sp ? sp->halforder : 0; }
+
+int itp_propagate(itp_split* sp, itp_stateset* s) {
+  if (!sp || !s || sp->pot->grid != s->grid) return ITP_E_INVALID;
+  itp_ctx* c = s->grid->ctx;
+  return guarded(c, [&]() -> int { return s->prec == ITP_FP64 ? propagate_impl<double>(sp, s) : propagate_impl<float>(sp, s); });
+}
+
+int itp_orthonormalize(itp_stateset* s, size_t* bad_n, double* bad_lambda) {
+  if (!s) return ITP_E_INVALID;
+  itp_ctx* c = s->grid->ctx;
+  return guarded(c, [&]() -> int {
+    return s->prec == ITP_FP64 ? ortho_impl<double>(s, bad_n, bad_lambda) : ortho_impl<float>(s, bad_n, bad_lambda);
+  });
+}
+
+int itp_overlap(itp_stateset* s, double* out) {
+  if (!s || !out) return ITP_E_INVALID;
+  itp_ctx* c = s->grid->ctx;
+  return guarded(c, [&]() -> int {
+    int rc = s->prec == ITP_FP64 ? gram_impl<double>(s) : gram_impl<float>(s);
+    if (rc) return rc;
+    std::vector<cplx<double>> S((size_t)s->np * s->np);
+    HIPCK(c, hipMemcpyAsync(S.data(), s->A[0], S.size() * 16, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    const size_t N = s->N;
+    cplx<double>* o = (cplx<double>*)out;
+    for (size_t i = 0; i < N; i++)
+      for (size_t j = 0; j < N; j++) o[j * N + i] = S[i * s->np + j];  // column-major
+    return ITP_OK;
+  });
+}
+
+int itp_hamiltonian_create(itp_potential* pot, double B, itp_hamiltonian** out) {
+  if (!pot || !out || !std::isfinite(B)) return ITP_E_INVALID;
+  itp_ctx* c = pot->grid->ctx;
+  return guarded(c, [&]() -> int {
+    std::unique_ptr<itp_hamiltonian> h(new itp_hamiltonian);
+    h->pot = pot;
+    h->B = B;
+    kin_tables(pot->grid->g, B, h->host);
+    std::vector<const double*> p;
+    int rc = upload_tables(c, {&h->host.mxy, &h->host.mx, &h->host.mx2, &h->host.my}, &h->dev, p);
+    if (rc) return rc;
+    h->d = KinHDev{p[0], p[1], p[2], p[3]};
+    *out = h.release();
+    return ITP_OK;
+  });
+}
+
+void itp_hamiltonian_destroy(itp_hamiltonian* h) {
+  if (!h) return;
+  (void)hipStreamSynchronize(h->pot->grid->ctx->stream);
+  if (h->dev) (void)hipFree(h->dev);
+  delete h;
+}
+
+int itp_energies(itp_hamiltonian* h, itp_stateset* s, double* E, double* sigma) {
+  if (!h || !s || h->pot->grid != s->grid) return ITP_E_INVALID;
+  itp_ctx* c = s->grid->ctx;
+  return guarded(c, [&]() -> int {
+    return s->prec == ITP_FP64 ? energies_impl<double>(h, s, E, sigma, nullptr, nullptr)
+                               : energies_impl<float>(h, s, E, sigma, nullptr, nullptr);
+  });
+}
+
+int itp_energies_complex(itp_hamiltonian* h, itp_stateset* s, double* mean, double* meansq) {
+  if (!h || !s || h->pot->grid != s->grid) return ITP_E_INVALID;
+  itp_ctx* c = s->grid->ctx;
+  return guarded(c, [&]() -> int {
+    return s->prec == ITP_FP64 ? energies_impl<double>(h, s, nullptr, nullptr, mean, meansq)
+                               : energies_impl<float>(h, s, nullptr, nullptr, mean, meansq);
+  });
+}
+
+int itp_apply_expkinetic_host(itp_grid* g, double B, double e, double coeff, double prefactor, double* fields,
+                              size_t n) {
+  if (!g || !fields) return ITP_E_INVALID;
+  itp_ctx* c = g->ctx;
+  return guarded(c, [&]() -> int {
+    ExpKinTabs t;
+    expkin_tables(g->g, B, e, coeff, prefactor, t);
+    double* dev = nullptr;
+    std::vector<const double*> p;
+    int rc = upload_tables(c, {&t.mult, &t.xm, &t.xm2, &t.ym}, &dev, p);
+    if (rc) return rc;
+    std::unique_ptr<double, void (*)(double*)> hold(dev, [](double* q) { (void)hipFree(q); });
+    DevFields df;
+    const size_t nel = n * (size_t)g->g.M;
+    if ((rc = hook_begin(c, fields, nel, df, true))) return rc;
+    std::vector<Sweep<double>> prog;
+    expkinetic_program<double>(branch_of(B, g->g.bt), KinDev{p[0], p[1], p[2], p[3]}, df.p, df.w, prog);
+    GpuBackend be;
+    be.st = c->stream;
+    run_program<double>(be, g->g, g->lg[0], prog, (int)n);
+    return hook_end(c, fields, nel, df, be.err);
+  });
+}
+
+int itp_apply_exppotential_host(itp_potential* pot, double e, double coeff, double prefactor, double* fields,
+                                size_t n) {
+  if (!pot || !fields) return ITP_E_INVALID;
+  itp_ctx* c = pot->grid->ctx;
+  itp_grid* g = pot->grid;
+  return guarded(c, [&]() -> int {
+    std::vector<double> vals;
+    double* dev = nullptr;
+    std::vector<const double*> p;
+    if (!pot->null) exppot_values(g->g, pot->V.data(), e, coeff, prefactor, vals);
+    int rc = upload_tables(c, {&vals}, &dev, p);
+    if (rc) return rc;
+    std::unique_ptr<double, void (*)(double*)> hold(dev, [](double* q) { (void)hipFree(q); });
+    DevFields df;
+    const size_t nel = n * (size_t)g->g.M;
+    if ((rc = hook_begin(c, fields, nel, df, false))) return rc;
+    std::vector<Sweep<double>> prog;
+    exppotential_program<double>(pot->null ? nullptr : p[0], prefactor, df.p, prog);
+    GpuBackend be;
+    be.st = c->stream;
+    run_program<double>(be, g->g, g->lg[0], prog, (int)n);
+    return hook_end(c, fields, nel, df, be.err);
+  });
+}
+
+int itp_apply_secondorder_host(itp_potential* pot, double B, double e, double prefactor, int exponent,
+                               double* fields, size_t n) {
+  if (!pot || !fields || exponent < 1) return ITP_E_INVALID;
+  itp_ctx* c = pot->grid->ctx;
+  itp_grid* g = pot->grid;
+  return guarded(c, [&]() -> int {
+    MemberTabs m;
+    const bool has_v = !pot->null;
+    if (has_v) {
+      expkin_tables(g->g, B, e, -1.0, 1.0, m.host);
+      exppot_values(g->g, pot->V.data(), e, -0.5, 1.0, m.vh);
+      if (exponent >= 2) exppot_values(g->g, pot->V.data(), e, -1.0, 1.0, m.vf);
+      exppot_values(g->g, pot->V.data(), e, -0.5, prefactor, m.vp);
+    } else {
+      expkin_tables(g->g, B, e, -1.0 * exponent, prefactor, m.host);
+    }
+    std::vector<const double*> p;
+    int rc = upload_tables(c, {&m.host.mult, &m.host.xm, &m.host.xm2, &m.host.ym, &m.vh, &m.vf, &m.vp}, &m.dev, p);
+    if (rc) return rc;
+    std::unique_ptr<double, void (*)(double*)> hold(m.dev, [](double* q) { (void)hipFree(q); });
+    m.d.kin = KinDev{p[0], p[1], p[2], p[3]};
+    m.d.vhalf = p[4];
+    m.d.vfull = p[5];
+    m.d.vpref = p[6];
+    DevFields df;
+    const size_t nel = n * (size_t)g->g.M;
+    if ((rc = hook_begin(c, fields, nel, df, true))) return rc;
+    cplx<double>* out = nullptr;
+    if ((rc = dalloc(c, (void**)&out, nel * 16))) return rc;
+    std::unique_ptr<cplx<double>, void (*)(cplx<double>*)> hold2(out, [](cplx<double>* q) { (void)hipFree(q); });
+    std::vector<Sweep<double>> prog;
+    member_program<double>(branch_of(B, g->g.bt), m.d, has_v ? exponent : 1, has_v, df.p, df.w, out, false, prog);
+    GpuBackend be;
+    be.st = c->stream;
+    run_program<double>(be, g->g, g->lg[0], prog, (int)n);
+    if (be.err != hipSuccess) return set_err(c, ITP_E_HIP, hipGetErrorString(be.err));
+    HIPCK(c, hipMemcpyAsync(df.p, out, nel * 16, hipMemcpyDeviceToDevice, c->stream));
+    return hook_end(c, fields, nel, df, hipSuccess);
+  });
+}
+
+int itp_apply_hamiltonian_host(itp_hamiltonian* h, double* fields, size_t n) {
+  if (!h || !fields) return ITP_E_INVALID;
+  itp_grid* g = h->pot->grid;
+  itp_ctx* c = g->ctx;
+  return guarded(c, [&]() -> int {
+    DevFields df;
+    const size_t nel = n * (size_t)g->g.M;
+    int rc = hook_begin(c, fields, nel, df, true);
+    if (rc) return rc;
+    cplx<double>* out = nullptr;
+    if ((rc = dalloc(c, (void**)&out, nel * 16))) return rc;
+    std::unique_ptr<cplx<double>, void (*)(cplx<double>*)> hold(out, [](cplx<double>* q) { (void)hipFree(q); });
+    std::vector<Sweep<double>> prog;
+    hamiltonian_program<double>(g->g, g->lg[0], branch_of(h->B, g->g.bt), h->d, h->pot->null ? nullptr : h->pot->dV,
+                                df.p, df.w, out, nullptr, nullptr, prog);
+    GpuBackend be;
+    be.st = c->stream;
+    run_program<double>(be, g->g, g->lg[0], prog, (int)n);
+    if (be.err != hipSuccess) return set_err(c, ITP_E_HIP, hipGetErrorString(be.err));
+    HIPCK(c, hipMemcpyAsync(df.p, out, nel * 16, hipMemcpyDeviceToDevice, c->stream));
+    return hook_end(c, fields, nel, df, hipSuccess);
+  });
+}
+
+int itp_eigh_host(itp_ctx* c, size_t n, double* A, double* w) {
+  if (!c || !A || !w || n < 1 || n > 4096) return ITP_E_INVALID;
+  return guarded(c, [&]() -> int {
+    // a throwaway state set of the right size carries the eigensolver workspace
+    itp_grid* g = nullptr;
+    int rc = itp_grid_create(c, 8, 8, 1.0, ITP_PERIODIC, &g);
+    if (rc) return rc;
+    std::unique_ptr<itp_grid, void (*)(itp_grid*)> hg(g, itp_grid_destroy);
+    itp_stateset* s = nullptr;
+    if ((rc = itp_stateset_create(g, n, ITP_FP64, &s))) return rc;
+    std::unique_ptr<itp_stateset, void (*)(itp_stateset*)> hs(s, itp_stateset_destroy);
+    const cplx<double>* a = (const cplx<double>*)A;
+    std::vector<cplx<double>> full((size_t)s->np * s->np, mk<double>(0.0, 0.0));
+    for (size_t i = 0; i < n; i++)
+      for (size_t j = 0; j < n; j++)
+        full[i * s->np + j] = (i <= j) ? a[j * n + i] : conj(a[i * n + j]);  // upper triangle, column-major
+    for (size_t i = 0; i < n; i++) full[i * s->np + i].y = 0;
+    HIPCK(c, hipMemcpy(s->A[0], full.data(), full.size() * 16, hipMemcpyHostToDevice));
+    HIPCK(c, hipMemset(s->bad, 0, sizeof(int)));
+    rc = eigen_stage(s, nullptr, nullptr, false);
+    if (rc && rc != ITP_E_NONPOSITIVE_EIG && rc != ITP_E_NONNORMAL_EIG) return rc;
+    std::vector<cplx<double>> C((size_t)s->nc * s->nc);
+    HIPCK(c, hipMemcpy(C.data(), s->C, C.size() * 16, hipMemcpyDeviceToHost));
+    HIPCK(c, hipMemcpy(w, s->lam, n * sizeof(double), hipMemcpyDeviceToHost));
+    cplx<double>* o = (cplx<double>*)A;
+    for (size_t r = 0; r < n; r++)
+      for (size_t cc = 0; cc < n; cc++) o[r * n + cc] = C[cc * s->nc + r];  // column r = eigenvector r
+    c->err.clear();
+    return ITP_OK;
+  });
+}
+
+int itp_init_states(uint64_t* rng_state, size_t sx, size_t sy, double dx, size_t N, double* out) {
+  if (!rng_state || !out) return ITP_E_INVALID;
+  const size_t M = sx * sy;
+  for (size_t n = 0; n < N; n++) {
+    double* s = out + 2 * n * M;
+    for (size_t i = 0; i < M; i++) {
+      s[2 * i] = 2 * (double(splitmix64(*rng_state) >> 11) * (1.0 / 9007199254740992.0)) - 1;
+      s[2 * i + 1] = 2 * (double(splitmix64(*rng_state) >> 11) * (1.0 / 9007199254740992.0)) - 1;
+    }
+    double acc = 0;
+    for (size_t i = 0; i < M; i++) acc += s[2 * i] * s[2 * i] + s[2 * i + 1] * s[2 * i + 1];
+    const double nrm = std::sqrt(acc) * dx;
+    const double f = 1.0 / nrm;
+    for (size_t i = 0; i < 2 * M; i++) s[i] *= f;
+  }
+  return ITP_OK;
+}
+
+}  // extern "C"

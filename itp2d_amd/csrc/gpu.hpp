@@ -1,0 +1,35 @@
+// Launchers of the gfx950 kernels (kernels.hip), called by the C ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "line_ops.hpp"
+
+namespace itp {
+
+template <class R>
+hipError_t launch_sweep(const SweepArgs<R>& a, int nblk, int nfields, hipStream_t st);
+hipError_t launch_energy_reduce(const cplx<double>* p1, const cplx<double>* p2, int nblk, int N, double dx,
+                                double* E, double* sigma, cplx<double>* mean, cplx<double>* msq, hipStream_t st);
+hipError_t launch_convert(const void* in, void* out, long long n, int to_f32, hipStream_t st);
+template <class R>
+hipError_t launch_gram(const cplx<R>* psi, long long M, int N, int nc, cplx<double>* part, int nchunk,
+                       long long kchunk, double dx, cplx<double>* S, int np, int* bad, hipStream_t st);
+size_t jacobi_lds_bytes(int np);
+hipError_t launch_jacobi(cplx<double>* A0, cplx<double>* A1, cplx<double>* V0, cplx<double>* V1, int np,
+                         int max_sweeps, double tol, double* info, int num_cus, hipStream_t st);
+hipError_t launch_eig_finish(const cplx<double>* A, const cplx<double>* V, int np, int N, double* lam_sorted,
+                             int* perm, cplx<double>* C, int nc, int scale, hipStream_t st);
+template <class R>
+hipError_t launch_rotate(const cplx<R>* in, cplx<R>* out, const cplx<double>* C, int nc, long long M, int N,
+                         hipStream_t st);
+
+// Runs engine sweeps on the GPU (the Backend of run_program).
+struct GpuBackend {
+  hipStream_t st = nullptr;
+  hipError_t err = hipSuccess;
+  template <class R>
+  void sweep(const SweepArgs<R>& a, int nblk, int nfields) {
+    if (err == hipSuccess) err = launch_sweep<R>(a, nblk, nfields, st);
+  }
+};
+
+}  // namespace itp

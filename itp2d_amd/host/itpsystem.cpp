@@ -1,0 +1,296 @@
+// ITPSystem: the imaginary-time-propagation loop (itpsystem.cpp:23-589 of
+// the reference) driving the gfx950 engine through the C++ facade, which in
+// turn only uses the C ABI. Convergence tests follow convergence.cpp:75-200.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <tuple>
+#include <vector>
+
+#include "../../include/itp2d_system.h"
+#include "itp2d.hpp"
+
+namespace itp2d {
+
+class ITPSystem {
+ public:
+  typedef std::tuple<double, double, size_t> Esn_tuple;
+
+  ITPSystem(itp_ctx* ctx, const itps_params& p, const double* V, const double* init)
+      : params(p), ctx_(ctx) {
+    if (p.n_eps) eps_list.assign(p.eps_values, p.eps_values + p.n_eps);
+    params.eps_values = nullptr;
+    if (params.n_wanted + params.ignore_lowest > params.N) throw GeneralError("more wanted states than states");
+    tr_.reset(new Transformer(*ctx_holder(), p.sx, p.sy, p.lenx / double(p.sx), BoundaryType(p.boundary)));
+    pot_.reset(new Potential(*tr_, V));
+    states_.reset(new StateSet(*tr_, p.N, Precision(p.precision)));
+    H_.reset(new Hamiltonian(*pot_, p.B));
+    exhausting = p.exhaust_eps != 0;
+    for (double e : eps_list) eps_values.push_back(e);
+    if (!eps_values.empty()) { eps = eps_values.front(); eps_values.pop_front(); }
+    else eps = 0.5;  // Parameters::default_initial_eps (parameters.cpp:49)
+    T_.reset(new MultiProductSplit(p.halforder, *pot_, eps, p.B));
+    rng = p.seed;
+    host_states.resize(p.N * p.sx * p.sy);
+    if (init) std::memcpy(host_states.data(), init, host_states.size() * sizeof(comp));
+    else init_noise();
+    states_->upload(host_states.data());
+    ts_conv.assign(p.N, 0);
+    fin_conv.assign(p.N, 0);
+  }
+
+  // ITPSystem::step (itpsystem.cpp:407-470)
+  void step() {
+    if (total_step_counter == 0) t0 = std::chrono::steady_clock::now();
+    if (total_step_counter >= params.max_steps) { error_flag = true; finish(); return; }
+    step_counter++;
+    total_step_counter++;
+    T_->operate(*states_);          // propagate (itpsystem.cpp:243-256)
+    orthonormalize();               // itpsystem.cpp:258-289
+    calculate_energies();           // itpsystem.cpp:473-511
+    check_timestep_convergence();
+    if (all_ts) {
+      check_final_convergence();
+      if (all_final) { finish(); return; }
+      change_time_step();
+    } else if (exhausting) {
+      change_time_step();
+    }
+  }
+
+  void finish() {
+    if (finished) return;
+    (void)itp_ctx_synchronize(ctx_);
+    total_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    finished = true;
+  }
+
+  void sorted(double* E, double* sd, size_t* idx) const {
+    for (size_t n = 0; n < esn.size(); n++) {
+      if (E) E[n] = std::get<0>(esn[n]);
+      if (sd) sd[n] = std::get<1>(esn[n]);
+      if (idx) idx[n] = std::get<2>(esn[n]);
+    }
+  }
+  void download(comp* out) { states_->download(out); }
+
+  itps_params params;
+  std::vector<double> eps_list;
+  double eps = 0;
+  bool finished = false, error_flag = false;
+  int total_step_counter = 0, step_counter = 0, recoveries = 0;
+  size_t n_ts = 0, n_fin = 0;
+  double total_time = 0;
+
+ private:
+  // the context belongs to the caller; the facade borrows it
+  Context* ctx_holder() {
+    if (!ctxw_) ctxw_.reset(new Context(ctx_, Context::Borrowed()));
+    return ctxw_.get();
+  }
+
+  void init_noise() {  // StateSet::init_to_gaussian_noise with uniform noise (stateset.cpp:124-131)
+    uint64_t st = rng;
+    itp_init_states(&st, params.sx, params.sy, params.lenx / double(params.sx), params.N,
+                    reinterpret_cast<double*>(host_states.data()));
+    rng = st;
+  }
+
+  void set_ts(size_t n, bool v) {
+    if (bool(ts_conv[n]) == v) return;
+    ts_conv[n] = v;
+    if (v) n_ts++; else n_ts--;
+  }
+  void set_fin(size_t n, bool v) {
+    if (bool(fin_conv[n]) == v) return;
+    fin_conv[n] = v;
+    if (v) n_fin++; else n_fin--;
+  }
+
+  void change_time_step() {  // itpsystem.cpp:202-233
+    if (!eps_values.empty()) {
+      eps = eps_values.front();
+      eps_values.pop_front();
+      if (eps_values.empty()) exhausting = false;
+    } else {
+      eps /= params.eps_divisor;
+    }
+    if (eps < params.min_time_step) { error_flag = true; finish(); return; }
+    T_->set_time_step(eps);
+    step_counter = 0;
+    all_ts = false;
+    for (size_t n = 0; n < params.N; n++) set_ts(n, false);
+  }
+
+  void orthonormalize() {
+    try {
+      states_->orthonormalize();
+    } catch (NonPositiveEigenvalue&) {
+      if (params.recover) {
+        recoveries++;
+        change_time_step();
+        init_noise();
+        states_->upload(host_states.data());
+      } else {
+        error_flag = true;
+        finish();
+      }
+    } catch (std::exception&) {
+      error_flag = true;
+      finish();
+    }
+  }
+
+  void calculate_energies() {
+    if (total_step_counter == 0) return;
+    std::vector<double> E, sd;
+    H_->mean_and_standard_deviation(*states_, E, sd);
+    esn.resize(params.N);
+    for (size_t n = 0; n < params.N; n++) esn[n] = Esn_tuple(E[n], sd[n], n);
+    std::sort(esn.begin(), esn.end());
+    if (energies.size() < size_t(total_step_counter)) energies.resize(total_step_counter);
+    if (stdevs.size() < size_t(total_step_counter)) stdevs.resize(total_step_counter);
+    auto& ne = energies[total_step_counter - 1];
+    auto& nd = stdevs[total_step_counter - 1];
+    ne.clear();
+    nd.clear();
+    for (auto& t : esn) { ne.push_back(std::get<0>(t)); nd.push_back(std::get<1>(t)); }
+  }
+
+  // ConvergenceTest::test for sorted position n (convergence.cpp:75-200)
+  bool test(const itps_test& t, size_t n) const {
+    switch (t.kind) {
+      case ITPS_TEST_NONE: return false;
+      case ITPS_TEST_ONESTEP: return step_counter == 1 && ts_conv[std::get<2>(esn[n])];
+      case ITPS_TEST_ABSCHANGE:
+      case ITPS_TEST_RELCHANGE: {
+        if (energies.size() < 2) return false;
+        const double a = energies[energies.size() - 1][n], b = energies[energies.size() - 2][n];
+        if (t.kind == ITPS_TEST_RELCHANGE) return std::fabs((a - b) / b) < t.p0;
+        return std::fabs(a - b) < t.p0;
+      }
+      case ITPS_TEST_ABSSTDEV: {
+        if (stdevs.size() < 2) return false;
+        const size_t last = stdevs.size() - 1;
+        const double a = stdevs[last][n], b = stdevs[last - 1][n];
+        return (a < t.p0) || (std::fabs(a - b) < t.p1);
+      }
+      case ITPS_TEST_RELSTDEV: {
+        if (energies.size() < 2) return false;
+        const size_t last = stdevs.size() - 1;
+        const double a = stdevs[last][n] / energies[last][n];
+        const double b = stdevs[last - 1][n] / energies[last - 1][n];
+        return (a < t.p0) || (std::fabs(a - b) < t.p1);
+      }
+      default: return false;
+    }
+  }
+
+  void check_timestep_convergence() {  // itpsystem.cpp:291-332
+    for (size_t n = 0; n < params.N; n++)
+      if (test(params.timestep_test, n)) set_ts(std::get<2>(esn[n]), true);
+    bool flag = true;
+    for (size_t n = params.ignore_lowest; n < params.ignore_lowest + params.n_wanted; n++)
+      if (!ts_conv[std::get<2>(esn[n])]) flag = false;
+    all_ts = flag;
+  }
+
+  void check_final_convergence() {  // itpsystem.cpp:334-372
+    for (size_t n = 0; n < params.N; n++) set_fin(std::get<2>(esn[n]), test(params.final_test, n));
+    bool flag = true;
+    for (size_t n = params.ignore_lowest; n < params.ignore_lowest + params.n_wanted; n++)
+      if (!fin_conv[std::get<2>(esn[n])]) flag = false;
+    all_final = flag;
+  }
+
+  itp_ctx* ctx_;
+  std::unique_ptr<Context> ctxw_;
+  std::unique_ptr<Transformer> tr_;
+  std::unique_ptr<Potential> pot_;
+  std::unique_ptr<StateSet> states_;
+  std::unique_ptr<Hamiltonian> H_;
+  std::unique_ptr<MultiProductSplit> T_;
+  std::vector<comp> host_states;
+  uint64_t rng = 0;
+  std::deque<double> eps_values;
+  bool exhausting = false;
+  bool all_ts = false, all_final = false;
+  std::vector<Esn_tuple> esn;
+  std::vector<std::vector<double>> energies, stdevs;
+  std::vector<char> ts_conv, fin_conv;
+  std::chrono::steady_clock::time_point t0;
+};
+
+}  // namespace itp2d
+
+struct itps_system {
+  std::unique_ptr<itp2d::ITPSystem> sys;
+  itp_ctx* ctx;
+};
+
+extern "C" {
+
+int itps_create(itp_ctx* ctx, const itps_params* p, const double* V, const double* init, itps_system** out) {
+  if (!ctx || !p || !out) return ITP_E_INVALID;
+  try {
+    std::unique_ptr<itps_system> s(new itps_system);
+    s->ctx = ctx;
+    s->sys.reset(new itp2d::ITPSystem(ctx, *p, V, init));
+    *out = s.release();
+    return ITP_OK;
+  } catch (const std::exception&) {
+    return ITP_E_INVALID;
+  }
+}
+
+void itps_destroy(itps_system* s) { delete s; }
+
+int itps_step(itps_system* s) {
+  if (!s) return ITP_E_INVALID;
+  try {
+    s->sys->step();
+    return ITP_OK;
+  } catch (const std::exception&) {
+    s->sys->error_flag = true;
+    s->sys->finish();
+    return ITP_E_HIP;
+  }
+}
+
+int itps_run(itps_system* s) {
+  if (!s) return ITP_E_INVALID;
+  int rc = ITP_OK;
+  while (!s->sys->finished) {
+    const int r = itps_step(s);
+    if (r) rc = r;
+  }
+  return rc;
+}
+
+int itps_finished(const itps_system* s) { return s && s->sys->finished; }
+int itps_error_flag(const itps_system* s) { return s && s->sys->error_flag; }
+int itps_total_steps(const itps_system* s) { return s ? s->sys->total_step_counter : 0; }
+double itps_eps(const itps_system* s) { return s ? s->sys->eps : 0; }
+size_t itps_num_converged(const itps_system* s) { return s ? s->sys->n_fin : 0; }
+size_t itps_num_timestep_converged(const itps_system* s) { return s ? s->sys->n_ts : 0; }
+int itps_sorted(const itps_system* s, double* E, double* sigma, size_t* index) {
+  if (!s) return ITP_E_INVALID;
+  s->sys->sorted(E, sigma, index);
+  return ITP_OK;
+}
+int itps_states(itps_system* s, double* out) {
+  if (!s || !out) return ITP_E_INVALID;
+  try {
+    s->sys->download(reinterpret_cast<itp2d::comp*>(out));
+    return ITP_OK;
+  } catch (const std::exception&) {
+    return ITP_E_HIP;
+  }
+}
+double itps_total_time(const itps_system* s) { return s ? s->sys->total_time : 0; }
+int itps_recoveries(const itps_system* s) { return s ? s->sys->recoveries : 0; }
+
+}  // extern "C"

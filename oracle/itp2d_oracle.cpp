@@ -611,7 +611,7 @@ int jacobi_eigh(size_t n, std::vector<cd>& a /* row-major full */, std::vector<d
   const double tol = 1e-14;
   bool converged = false;
   for (int sweep = 0; sweep < 60 && !converged; sweep++) {
-    double maxoff = 0;
+    double maxoff = 0, maxb = 0, maxd = 0;
     for (size_t round = 0; round + 1 < np; round++) {
       for (size_t k = 0; k < m; k++) {
         size_t p = pos[2 * k], q = pos[2 * k + 1];
@@ -623,6 +623,8 @@ int jacobi_eigh(size_t n, std::vector<cd>& a /* row-major full */, std::vector<d
         const double scale = std::sqrt(std::fabs(app * aqq));
         const double off = scale > 0 ? ab / scale : ab;
         maxoff = std::max(maxoff, off);
+        maxb = std::max(maxb, ab);
+        maxd = std::max(maxd, std::max(std::fabs(app), std::fabs(aqq)));
         if (ab == 0 || off < 1e-17) { cs[k] = 1; sj[k] = 0; continue; }
         const double tau = (aqq - app) / (2 * ab);
         const double t = (tau >= 0 ? 1.0 : -1.0) / (std::fabs(tau) + std::hypot(1.0, tau));
@@ -672,7 +674,9 @@ int jacobi_eigh(size_t n, std::vector<cd>& a /* row-major full */, std::vector<d
       }
       pos.swap(np_pos);
     }
-    if (maxoff < tol) converged = true;
+    // relative off-diagonal below tol, or all off-diagonals at the rounding
+    // level of the largest diagonal element
+    if (maxoff < tol || maxb <= 1e-15 * maxd) converged = true;
   }
   if (!converged) return ORC_E_EIGENSOLVER;
   // drop the padding player (its row/column stayed zero and untouched)

@@ -1,0 +1,195 @@
+// CPU emulation of the gfx950 sweep kernels, for tests only.
+//
+// Runs the product's own line-operation passes (itp2d_amd/csrc/line_ops.hpp)
+// and sweep sequencing (itp2d_amd/csrc/engine.hpp) on the host, one emulated
+// workgroup at a time, passes in order, threads 0..255 in turn. This checks
+// the kernels' indexing and algebra against the oracle without a GPU.
+#include <vector>
+#include <memory>
+#include <omp.h>
+#include "../../itp2d_amd/csrc/engine.hpp"
+
+using namespace itp;
+
+namespace {
+struct CpuExec {
+  int nthr = 256;
+  template <class F> void run(F&& f) { for (int t = 0; t < nthr; t++) f(t, nthr); }
+};
+
+struct CpuBackend {
+  template <class R>
+  void sweep(const SweepArgs<R>& a, int nblk, int nfields) {
+    const size_t bytes = sweep_lds_bytes<R>(a.L, a.plan.n, 256);
+#pragma omp parallel
+    {
+      std::vector<char> lds(bytes);
+#pragma omp for collapse(2) schedule(static)
+      for (int f = 0; f < nfields; f++)
+        for (int b = 0; b < nblk; b++) {
+          SweepLds<R> s = carve_lds<R>(lds.data(), a.L, a.plan.n, 256);
+          CpuExec ex;
+          sweep_program<R>(ex, a, s, f, b);
+        }
+    }
+  }
+};
+
+struct Env {
+  GridInfo g;
+  LineGeom lg;
+  std::vector<cplx<double>> twx, mkx, twy, mky;
+  Env(int sx, int sy, double dx, int bt) {
+    g = make_grid(sx, sy, dx, bt);
+    if (!make_plan(sx, lg.plan_x) || !make_plan(sy, lg.plan_y)) throw std::runtime_error("plan");
+    twiddle_tables(sx, twx, mkx);
+    twiddle_tables(sy, twy, mky);
+    lg.tw_x = twx.data(); lg.mk_x = mkx.data(); lg.tw_y = twy.data(); lg.mk_y = mky.data();
+  }
+};
+
+template <class F> int guard(F f) { try { return f(); } catch (const std::exception&) { return 1; } }
+
+// FP64 or FP32 run of a field program: convert in/out for FP32.
+template <class R>
+std::vector<cplx<R>> to_r(const double* d, size_t n) {
+  std::vector<cplx<R>> v(n);
+  for (size_t i = 0; i < n; i++) v[i] = mk<R>(R(d[2 * i]), R(d[2 * i + 1]));
+  return v;
+}
+template <class R>
+void from_r(const std::vector<cplx<R>>& v, double* d) {
+  for (size_t i = 0; i < v.size(); i++) { d[2 * i] = double(v[i].x); d[2 * i + 1] = double(v[i].y); }
+}
+}  // namespace
+
+extern "C" {
+
+int emu_plan(int n, int* radix, int* nst) {
+  LinePlan p;
+  if (!make_plan(n, p)) return 1;
+  *nst = p.nst;
+  for (int i = 0; i < p.nst; i++) radix[i] = p.radix[i];
+  return 0;
+}
+
+int emu_transform(int sx, int sy, double dx, int bt, double* data, int nfields, int kind, int fp32) {
+  return guard([&] {
+    Env e(sx, sy, dx, bt);
+    CpuBackend be;
+    auto go = [&](auto tag) {
+      using R = decltype(tag);
+      auto f = to_r<R>(data, (size_t)nfields * e.g.M);
+      std::vector<Sweep<R>> prog;
+      if (!transform_program<R>(kind, f.data(), prog)) throw std::runtime_error("kind");
+      run_program<R>(be, e.g, e.lg, prog, nfields);
+      from_r<R>(f, data);
+    };
+    if (fp32) go(float()); else go(double());
+    return 0;
+  });
+}
+
+int emu_expkinetic(int sx, int sy, double dx, int bt, double B, double eps, double coeff, double pref,
+                   double* data, int nfields) {
+  return guard([&] {
+    Env e(sx, sy, dx, bt);
+    CpuBackend be;
+    ExpKinTabs t;
+    expkin_tables(e.g, B, eps, coeff, pref, t);
+    KinDev k{t.mult.data(), t.xm.data(), t.xm2.data(), t.ym.data()};
+    auto f = to_r<double>(data, (size_t)nfields * e.g.M);
+    std::vector<cplx<double>> w(f.size());
+    std::vector<Sweep<double>> prog;
+    expkinetic_program<double>(branch_of(B, bt), k, f.data(), w.data(), prog);
+    run_program<double>(be, e.g, e.lg, prog, nfields);
+    from_r<double>(f, data);
+    return 0;
+  });
+}
+
+// One SecondOrderSplit member (secondordersplit.cpp:21-49), or the whole
+// MultiProductSplit when halforder > 0 (exponent ignored).
+int emu_split(int sx, int sy, double dx, int bt, double B, const double* V, double eps, double prefactor,
+              int exponent, int halforder, double* data, int nfields, int fp32) {
+  return guard([&] {
+    Env e(sx, sy, dx, bt);
+    CpuBackend be;
+    const Branch br = branch_of(B, bt);
+    const bool has_v = V != nullptr;
+    struct MemberHost { ExpKinTabs k; std::vector<double> vh, vf, vp; MemberDev d; };
+    std::vector<std::unique_ptr<MemberHost>> mem;
+    std::vector<double> coefs;
+    std::vector<int> exps;
+    std::vector<double> steps;
+    if (halforder > 0) {
+      const int h = has_v ? halforder : 1;
+      coefs = mps_coefficients(h);
+      for (int t = 1; t <= h; t++) { exps.push_back(t); steps.push_back(eps / t); }
+    } else {
+      coefs.push_back(prefactor); exps.push_back(exponent); steps.push_back(eps);
+    }
+    for (size_t i = 0; i < exps.size(); i++) {
+      std::unique_ptr<MemberHost> m(new MemberHost);
+      const double h = steps[i];
+      if (has_v) {
+        expkin_tables(e.g, B, h, -1.0, 1.0, m->k);
+        exppot_values(e.g, V, h, -0.5, 1.0, m->vh);
+        exppot_values(e.g, V, h, -1.0, 1.0, m->vf);
+        exppot_values(e.g, V, h, -0.5, coefs[i], m->vp);
+      } else {
+        expkin_tables(e.g, B, h, -1.0 * exps[i], coefs[i], m->k);
+      }
+      m->d.kin = KinDev{m->k.mult.data(), m->k.xm.data(), m->k.xm2.data(), m->k.ym.data()};
+      m->d.vhalf = m->vh.data(); m->d.vfull = m->vf.data(); m->d.vpref = m->vp.data();
+      mem.push_back(std::move(m));
+    }
+    auto go = [&](auto tag) {
+      using R = decltype(tag);
+      auto in = to_r<R>(data, (size_t)nfields * e.g.M);
+      std::vector<cplx<R>> w(in.size()), out(in.size());
+      std::vector<Sweep<R>> prog;
+      for (size_t i = 0; i < mem.size(); i++)
+        member_program<R>(br, mem[i]->d, has_v ? exps[i] : 1, has_v, in.data(), w.data(), out.data(), i > 0, prog);
+      run_program<R>(be, e.g, e.lg, prog, nfields);
+      from_r<R>(out, data);
+    };
+    if (fp32) go(float()); else go(double());
+    return 0;
+  });
+}
+
+// H applied once (out) and the energies of the fields (E, sigma by field).
+int emu_hamiltonian(int sx, int sy, double dx, int bt, double B, const double* V, double* data, int nfields,
+                    double* E, double* sigma, int fp32) {
+  return guard([&] {
+    Env e(sx, sy, dx, bt);
+    CpuBackend be;
+    KinTabs t;
+    kin_tables(e.g, B, t);
+    KinHDev k{t.mxy.data(), t.mx.data(), t.mx2.data(), t.my.data()};
+    const Branch br = branch_of(B, bt);
+    auto go = [&](auto tag) {
+      using R = decltype(tag);
+      auto psi = to_r<R>(data, (size_t)nfields * e.g.M);
+      std::vector<cplx<R>> w1(psi.size()), h1(psi.size());
+      const int nbx = e.g.sy, nby = e.g.sx;
+      std::vector<cplx<double>> p1((size_t)nfields * std::max(nbx, nby)), p2(p1.size());
+      std::vector<Sweep<R>> prog;
+      const int nb = hamiltonian_program<R>(e.g, e.lg, br, k, V, psi.data(), w1.data(), h1.data(), psi.data(),
+                                            p1.data(), prog);
+      run_program<R>(be, e.g, e.lg, prog, nfields);
+      prog.clear();
+      hamiltonian_program<R>(e.g, e.lg, br, k, V, h1.data(), w1.data(), nullptr, psi.data(), p2.data(), prog);
+      run_program<R>(be, e.g, e.lg, prog, nfields);
+      for (int n = 0; n < nfields; n++)
+        energy_from_partials(p1.data() + (size_t)n * nb, p2.data() + (size_t)n * nb, nb, nb, e.g.dx, E + n,
+                             sigma + n, nullptr, nullptr);
+      from_r<R>(h1, data);
+    };
+    if (fp32) go(float()); else go(double());
+    return 0;
+  });
+}
+
+}  // extern "C"

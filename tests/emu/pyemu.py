@@ -1,0 +1,73 @@
+"""ctypes wrapper for the CPU emulation of the sweep kernels (tests only)."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "build", "libemu.so")
+_lib = None
+_dp = C.POINTER(C.c_double)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+        _lib = C.CDLL(LIB)
+        _lib.emu_transform.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, _dp, C.c_int, C.c_int, C.c_int]
+        _lib.emu_expkinetic.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double,
+                                        C.c_double, C.c_double, _dp, C.c_int]
+        _lib.emu_split.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, _dp, C.c_double,
+                                   C.c_double, C.c_int, C.c_int, _dp, C.c_int, C.c_int]
+        _lib.emu_hamiltonian.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, _dp, _dp,
+                                         C.c_int, _dp, _dp, C.c_int]
+        _lib.emu_plan.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def plan(n):
+    r = (C.c_int * 8)()
+    nst = C.c_int(0)
+    if lib().emu_plan(n, r, C.byref(nst)):
+        return None
+    return list(r)[: nst.value]
+
+
+def transform(fields, dx, bt, kind, fp32=False):
+    f = np.ascontiguousarray(fields, dtype=np.complex128).copy()
+    n, sy, sx = f.shape
+    assert lib().emu_transform(sx, sy, dx, bt, _p(f.view(np.float64)), n, kind, int(fp32)) == 0
+    return f
+
+
+def expkinetic(fields, dx, bt, B, e, coeff=-1.0, pref=1.0):
+    f = np.ascontiguousarray(fields, dtype=np.complex128).copy()
+    n, sy, sx = f.shape
+    assert lib().emu_expkinetic(sx, sy, dx, bt, B, e, coeff, pref, _p(f.view(np.float64)), n) == 0
+    return f
+
+
+def split(fields, dx, bt, B, V, eps, prefactor=1.0, exponent=1, halforder=0, fp32=False):
+    f = np.ascontiguousarray(fields, dtype=np.complex128).copy()
+    n, sy, sx = f.shape
+    Vc = None if V is None else np.ascontiguousarray(V, dtype=np.float64)
+    assert lib().emu_split(sx, sy, dx, bt, B, _p(Vc), eps, prefactor, exponent, halforder,
+                           _p(f.view(np.float64)), n, int(fp32)) == 0
+    return f
+
+
+def hamiltonian(fields, dx, bt, B, V, fp32=False):
+    f = np.ascontiguousarray(fields, dtype=np.complex128).copy()
+    n, sy, sx = f.shape
+    Vc = None if V is None else np.ascontiguousarray(V, dtype=np.float64)
+    E = np.zeros(n)
+    sd = np.zeros(n)
+    assert lib().emu_hamiltonian(sx, sy, dx, bt, B, _p(Vc), _p(f.view(np.float64)), n, _p(E), _p(sd),
+                                 int(fp32)) == 0
+    return f, E, sd

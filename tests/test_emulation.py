@@ -1,0 +1,78 @@
+"""The gfx950 sweep kernels' own pass code (itp2d_amd/csrc/line_ops.hpp) and
+sweep sequencing (engine.hpp), run on the CPU one emulated workgroup at a
+time, against the oracle. Checks indexing, Makhoul reorderings, the magnetic
+Dirichlet split and the epilogues without a GPU."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+from itp2d_amd import workloads as W
+from tests.emu import pyemu as E
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+def test_radix_plans_even():
+    for n in [8, 16, 40, 64, 128, 192, 256, 512, 1024]:
+        p = E.plan(n)
+        assert p is not None and len(p) % 2 == 0 and int(np.prod(p)) == n and max(p) <= 8
+
+
+@pytest.mark.parametrize("sx,sy", [(16, 16), (64, 64), (40, 40), (192, 64), (64, 128)])
+def test_transforms(sx, sy):
+    rng = np.random.default_rng(sx * sy)
+    f = rng.standard_normal((2, sy, sx)) + 1j * rng.standard_normal((2, sy, sx))
+    for k, name in enumerate(O.TRANSFORMS):
+        a = E.transform(f, 0.1, 0, k)
+        b = np.stack([O.transform(x, name) for x in f])
+        assert rel(a, b) < 1e-14, name
+
+
+def _setup(sx, sy, L=12.0):
+    dx = L / sx
+    V = W.harmonic(sx, sy, dx) + 0.3 * np.exp(-((W.positions(sx, sy, dx)[0][None, :] - 1) ** 2))
+    rng = np.random.default_rng(sx + 7 * sy)
+    f = rng.standard_normal((3, sy, sx)) + 1j * rng.standard_normal((3, sy, sx))
+    f /= np.sqrt((np.abs(f) ** 2).sum(axis=(1, 2), keepdims=True)) * dx
+    return dx, V, f
+
+
+BRANCHES = [(0, 0.0), (1, 0.0), (0, 1.3), (1, 1.3)]
+
+
+@pytest.mark.parametrize("bt,B", BRANCHES)
+@pytest.mark.parametrize("sx,sy", [(64, 64), (40, 40), (64, 128)])
+def test_operators(bt, B, sx, sy):
+    dx, V, f = _setup(sx, sy)
+    assert rel(E.expkinetic(f, dx, bt, B, 0.07, -1.0, 0.9), O.apply_expkinetic(f, dx, bt, B, 0.07, -1.0, 0.9)) < 1e-14
+    for ex in (1, 2, 3):
+        a = E.split(f, dx, bt, B, V, 0.05, 0.7, ex)
+        b = O.apply_secondorder(f, dx, bt, B, V, 0.05, 0.7, ex)
+        assert rel(a, b) < 1e-14
+    assert rel(E.split(f, dx, bt, B, None, 0.05, 1.0, 1), O.apply_secondorder(f, dx, bt, B, None, 0.05, 1.0, 1)) < 1e-14
+    for h in (1, 2, 3):
+        assert rel(E.split(f, dx, bt, B, V, 0.05, halforder=h), O.propagate(f, dx, bt, B, V, h, 0.05)) < 1e-14
+
+
+@pytest.mark.parametrize("bt,B", BRANCHES)
+def test_hamiltonian_and_energies(bt, B):
+    dx, V, f = _setup(64, 64)
+    hf, Ee, se = E.hamiltonian(f, dx, bt, B, V)
+    assert rel(hf, O.apply_hamiltonian(f, dx, bt, B, V)) < 1e-14
+    Eo, so, _, _ = O.energies(f, dx, bt, B, V)
+    assert np.abs(Ee - Eo).max() < 1e-12 * np.abs(Eo).max()
+    assert np.abs(se - so).max() < 1e-10 * np.abs(so).max()
+    _, En, _ = E.hamiltonian(f, dx, bt, B, None)
+    En_o, _, _, _ = O.energies(f, dx, bt, B, None)
+    assert np.abs(En - En_o).max() < 1e-12 * np.abs(En_o).max()
+
+
+@pytest.mark.parametrize("bt,B", BRANCHES)
+def test_fp32_propagation_tolerance(bt, B):
+    """complex-FP32 fields: one MPS step within FP32 rounding of the FP64 result."""
+    dx, V, f = _setup(64, 64)
+    a = E.split(f, dx, bt, B, V, 0.05, halforder=2, fp32=True)
+    b = O.propagate(f, dx, bt, B, V, 2, 0.05)
+    assert rel(a, b) < 2e-6

@@ -1,0 +1,240 @@
+"""Parity of the gfx950 path (called through the C ABI) with the FP64 oracle.
+
+Tolerances: FP64 fields agree with the oracle to rounding (relative 1e-13 for
+single operators, 1e-10 for converged eigenvalues, as the plain FP64 GPU path
+requires); complex-FP32 fields within 2e-6 relative for one propagation.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import itp2d_amd as I
+from itp2d_amd import workloads as W
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+def rand_states(n, sx, sy, dx, seed):
+    rng = np.random.default_rng(seed)
+    f = rng.standard_normal((n, sy, sx)) + 1j * rng.standard_normal((n, sy, sx))
+    return f / (np.sqrt((np.abs(f) ** 2).sum(axis=(1, 2), keepdims=True)) * dx)
+
+
+@pytest.mark.parametrize("sx,sy", [(16, 16), (40, 40), (64, 64), (256, 256), (192, 64)])
+def test_transforms(ctx, sx, sy):
+    say(f"transforms {sx}x{sy}")
+    g = I.Grid(ctx, sx, sy, 0.1, I.PERIODIC)
+    f = rand_states(2, sx, sy, 0.1, sx + sy)
+    for name in I.TRANSFORMS:
+        a = g.transform(f, name)
+        b = np.stack([O.transform(x, name) for x in f])
+        assert rel(a, b) < 1e-13, name
+
+
+BRANCHES = [(I.PERIODIC, 0.0), (I.DIRICHLET, 0.0), (I.PERIODIC, 1.3), (I.DIRICHLET, 1.3)]
+
+
+@pytest.mark.parametrize("bt,B", BRANCHES)
+def test_operators(ctx, bt, B):
+    sx, sy, L = 64, 128, 12.0
+    dx = L / sx
+    say(f"operators bt={bt} B={B}")
+    g = I.Grid(ctx, sx, sy, dx, bt)
+    V = W.harmonic(sx, sy, dx)
+    pot = I.Potential(g, V)
+    f = rand_states(3, sx, sy, dx, 3)
+    assert rel(g.apply_expkinetic(f, B, 0.07, -1.0, 0.9), O.apply_expkinetic(f, dx, bt, B, 0.07, -1.0, 0.9)) < 1e-13
+    assert rel(pot.apply_exppotential(f, 0.07, -0.5, 0.8), O.apply_exppotential(f, dx, V, 0.07, -0.5, 0.8)) < 1e-14
+    for ex in (1, 2, 3):
+        a = pot.apply_secondorder(f, B, 0.05, 0.7, ex)
+        assert rel(a, O.apply_secondorder(f, dx, bt, B, V, 0.05, 0.7, ex)) < 1e-13
+    H = I.Hamiltonian(pot, B)
+    assert rel(H.apply(f), O.apply_hamiltonian(f, dx, bt, B, V)) < 1e-13
+
+
+@pytest.mark.parametrize("bt,B", BRANCHES)
+@pytest.mark.parametrize("h", [1, 2, 3])
+def test_propagate_and_energies(ctx, bt, B, h):
+    sx = sy = 64
+    dx = 12.0 / sx
+    say(f"propagate bt={bt} B={B} h={h}")
+    g = I.Grid(ctx, sx, sy, dx, bt)
+    V = W.harmonic(sx, sy, dx)
+    pot = I.Potential(g, V)
+    f = rand_states(5, sx, sy, dx, 5)
+    S = I.StateSet(g, 5)
+    S.upload(f)
+    T = I.MultiProductSplit(pot, h, 0.05, B)
+    T.propagate(S)
+    out = S.download()
+    assert rel(out, O.propagate(f, dx, bt, B, V, h, 0.05)) < 1e-13
+    H = I.Hamiltonian(pot, B)
+    E, sd = H.energies(S)
+    Eo, so, _, _ = O.energies(out, dx, bt, B, V)
+    assert np.abs(E - Eo).max() < 1e-12 * np.abs(Eo).max()
+    assert np.abs(sd - so).max() < 1e-9 * np.abs(so).max()
+
+
+def test_null_potential_split(ctx):
+    sx = sy = 32
+    dx = 0.2
+    g = I.Grid(ctx, sx, sy, dx, I.DIRICHLET)
+    pot = I.Potential(g, None)
+    T = I.MultiProductSplit(pot, 4, 0.05, 0.7)
+    assert T.halforder == 1
+    f = rand_states(2, sx, sy, dx, 9)
+    S = I.StateSet(g, 2)
+    S.upload(f)
+    T.propagate(S)
+    assert rel(S.download(), O.propagate(f, dx, I.DIRICHLET, 0.7, None, 4, 0.05)) < 1e-13
+
+
+@pytest.mark.parametrize("N", [1, 7, 64, 130])
+def test_overlap_and_orthonormalize(ctx, N):
+    sx = sy = 64
+    dx = 0.2
+    say(f"overlap/orthonormalize N={N}")
+    g = I.Grid(ctx, sx, sy, dx, I.PERIODIC)
+    f = rand_states(N, sx, sy, dx, N)
+    f *= np.linspace(1, 2, N)[:, None, None]
+    S = I.StateSet(g, N)
+    S.upload(f)
+    Sg = S.overlap()
+    So = O.overlap(f, dx)
+    assert rel(Sg, So) < 1e-14
+    S.orthonormalize()
+    out = S.download()
+    assert O.how_orthonormal(out, dx) < 1e-13
+    rc, ref, _ = O.orthonormalize(f, dx)
+    assert rc == 0
+    # slot r holds the r-th smallest overlap eigenvalue on both sides; equal up to a phase
+    proj = np.abs(np.einsum("nyx,nyx->n", np.conj(out), ref)) * dx * dx
+    assert np.abs(proj - 1).max() < 1e-10
+
+
+@pytest.mark.parametrize("n", [2, 8, 33, 200])
+def test_eigh(ctx, n):
+    say(f"eigh n={n}")
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    A = A + A.conj().T
+    w, V = I.eigh(ctx, A)
+    assert np.abs(w - np.linalg.eigvalsh(A)).max() < 1e-12 * np.abs(w).max()
+    assert np.abs(V @ np.diag(w) @ V.conj().T - A).max() < 1e-12 * np.abs(A).max()
+
+
+def test_zero_state_raises_nonpositive(ctx):
+    sx = sy = 16
+    dx = 0.5
+    g = I.Grid(ctx, sx, sy, dx, I.PERIODIC)
+    f = rand_states(4, sx, sy, dx, 4)
+    f[2] = 0
+    S = I.StateSet(g, 4)
+    S.upload(f)
+    with pytest.raises(I.NonPositiveEigenvalue) as e:
+        S.orthonormalize()
+    assert e.value.n == 0 and e.value.value == 0.0
+    assert np.array_equal(S.download(), f)
+
+
+@pytest.mark.parametrize("bt,B", BRANCHES)
+def test_fp32_fields(ctx, bt, B):
+    sx = sy = 64
+    dx = 12.0 / sx
+    g = I.Grid(ctx, sx, sy, dx, bt)
+    V = W.harmonic(sx, sy, dx)
+    pot = I.Potential(g, V)
+    f = rand_states(4, sx, sy, dx, 11)
+    S = I.StateSet(g, 4, I.FP32)
+    S.upload(f)
+    T = I.MultiProductSplit(pot, 2, 0.05, B)
+    T.propagate(S)
+    out = S.download()
+    assert rel(out, O.propagate(f, dx, bt, B, V, 2, 0.05)) < 2e-6
+    E, _ = I.Hamiltonian(pot, B).energies(S)
+    Eo, _, _, _ = O.energies(out, dx, bt, B, V)
+    assert np.abs(E - Eo).max() < 1e-5 * np.abs(Eo).max()
+
+
+@pytest.mark.parametrize("name", ["c1_ho64", "c2_small"])
+def test_itp_loop_matches_oracle(ctx, name):
+    """Full ITP runs: same step count, same converged count, eigenvalues to 1e-10."""
+    say(f"ITP loop {name}")
+    cfg, V = W.config(name)
+    ref = np.load(os.path.join(GOLD, f"itp_{name}.npz"))
+    s = I.ITPSystem(ctx, cfg, V)
+    s.run()
+    E, sd, _ = s.sorted()
+    assert not s.error_flag
+    assert s.total_steps == int(ref["steps"])
+    assert s.num_converged == int(ref["converged"])
+    nw = cfg["n_wanted"]
+    assert np.abs(E[:nw] - ref["E"][:nw]).max() / np.abs(ref["E"][:nw]).max() < 1e-10
+    if name == "c1_ho64":
+        # analytic levels up to the 2nd-order splitting shift (eps^2 scale)
+        assert np.abs(E[:16] - W.ho_levels(16)).max() < 1e-5
+
+
+def _ho(sx, L):
+    return W.harmonic(sx, sx, L / sx)
+
+
+@pytest.mark.parametrize("case", ["ho_periodic", "ho_dirichlet", "fd_periodic", "fd_dirichlet", "magnetic_box"])
+def test_reference_big_tests(ctx, case):
+    """test_itp.cpp:56-443 on the GPU path (40x40 / 64x64 grids, h=5)."""
+    say(f"reference big test {case}")
+    spec = {
+        "ho_periodic": (40, 12.0, I.PERIODIC, 0.0, 14, 8, 1.0, ("relstdev", 1e-4, 1e-5), ("relstdev", 1e-4)),
+        "ho_dirichlet": (40, 12.0, I.DIRICHLET, 0.0, 14, 8, 1.0, ("relstdev", 1e-4, 1e-5), ("relstdev", 1e-4)),
+        "fd_periodic": (40, 7.0, I.PERIODIC, 10.0, 14, 8, 1.0, ("relstdev", 1e-3, 1e-4), ("relstdev", 1e-3)),
+        "fd_dirichlet": (40, 10.0, I.DIRICHLET, 1.0, 14, 8, 1.0, ("relstdev", 1e-3, 1e-4), ("relstdev", 1e-3)),
+        "magnetic_box": (64, np.pi, I.DIRICHLET, 1.0, 16, 8, 0.3, ("abschange", 1e-3), ("onestep",)),
+    }[case]
+    sx, L, bt, B, N, nw, eps, tt, ft = spec
+    V = None if case == "magnetic_box" else _ho(sx, L)
+    cfg = dict(sx=sx, sy=sx, lenx=L, boundary=bt, N=N, n_wanted=nw, halforder=5, B=B, eps_values=[eps],
+               eps_divisor=5.0, max_steps=500, min_time_step=1e-9, seed=W.SEED_BASE, timestep_test=tt,
+               final_test=ft)
+    s = I.ITPSystem(ctx, cfg, V)
+    s.run()
+    assert not s.error_flag
+    E = s.sorted()[0][:nw]
+    if case.startswith("ho"):
+        assert np.abs(E - W.ho_levels(nw)).max() < 1e-4
+    elif case.startswith("fd"):
+        U = np.sqrt(1 + 0.25 * B * B)
+        fd = np.sort([(2 * n + abs(m) + 1) * U - 0.5 * m * B for n in range(nw) for m in range(-nw, nw + 1)])[:nw]
+        assert np.abs(E - fd).max() < 1e-3
+    else:
+        known = np.array([1.077814, 2.134673, 3.089442, 3.551456, 5.075289, 5.499240, 5.729555, 7.042548])
+        assert np.abs(E - known).max() < 2e-3
+
+
+def test_c2_full_size_steps(ctx):
+    """The 256x256, N=500 Dirichlet B=1 workload: a few steps keep the set
+    orthonormal and the energies finite and ordered (size-independent checks)."""
+    say("C2 full size, 3 steps")
+    cfg, V = W.config("c2_dirichlet256")
+    s = I.ITPSystem(ctx, cfg, V)
+    for _ in range(3):
+        assert s.step() == 0
+    assert not s.error_flag
+    E, sd, idx = s.sorted()
+    assert np.all(np.isfinite(E)) and np.all(np.diff(E) >= 0)
+    assert sorted(idx.tolist()) == list(range(cfg["N"]))
+    st = s.states()
+    dx = cfg["lenx"] / cfg["sx"]
+    A = st.reshape(cfg["N"], -1)
+    G = (A.conj() @ A.T) * dx * dx
+    assert np.abs(G - np.eye(cfg["N"])).max() < 1e-11
