@@ -515,7 +515,7 @@ int itp_grid_create(itp_ctx* c, size_t sx, size_t sy, double dx, int boundary, i
 
 void itp_grid_destroy(itp_grid* g) {
   if (!g) return;
-  (void)hipStreamSynchronize(g->ctx->stream);
+  (void)hipDeviceSynchronize();  // no parent access: parents may already be gone
   if (g->d_tw) (void)hipFree(g->d_tw);
   delete g;
 }
@@ -596,7 +596,7 @@ int itp_potential_create(itp_grid* g, const double* V, itp_potential** out) {
 
 void itp_potential_destroy(itp_potential* p) {
   if (!p) return;
-  (void)hipStreamSynchronize(p->grid->ctx->stream);
+  (void)hipDeviceSynchronize();  // no parent access: parents may already be gone
   if (p->dV) (void)hipFree(p->dV);
   delete p;
 }
@@ -659,7 +659,7 @@ int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out
 
 void itp_stateset_destroy(itp_stateset* s) {
   if (!s) return;
-  (void)hipStreamSynchronize(s->grid->ctx->stream);
+  (void)hipDeviceSynchronize();  // no parent access: parents may already be gone
   void* ptrs[] = {s->buf[0], s->buf[1], s->part, s->A[0], s->A[1], s->V[0], s->V[1], s->Vinit, s->C,
                   s->lam, s->perm, s->info, s->bad, s->epart, s->dE, s->dmean};
   for (void* p : ptrs)
@@ -727,7 +727,7 @@ int itp_split_create(itp_potential* pot, int halforder, double eps, double B, it
 
 void itp_split_destroy(itp_split* sp) {
   if (!sp) return;
-  (void)hipStreamSynchronize(sp->pot->grid->ctx->stream);
+  (void)hipDeviceSynchronize();  // no parent access: parents may already be gone
   for (auto& m : sp->members) free_member(m.get());
   delete sp;
 }
@@ -793,7 +793,7 @@ int itp_hamiltonian_create(itp_potential* pot, double B, itp_hamiltonian** out) 
 
 void itp_hamiltonian_destroy(itp_hamiltonian* h) {
   if (!h) return;
-  (void)hipStreamSynchronize(h->pot->grid->ctx->stream);
+  (void)hipDeviceSynchronize();  // no parent access: parents may already be gone
   if (h->dev) (void)hipFree(h->dev);
   delete h;
 }

@@ -95,7 +95,8 @@ hipError_t launch_convert(const void* in, void* out, long long n, int to_f32, hi
 // Overlap matrix on FP64 MFMA.
 //
 // v_mfma_f64_16x16x4f64: D(16x16) += A(16x4) B(4x16). Lane l holds
-// A[l%16][l/16], B[l/16][l%16] and D[4(l/16)+v][l%16], v = 0..3.
+// A[l%16][l/16], B[l/16][l%16] and D[l/16 + 4v][l%16], v = 0..3 (measured on
+// gfx950 with indicator inputs; the f32 16x16 shapes use 4(l/16)+v instead).
 // S_ij = sum_t conj(psi_i(t)) psi_j(t): with a = psi_i, b = psi_j,
 // Re += a.x b.x + a.y b.y, Im += a.x b.y - a.y b.x.
 // ---------------------------------------------------------------------------
@@ -149,7 +150,7 @@ __global__ __launch_bounds__(256) void gram_kernel(const cplx<R>* psi, long long
   for (int mi = 0; mi < 2; mi++)
     for (int nj = 0; nj < 2; nj++)
       for (int v = 0; v < 4; v++) {
-        const int i = bi * 64 + wi * 32 + mi * 16 + 4 * lk + v;
+        const int i = bi * 64 + wi * 32 + mi * 16 + lk + 4 * v;
         const int j = bj * 64 + wj * 32 + nj * 16 + li;
         P[(long long)i * nc + j] = mk<double>(accr[mi][nj][v], acci[mi][nj][v]);
       }
@@ -409,7 +410,7 @@ __global__ __launch_bounds__(256) void rotate_kernel(const cplx<R>* in, cplx<R>*
   for (int mi = 0; mi < 2; mi++)
     for (int nj = 0; nj < 2; nj++)
       for (int v = 0; v < 4; v++) {
-        const int r = r0 + wr * 32 + mi * 16 + 4 * lk + v;
+        const int r = r0 + wr * 32 + mi * 16 + lk + 4 * v;
         const long long t = t0 + wt * 32 + nj * 16 + li;
         if (r < N && t < M) out[(long long)r * M + t] = mk<R>(R(accr[mi][nj][v]), R(acci[mi][nj][v]));
       }

@@ -112,9 +112,12 @@ def test_overlap_and_orthonormalize(ctx, N):
     S.upload(f)
     Sg = S.overlap()
     So = O.overlap(f, dx)
-    assert rel(Sg, So) < 1e-14
+    say(f"overlap N={N}: rel diff {rel(Sg, So):.2e}")
+    # different summation orders: the difference is at the sqrt(M) eps level
+    assert rel(Sg, So) < 1e-13
     S.orthonormalize()
     out = S.download()
+    say(f"orthonormalize N={N}: how_orthonormal {O.how_orthonormal(out, dx):.2e}")
     assert O.how_orthonormal(out, dx) < 1e-13
     rc, ref, _ = O.orthonormalize(f, dx)
     assert rc == 0
