@@ -14,6 +14,7 @@
 
 #include "../../include/itp2d.h"
 #include "engine.hpp"
+#include "block_jacobi.hpp"
 #include "gpu.hpp"
 
 using namespace itp;
@@ -63,6 +64,9 @@ struct itp_stateset {
   cplx<double>* A[2] = {nullptr, nullptr};
   cplx<double>* V[2] = {nullptr, nullptr};
   cplx<double>* Vinit = nullptr;
+  cplx<double>* bjQ = nullptr;     // block Jacobi rotations and diagonalised sub-blocks
+  cplx<double>* bjD = nullptr;
+  double* bjslots = nullptr;
   cplx<double>* C = nullptr;
   double* lam = nullptr;
   int* perm = nullptr;
@@ -98,6 +102,8 @@ struct itp_hamiltonian {
 };
 
 namespace {
+
+constexpr int kMaxSweeps = 40;
 
 int set_err(itp_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -327,7 +333,8 @@ int eigen_stage(itp_stateset* s, size_t* bad_n, double* bad_lambda, bool scale) 
   const int N = (int)s->N;
   Phase pe(c, ITP_PHASE_EIGENSOLVE);
   HIPCK(c, hipMemcpyAsync(s->V[0], s->Vinit, (size_t)s->np * s->np * 16, hipMemcpyDeviceToDevice, c->stream));
-  HIPCK(c, launch_jacobi(s->A[0], s->A[1], s->V[0], s->V[1], s->np, 60, 1e-14, s->info, c->num_cus, c->stream));
+  HIPCK(c, launch_block_jacobi(s->A[0], s->A[1], s->V[0], s->V[1], s->bjQ, s->bjD, s->bjslots, s->np,
+                               kMaxSweeps, 8, 1e-14, s->info, c->num_cus, c->stream));
   HIPCK(c, launch_eig_finish(s->A[0], s->V[0], s->np, N, s->lam, s->perm, s->C, s->nc, scale ? 1 : 0, c->stream));
   double info[4];
   int bad = 0;
@@ -618,8 +625,8 @@ int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out
       if ((rc = dalloc(c, &s->buf[b], N * M * s->esz))) return rc;
       HIPCK(c, hipMemset(s->buf[b], 0, N * M * s->esz));
     }
-    s->np = (int)(N + (N & 1));
     s->nc = (int)((N + 63) / 64 * 64);
+    s->np = s->nc;  // the eigensolver works on the zero-padded nc x nc matrix
     const int nb = s->nc / 64;
     const int npairs = nb * (nb + 1) / 2;
     long long nchunk = (2048 + npairs - 1) / npairs;
@@ -637,6 +644,12 @@ int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out
       HIPCK(c, hipMemset(s->A[b], 0, np2));
     }
     if ((rc = dalloc(c, (void**)&s->Vinit, np2))) return rc;
+    {
+      const size_t qbytes = (size_t)(s->np / JS) * JS2 * 16;
+      if ((rc = dalloc(c, (void**)&s->bjQ, qbytes))) return rc;
+      if ((rc = dalloc(c, (void**)&s->bjD, qbytes))) return rc;
+      if ((rc = dalloc(c, (void**)&s->bjslots, 3 * kMaxSweeps * sizeof(double)))) return rc;
+    }
     {
       std::vector<cplx<double>> I((size_t)s->np * s->np, mk<double>(0.0, 0.0));
       for (int i = 0; i < s->np; i++) I[(size_t)i * s->np + i] = mk<double>(1.0, 0.0);
@@ -660,7 +673,8 @@ int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out
 void itp_stateset_destroy(itp_stateset* s) {
   if (!s) return;
   (void)hipDeviceSynchronize();  // no parent access: parents may already be gone
-  void* ptrs[] = {s->buf[0], s->buf[1], s->part, s->A[0], s->A[1], s->V[0], s->V[1], s->Vinit, s->C,
+  void* ptrs[] = {s->buf[0], s->buf[1], s->part, s->A[0], s->A[1], s->V[0], s->V[1], s->Vinit, s->bjQ, s->bjD,
+                  s->bjslots, s->C,
                   s->lam, s->perm, s->info, s->bad, s->epart, s->dE, s->dmean};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);

@@ -8,6 +8,8 @@
 #include <memory>
 #include <omp.h>
 #include "../../itp2d_amd/csrc/engine.hpp"
+#include "../../itp2d_amd/csrc/block_jacobi.hpp"
+#include <algorithm>
 
 using namespace itp;
 
@@ -190,6 +192,59 @@ int emu_hamiltonian(int sx, int sy, double dx, int bt, double B, const double* V
     if (fp32) go(float()); else go(double());
     return 0;
   });
+}
+
+// Block Jacobi eigensolver phases (block_jacobi.hpp) run in order on the CPU.
+// A: n x n complex column-major (upper triangle read). Outputs ascending
+// eigenvalues w and eigenvectors (columns) in A. Returns sweeps done, or -1.
+int emu_block_jacobi(int n, double* Ain, double* w, int max_inner) {
+  const int np = (n + 63) / 64 * 64;
+  std::vector<cplx<double>> A0((size_t)np * np, mk<double>(0.0, 0.0)), A1(A0.size()), V0(A0.size()), V1(A0.size());
+  const cplx<double>* a = reinterpret_cast<const cplx<double>*>(Ain);
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) A0[(size_t)i * np + j] = (i <= j) ? a[(size_t)j * n + i] : conj(a[(size_t)i * n + j]);
+  for (int i = 0; i < n; i++) A0[(size_t)i * np + i].y = 0;
+  for (int i = 0; i < np; i++) V0[(size_t)i * np + i] = mk<double>(1.0, 0.0);
+  const int P = np / JS, nb = np / JB;
+  std::vector<cplx<double>> Q((size_t)P * JS2), D(Q.size());
+  std::vector<double> slots(3 * 64, 0.0);
+  BlockJacobiArgs args;
+  args.A[0] = A0.data(); args.A[1] = A1.data(); args.V[0] = V0.data(); args.V[1] = V1.data();
+  args.Q = Q.data(); args.D = D.data(); args.slots = slots.data(); args.info = nullptr;
+  args.np = np; args.max_sweeps = 40; args.max_inner = max_inner; args.tol = 1e-14;
+  std::vector<char> lds(bj_lds_bytes(256));
+  BJLds ls = bj_carve(lds.data());
+  CpuExec ex;
+  int cur = 0, sweeps = 0;
+  bool conv = false;
+  for (int sw = 0; sw < args.max_sweeps && !conv; sw++) {
+    double mx[3] = {0, 0, 0};
+    for (int r = 0; r < nb - 1; r++) {
+      const cplx<double>* A = args.A[cur];
+      cplx<double>* An = args.A[cur ^ 1];
+      for (int k = 0; k < P; k++) {
+        double o3[3];
+        bj_inner(ex, args, A, k, ls, o3);
+        for (int q = 0; q < 3; q++) mx[q] = std::max(mx[q], o3[q]);
+      }
+      for (int t = 0; t < bj_items(np); t++) bj_apply(ex, args, A, An, args.V[cur], args.V[cur ^ 1], t, ls);
+      cur ^= 1;
+    }
+    sweeps++;
+    if (mx[0] < args.tol || mx[1] <= 1e-15 * mx[2]) conv = true;
+  }
+  if (!conv) return -1;
+  const cplx<double>* A = args.A[cur];
+  const cplx<double>* V = args.V[cur];
+  std::vector<int> idx(n);
+  for (int i = 0; i < n; i++) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return A[(size_t)x * np + x].x < A[(size_t)y * np + y].x; });
+  cplx<double>* out = reinterpret_cast<cplx<double>*>(Ain);
+  for (int r = 0; r < n; r++) {
+    w[r] = A[(size_t)idx[r] * np + idx[r]].x;
+    for (int c = 0; c < n; c++) out[(size_t)r * n + c] = V[(size_t)c * np + idx[r]];
+  }
+  return sweeps;
 }
 
 }  // extern "C"

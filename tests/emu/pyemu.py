@@ -24,6 +24,7 @@ def lib():
         _lib.emu_hamiltonian.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, _dp, _dp,
                                          C.c_int, _dp, _dp, C.c_int]
         _lib.emu_plan.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        _lib.emu_block_jacobi.argtypes = [C.c_int, _dp, _dp, C.c_int]
     return _lib
 
 
@@ -71,3 +72,13 @@ def hamiltonian(fields, dx, bt, B, V, fp32=False):
     assert lib().emu_hamiltonian(sx, sy, dx, bt, B, _p(Vc), _p(f.view(np.float64)), n, _p(E), _p(sd),
                                  int(fp32)) == 0
     return f, E, sd
+
+
+def block_jacobi(A, max_inner=8):
+    """Emulated block Jacobi: (ascending eigenvalues, eigenvectors in columns, sweeps)."""
+    A = np.asarray(A, dtype=np.complex128)
+    n = A.shape[0]
+    a = np.ascontiguousarray(A.T).ravel().copy()
+    w = np.zeros(n)
+    sweeps = lib().emu_block_jacobi(n, _p(a.view(np.float64)), _p(w), max_inner)
+    return w, np.ascontiguousarray(a.reshape(n, n).T), sweeps

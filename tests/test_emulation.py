@@ -76,3 +76,28 @@ def test_fp32_propagation_tolerance(bt, B):
     a = E.split(f, dx, bt, B, V, 0.05, halforder=2, fp32=True)
     b = O.propagate(f, dx, bt, B, V, 2, 0.05)
     assert rel(a, b) < 2e-6
+
+
+@pytest.mark.parametrize("n", [2, 7, 33, 64, 100])
+def test_block_jacobi_emulated(n):
+    """The block Jacobi phases (block_jacobi.hpp) run on the CPU: eigenpairs of random Hermitian matrices."""
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    A = A + A.conj().T
+    w, V, sweeps = E.block_jacobi(A)
+    assert sweeps > 0
+    scale = np.abs(np.linalg.eigvalsh(A)).max()
+    assert np.abs(w - np.linalg.eigvalsh(A)).max() < 1e-13 * n * scale
+    assert np.abs(V @ np.diag(w) @ V.conj().T - A).max() < 1e-13 * n * scale
+    assert np.abs(V.conj().T @ V - np.eye(n)).max() < 1e-14 * n
+
+
+def test_block_jacobi_near_diagonal_converges_fast():
+    rng = np.random.default_rng(2)
+    n = 150
+    d = np.exp(-0.1 * np.sort(rng.uniform(1, 30, n)))
+    Eps = 1e-7 * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    A = np.diag(d) + Eps + Eps.conj().T
+    w, V, sweeps = E.block_jacobi(A)
+    assert sweeps <= 3
+    assert np.abs(w - np.linalg.eigvalsh(A)).max() < 5e-14
