@@ -164,6 +164,13 @@ int itp_ctx_enable_timing(itp_ctx* ctx, int enable);
 /* Accumulated event time per phase since the last reset (ms). */
 int itp_ctx_phase_times(itp_ctx* ctx, double* ms, long long* counts);
 int itp_ctx_reset_phase_times(itp_ctx* ctx);
+/* Eigensolver statistics since the last reset: calls and outer sweeps. */
+int itp_ctx_eig_stats(itp_ctx* ctx, long long* calls, long long* sweeps);
+/* Upper bound on the element sweeps spent on each block sub-problem per round
+ * of the block Jacobi eigensolver (default 1). */
+int itp_ctx_set_eig_inner(itp_ctx* ctx, int inner);
+/* Workgroups of the cooperative eigensolver launch (default 64). */
+int itp_ctx_set_eig_grid(itp_ctx* ctx, int workgroups);
 
 #ifdef __cplusplus
 }

@@ -80,6 +80,9 @@ def lib():
             "itp_ctx_enable_timing": [_vp, C.c_int],
             "itp_ctx_phase_times": [_vp, _dp, C.POINTER(C.c_longlong)],
             "itp_ctx_reset_phase_times": [_vp],
+            "itp_ctx_eig_stats": [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
+            "itp_ctx_set_eig_inner": [_vp, C.c_int],
+            "itp_ctx_set_eig_grid": [_vp, C.c_int],
             "itp_grid_create": [_vp, _sz, _sz, C.c_double, C.c_int, C.POINTER(_vp)],
             "itp_grid_destroy": [_vp],
             "itp_grid_kvalues": [_vp, _dp, _dp, _dp, _dp],
@@ -191,6 +194,17 @@ class Context:
 
     def reset_phase_times(self):
         lib().itp_ctx_reset_phase_times(self.h)
+
+    def eig_stats(self):
+        calls, sweeps = C.c_longlong(0), C.c_longlong(0)
+        lib().itp_ctx_eig_stats(self.h, C.byref(calls), C.byref(sweeps))
+        return calls.value, sweeps.value
+
+    def set_eig_inner(self, inner):
+        self.check(lib().itp_ctx_set_eig_inner(self.h, int(inner)), "set_eig_inner")
+
+    def set_eig_grid(self, workgroups):
+        self.check(lib().itp_ctx_set_eig_grid(self.h, int(workgroups)), "set_eig_grid")
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

@@ -4,17 +4,18 @@
 //
 // The np x np matrix (np a multiple of 2*JB) is split into nb = np/JB blocks
 // of JB indices. An outer sweep is nb-1 rounds of the round-robin pairing of
-// blocks (dense_ops.hpp's rr_next at block granularity). In a round, every
-// block pair k (super-block of 2*JB indices at positions 2k, 2k+1) is:
-//   phase 1: loaded into LDS and diagonalised by element Jacobi sweeps
-//            (jacobi_block / jacobi_vrow on the 2JB x 2JB sub-problem),
-//            accumulating the 2JB x 2JB unitary Q_k;
-//   phase 2: every super-tile of A becomes Q_K^H A[K, K'] Q_K' and every
-//            super-column of V becomes V[:, K] Q_K, written at the next
-//            round's block positions (ping-pong buffers).
-// After a full outer sweep blocks are back in their original positions, so the
-// diagonal of A holds the eigenvalues and V's columns the eigenvectors in the
-// original index order.
+// block positions (dense_ops.hpp's rr_next at block granularity); a table
+// pos2blk maps positions to blocks, so the matrix itself never moves. In a
+// round, every block pair k (blocks at positions 2k, 2k+1) is:
+//   phase 1: gathered into LDS as a 2JB x 2JB sub-problem; if its relative
+//            off-diagonal part is already below the tolerance it is left alone
+//            (identity rotation), otherwise it is diagonalised by element
+//            Jacobi sweeps (jacobi_block / jacobi_vrow), accumulating Q_k;
+//   phase 2: every super-tile of A touched by a non-identity rotation becomes
+//            Q_K^H A[K, K'] Q_K' (in place, with its mirror) and every
+//            super-column of V becomes V[:, K] Q_K.
+// Convergence is checked on the whole matrix before the first sweep and after
+// each sweep, so an already diagonal matrix costs one check and no sweep.
 #pragma once
 
 #include "dense_ops.hpp"
@@ -24,40 +25,49 @@ namespace itp {
 constexpr int JB = 16;          // indices per block
 constexpr int JS = 2 * JB;      // super-block (block pair) size
 constexpr int JS2 = JS * JS;
+constexpr int kMaxBlocks = 512; // np <= 8192
 
 struct BlockJacobiArgs {
-  cplx<double>* A[2];
-  cplx<double>* V[2];
+  cplx<double>* A;
+  cplx<double>* V;
   cplx<double>* Q;       // [nb/2][JS][JS] rotations of the round
   cplx<double>* D;       // [nb/2][JS][JS] diagonalised sub-blocks of the round
-  double* slots;         // [max_sweeps][3]: max relative off-diagonal, max |off|, max |diag| (as atomics)
-  double* info;          // [0] sweeps, [1] converged, [2] final buffer
+  int* ident;            // [nb/2] 1 if Q_k is the identity
+  double* slots;         // [max_sweeps + 1]: largest element ratio of each whole-matrix check (atomic max)
+  double* info;          // [0] sweeps, [1] converged
   int np;
   int max_sweeps;
   int max_inner;
   double tol;
 };
 
-// LDS for one workgroup: 4 JS x JS complex matrices + rotation parameters + reductions
+// LDS for one workgroup: 4 JS x JS complex matrices + rotation parameters +
+// reductions + the position table.
 struct BJLds {
   cplx<double>* m[4];
   double* cs;
   cplx<double>* sj;
-  double* red;  // [3 * nthreads]
+  double* red;  // [3 * nthreads + 3]
+  int* pos2blk; // [kMaxBlocks] block at each position this round
+  int* tmpblk;  // [kMaxBlocks]
+  double* floor_; // [1] absolute convergence floor of the current sweep
 };
 
 ITP_HD size_t bj_lds_bytes(int nthreads) {
   return (size_t)4 * JS2 * sizeof(cplx<double>) + JB * (sizeof(double) + sizeof(cplx<double>)) +
-         (size_t)3 * nthreads * sizeof(double) + 64;
+         (size_t)(3 * nthreads + 3) * sizeof(double) + 2 * kMaxBlocks * sizeof(int) + 2 * sizeof(double) + 64;
 }
 
-ITP_HD BJLds bj_carve(void* base) {
+ITP_HD BJLds bj_carve(void* base, int nthreads) {
   BJLds s;
   cplx<double>* p = reinterpret_cast<cplx<double>*>(base);
   for (int i = 0; i < 4; i++) { s.m[i] = p; p += JS2; }
   s.sj = p; p += JB;
   s.cs = reinterpret_cast<double*>(p);
   s.red = s.cs + JB;
+  s.pos2blk = reinterpret_cast<int*>(s.red + 3 * nthreads + 3);
+  s.tmpblk = s.pos2blk + kMaxBlocks;
+  s.floor_ = reinterpret_cast<double*>(s.tmpblk + kMaxBlocks);
   return s;
 }
 
@@ -74,163 +84,210 @@ ITP_HD double bitsd(unsigned long long u) {
   return c.d;
 }
 
-// Phase 1 for block pair k. Writes Q_k and the diagonalised sub-block D_k,
-// and returns (in out3) the largest relative and absolute off-diagonal and
-// largest diagonal seen in the first inner sweep (the sub-problem as loaded).
+// Global index of local index i (0..2JB) of the super-block made of blocks b1, b2.
+ITP_HD long long bj_gidx(int b1, int b2, int i) {
+  return i < JB ? (long long)b1 * JB + i : (long long)b2 * JB + (i - JB);
+}
+
+// Convergence of one off-diagonal element: ratio < 1 means converged, with
+//   ratio = |a_ij| / max(tol sqrt(|a_ii a_jj|), floor),
+// floor = np eps max_k |a_kk| from the last whole-matrix check. The same
+// element test decides the whole-matrix check and the skipping of
+// sub-problems, so no unconverged element can be left alone.
+ITP_HD double bj_ratio(cplx<double> aij, double aii, double ajj, double tol, double floor) {
+  const double ab = std::hypot(aij.x, aij.y);
+  double den = tol * std::sqrt(std::fabs(aii * ajj));
+  den = den > floor ? den : floor;
+  return den > 0 ? ab / den : (ab > 0 ? 1e300 : 0.0);
+}
+
+// Block reduction of per-thread maxima red[tid] into red[nt] (one thread, fixed order).
 template <class Exec>
-ITP_HD void bj_inner(Exec& ex, const BlockJacobiArgs& a, const cplx<double>* A, int k, const BJLds& s,
-                     double* out3) {
+ITP_HD void bj_reduce_max(Exec& ex, const BJLds& s) {
+  ex.run([&](int tid, int nt) {
+    if (tid == 0) {
+      double m = 0;
+      for (int t = 0; t < nt; t++) m = s.red[t] > m ? s.red[t] : m;
+      s.red[nt] = m;
+    }
+  });
+}
+
+// Phase 1 for block pair k (blocks b1, b2): writes Q_k, D_k and ident[k].
+template <class Exec>
+ITP_HD void bj_inner(Exec& ex, const BlockJacobiArgs& a, int k, int b1, int b2, const BJLds& s) {
   const int np = a.np;
-  const long long r0 = (long long)k * JS;
   cplx<double>* M0 = s.m[0];
   cplx<double>* M1 = s.m[1];
   cplx<double>* Q0 = s.m[2];
   cplx<double>* Q1 = s.m[3];
+  const cplx<double>* A = a.A;
+  const double fl = s.floor_[0];
   ex.run([&](int tid, int nt) {
+    double r = 0;
     for (int e = tid; e < JS2; e += nt) {
       const int i = e / JS, j = e - i * JS;
-      M0[e] = A[(r0 + i) * np + r0 + j];
+      const long long gi = bj_gidx(b1, b2, i), gj = bj_gidx(b1, b2, j);
+      M0[e] = A[gi * np + gj];
       Q0[e] = mk<double>(i == j ? 1.0 : 0.0, 0.0);
-    }
-  });
-  int cur = 0;
-  double first[3] = {0, 0, 0};
-  for (int sw = 0; sw < a.max_inner; sw++) {
-    for (int r = 0; r < JS - 1; r++) {
-      const cplx<double>* Mc = cur ? M1 : M0;
-      cplx<double>* Mn = cur ? M0 : M1;
-      const cplx<double>* Qc = cur ? Q1 : Q0;
-      cplx<double>* Qn = cur ? Q0 : Q1;
-      ex.run([&](int tid, int nt) {
-        for (int q = tid; q < JB; q += nt) {
-          const int p0 = 2 * q, p1 = p0 + 1;
-          double c, off;
-          cplx<double> sv;
-          const double app = Mc[p0 * JS + p0].x, aqq = Mc[p1 * JS + p1].x;
-          const cplx<double> b = Mc[p0 * JS + p1];
-          jacobi_rotation(app, aqq, b, c, sv, off);
-          s.cs[q] = c;
-          s.sj[q] = sv;
-          const double ab = std::hypot(b.x, b.y);
-          const double dd = std::fabs(app) > std::fabs(aqq) ? std::fabs(app) : std::fabs(aqq);
-          double* rd = s.red + 3 * tid;
-          if (r == 0) { rd[0] = 0; rd[1] = 0; rd[2] = 0; }
-          rd[0] = off > rd[0] ? off : rd[0];
-          rd[1] = ab > rd[1] ? ab : rd[1];
-          rd[2] = dd > rd[2] ? dd : rd[2];
-        }
-      });
-      ex.run([&](int tid, int nt) {
-        const int nblk = JB * JB;
-        for (int it = tid; it < nblk + JS * JB; it += nt) {
-          if (it < nblk) {
-            const int kI = it / JB, kJ = it - kI * JB;
-            if (kJ >= kI) jacobi_block(Mc, Mn, JS, JB, kI, kJ, s.cs, s.sj);
-          } else {
-            const int i2 = it - nblk;
-            jacobi_vrow(Qc, Qn, JS, JB, i2 / JB, i2 % JB, s.cs, s.sj);
-          }
-        }
-      });
-      cur ^= 1;
-    }
-    // sweep maxima (threads < JB hold them)
-    double mx[3] = {0, 0, 0};
-    ex.run([&](int tid, int nt) {
-      (void)nt;
-      if (tid == 0) {
-        double m0 = 0, m1 = 0, m2 = 0;
-        for (int t = 0; t < JB; t++) {
-          const double* rd = s.red + 3 * t;
-          m0 = rd[0] > m0 ? rd[0] : m0;
-          m1 = rd[1] > m1 ? rd[1] : m1;
-          m2 = rd[2] > m2 ? rd[2] : m2;
-        }
-        s.red[3 * JB] = m0;
-        s.red[3 * JB + 1] = m1;
-        s.red[3 * JB + 2] = m2;
+      if (j > i) {
+        const double q = bj_ratio(A[gi * np + gj], A[gi * np + gi].x, A[gj * np + gj].x, a.tol, fl);
+        r = q > r ? q : r;
       }
-    });
-    mx[0] = s.red[3 * JB];
-    mx[1] = s.red[3 * JB + 1];
-    mx[2] = s.red[3 * JB + 2];
-    if (sw == 0) { first[0] = mx[0]; first[1] = mx[1]; first[2] = mx[2]; }
-    const bool done = (mx[0] < a.tol) || (mx[1] <= 1e-15 * mx[2]);
-    ex.run([&](int tid, int nt) { (void)tid; (void)nt; });  // all threads read the maxima before reuse
-    if (done) break;
+    }
+    s.red[tid] = r;
+  });
+  bj_reduce_max(ex, s);
+  int cur = 0;
+  bool skip = false;
+  ex.run([&](int tid, int nt) { (void)tid; skip = s.red[nt] < 1.0; });
+  if (!skip) {
+    for (int sw = 0; sw < a.max_inner; sw++) {
+      for (int r = 0; r < JS - 1; r++) {
+        const cplx<double>* Mc = cur ? M1 : M0;
+        cplx<double>* Mn = cur ? M0 : M1;
+        const cplx<double>* Qc = cur ? Q1 : Q0;
+        cplx<double>* Qn = cur ? Q0 : Q1;
+        ex.run([&](int tid, int nt) {
+          if (tid < JB) {
+            const int p0 = 2 * tid, p1 = p0 + 1;
+            double c, off;
+            cplx<double> sv;
+            jacobi_rotation(Mc[p0 * JS + p0].x, Mc[p1 * JS + p1].x, Mc[p0 * JS + p1], c, sv, off);
+            s.cs[tid] = c;
+            s.sj[tid] = sv;
+            const double q = bj_ratio(Mc[p0 * JS + p1], Mc[p0 * JS + p0].x, Mc[p1 * JS + p1].x, a.tol, fl);
+            if (r == 0) s.red[tid] = 0;
+            s.red[tid] = q > s.red[tid] ? q : s.red[tid];
+          }
+          (void)nt;
+        });
+        ex.run([&](int tid, int nt) {
+          const int nblk = JB * JB;
+          for (int it = tid; it < nblk + JS * JB; it += nt) {
+            if (it < nblk) {
+              const int kI = it / JB, kJ = it - kI * JB;
+              if (kJ >= kI) jacobi_block(Mc, Mn, JS, JB, kI, kJ, s.cs, s.sj);
+            } else {
+              const int i2 = it - nblk;
+              jacobi_vrow(Qc, Qn, JS, JB, i2 / JB, i2 % JB, s.cs, s.sj);
+            }
+          }
+        });
+        cur ^= 1;
+      }
+      // stop when a whole inner sweep met only converged pairs
+      bool done = false;
+      ex.run([&](int tid, int nt) {
+        (void)nt;
+        (void)tid;
+        double m = 0;
+        for (int t = 0; t < JB; t++) m = s.red[t] > m ? s.red[t] : m;
+        done = m < 1.0;
+      });
+      if (done) break;
+    }
   }
   const cplx<double>* Mf = cur ? M1 : M0;
   const cplx<double>* Qf = cur ? Q1 : Q0;
   cplx<double>* Qg = a.Q + (long long)k * JS2;
   cplx<double>* Dg = a.D + (long long)k * JS2;
+  int* idg = a.ident + k;
   ex.run([&](int tid, int nt) {
-    for (int e = tid; e < JS2; e += nt) { Qg[e] = Qf[e]; Dg[e] = Mf[e]; }
+    if (!skip)
+      for (int e = tid; e < JS2; e += nt) { Qg[e] = Qf[e]; Dg[e] = Mf[e]; }
+    if (tid == 0) *idg = skip ? 1 : 0;
   });
-  out3[0] = first[0];
-  out3[1] = first[1];
-  out3[2] = first[2];
 }
 
-// Global row index after the round for local row r of pair K (2JB rows).
-ITP_HD long long bj_newrow(int K, int r, int mo) {
-  const int blk = 2 * K + r / JB;
-  return (long long)rr_next(blk, mo) * JB + (r % JB);
+// 32 x 32 complex product C = op(A) B from LDS operands (op = conjugate
+// transpose if ah). On the GPU: v_mfma_f64_16x16x4f64, one 16 x 16 quarter per
+// wavefront (lane l: A[l%16][l/16], B[l/16][l%16], D[l/16+4v][l%16]); in the
+// CPU emulation: a plain loop.
+template <class Exec>
+ITP_HD void bj_cgemm32(Exec& ex, const cplx<double>* A, bool ah, const cplx<double>* B, cplx<double>* C) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  (void)ex;
+  typedef double d4 __attribute__((ext_vector_type(4)));
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wi = w >> 1, wj = w & 1;
+  const int li = lane & 15, lk = lane >> 4;
+  d4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+  for (int kk = 0; kk < JS / 4; kk++) {
+    const int kx = kk * 4 + lk;
+    const cplx<double> av = ah ? conj(A[kx * JS + wi * 16 + li]) : A[(wi * 16 + li) * JS + kx];
+    const cplx<double> bv = B[kx * JS + wj * 16 + li];
+    cr = __builtin_amdgcn_mfma_f64_16x16x4f64(av.x, bv.x, cr, 0, 0, 0);
+    cr = __builtin_amdgcn_mfma_f64_16x16x4f64(-av.y, bv.y, cr, 0, 0, 0);
+    ci = __builtin_amdgcn_mfma_f64_16x16x4f64(av.x, bv.y, ci, 0, 0, 0);
+    ci = __builtin_amdgcn_mfma_f64_16x16x4f64(av.y, bv.x, ci, 0, 0, 0);
+  }
+  for (int v = 0; v < 4; v++) C[(wi * 16 + lk + 4 * v) * JS + wj * 16 + li] = mk<double>(cr[v], ci[v]);
+  __syncthreads();
+#else
+  ex.run([&](int tid, int nt) {
+    for (int e = tid; e < JS2; e += nt) {
+      const int i = e / JS, j = e - i * JS;
+      cplx<double> acc = mk<double>(0.0, 0.0);
+      for (int l = 0; l < JS; l++) {
+        const cplx<double> av = ah ? conj(A[l * JS + i]) : A[i * JS + l];
+        acc = acc + av * B[l * JS + j];
+      }
+      C[e] = acc;
+    }
+  });
+#endif
 }
 
 // Phase 2 work item t: A super-tiles (K <= K') first, then V super-columns.
+// pos2blk gives the blocks of each pair for this round.
 template <class Exec>
-ITP_HD void bj_apply(Exec& ex, const BlockJacobiArgs& a, const cplx<double>* A, cplx<double>* An,
-                     const cplx<double>* V, cplx<double>* Vn, int t, const BJLds& s) {
+ITP_HD void bj_apply(Exec& ex, const BlockJacobiArgs& a, int t, const BJLds& s) {
   const int np = a.np;
-  const int P = np / JS;       // block pairs
-  const int mo = np / JB / 2;  // outer round-robin pairs
+  const int P = np / JS;
   const int ntA = P * (P + 1) / 2;
   cplx<double>* X = s.m[0];
   cplx<double>* Qa = s.m[1];
   cplx<double>* Qb = s.m[2];
   cplx<double>* T = s.m[3];
+  cplx<double>* A = a.A;
+  cplx<double>* V = a.V;
   if (t < ntA) {
     int K = 0, rem = t;
     while (rem >= P - K) { rem -= P - K; K++; }
     const int K2 = K + rem;
+    if (a.ident[K] && a.ident[K2]) return;
+    const int a1 = s.pos2blk[2 * K], a2 = s.pos2blk[2 * K + 1];
+    const int c1 = s.pos2blk[2 * K2], c2 = s.pos2blk[2 * K2 + 1];
     if (K == K2) {
       const cplx<double>* Dg = a.D + (long long)K * JS2;
       ex.run([&](int tid, int nt) {
         for (int e = tid; e < JS2; e += nt) {
           const int i = e / JS, j = e - i * JS;
-          An[bj_newrow(K, i, mo) * np + bj_newrow(K, j, mo)] = Dg[e];
+          A[bj_gidx(a1, a2, i) * np + bj_gidx(a1, a2, j)] = Dg[e];
         }
       });
       return;
     }
     const cplx<double>* QgA = a.Q + (long long)K * JS2;
     const cplx<double>* QgB = a.Q + (long long)K2 * JS2;
+    const bool ia = a.ident[K] != 0, ib = a.ident[K2] != 0;
     ex.run([&](int tid, int nt) {
       for (int e = tid; e < JS2; e += nt) {
         const int i = e / JS, j = e - i * JS;
-        X[e] = A[((long long)K * JS + i) * np + (long long)K2 * JS + j];
-        Qa[e] = QgA[e];
-        Qb[e] = QgB[e];
+        X[e] = A[bj_gidx(a1, a2, i) * np + bj_gidx(c1, c2, j)];
+        Qa[e] = ia ? mk<double>(i == j ? 1.0 : 0.0, 0.0) : QgA[e];
+        Qb[e] = ib ? mk<double>(i == j ? 1.0 : 0.0, 0.0) : QgB[e];
       }
     });
-    // T = X Qb
+    bj_cgemm32(ex, X, false, Qb, T);   // T = X Qb
+    bj_cgemm32(ex, Qa, true, T, X);    // X = Qa^H T
     ex.run([&](int tid, int nt) {
       for (int e = tid; e < JS2; e += nt) {
         const int i = e / JS, j = e - i * JS;
-        cplx<double> acc = mk<double>(0.0, 0.0);
-        for (int l = 0; l < JS; l++) acc = acc + X[i * JS + l] * Qb[l * JS + j];
-        T[e] = acc;
-      }
-    });
-    // Y = Qa^H T, written with its mirror
-    ex.run([&](int tid, int nt) {
-      for (int e = tid; e < JS2; e += nt) {
-        const int i = e / JS, j = e - i * JS;
-        cplx<double> acc = mk<double>(0.0, 0.0);
-        for (int l = 0; l < JS; l++) acc = acc + cmulc(Qa[l * JS + i], T[l * JS + j]);
-        const long long gi = bj_newrow(K, i, mo), gj = bj_newrow(K2, j, mo);
-        An[gi * np + gj] = acc;
-        An[gj * np + gi] = conj(acc);
+        const long long gi = bj_gidx(a1, a2, i), gj = bj_gidx(c1, c2, j);
+        A[gi * np + gj] = X[e];
+        A[gj * np + gi] = conj(X[e]);
       }
     });
     return;
@@ -238,20 +295,21 @@ ITP_HD void bj_apply(Exec& ex, const BlockJacobiArgs& a, const cplx<double>* A, 
   // V super-column: rows [R*JS, R*JS+JS), pair K
   const int t2 = t - ntA;
   const int R = t2 / P, K = t2 - R * P;
+  if (a.ident[K]) return;
+  const int b1 = s.pos2blk[2 * K], b2 = s.pos2blk[2 * K + 1];
   const cplx<double>* Qg = a.Q + (long long)K * JS2;
   ex.run([&](int tid, int nt) {
     for (int e = tid; e < JS2; e += nt) {
       const int i = e / JS, j = e - i * JS;
-      X[e] = V[((long long)R * JS + i) * np + (long long)K * JS + j];
+      X[e] = V[((long long)R * JS + i) * np + bj_gidx(b1, b2, j)];
       Qa[e] = Qg[e];
     }
   });
+  bj_cgemm32(ex, X, false, Qa, T);
   ex.run([&](int tid, int nt) {
     for (int e = tid; e < JS2; e += nt) {
       const int i = e / JS, j = e - i * JS;
-      cplx<double> acc = mk<double>(0.0, 0.0);
-      for (int l = 0; l < JS; l++) acc = acc + X[i * JS + l] * Qa[l * JS + j];
-      Vn[((long long)R * JS + i) * np + bj_newrow(K, j, mo)] = acc;
+      V[((long long)R * JS + i) * np + bj_gidx(b1, b2, j)] = T[e];
     }
   });
 }
@@ -259,6 +317,58 @@ ITP_HD void bj_apply(Exec& ex, const BlockJacobiArgs& a, const cplx<double>* A, 
 ITP_HD int bj_items(int np) {
   const int P = np / JS;
   return P * (P + 1) / 2 + P * P;
+}
+
+// Position table for the next round (all workgroups hold identical copies).
+template <class Exec>
+ITP_HD void bj_next_positions(Exec& ex, const BJLds& s, int nb) {
+  ex.run([&](int tid, int nt) {
+    for (int p = tid; p < nb; p += nt) s.tmpblk[rr_next(p, nb / 2)] = s.pos2blk[p];
+  });
+  ex.run([&](int tid, int nt) {
+    for (int p = tid; p < nb; p += nt) s.pos2blk[p] = s.tmpblk[p];
+  });
+}
+
+template <class Exec>
+ITP_HD void bj_reset_positions(Exec& ex, const BJLds& s, int nb) {
+  ex.run([&](int tid, int nt) {
+    for (int p = tid; p < nb; p += nt) s.pos2blk[p] = p;
+  });
+}
+
+// Whole-matrix check, elements [e0, e1) of this workgroup's slice: every
+// workgroup first derives the same floor from the whole diagonal, then the
+// largest element ratio of its slice lands in red[nt].
+template <class Exec>
+ITP_HD void bj_check_rows(Exec& ex, const BlockJacobiArgs& a, long long e0, long long e1, const BJLds& s) {
+  const int np = a.np;
+  const cplx<double>* A = a.A;
+  ex.run([&](int tid, int nt) {
+    double m = 0;
+    for (int i = tid; i < np; i += nt) {
+      const double d = std::fabs(A[(long long)i * np + i].x);
+      m = d > m ? d : m;
+    }
+    s.red[tid] = m;
+  });
+  bj_reduce_max(ex, s);
+  ex.run([&](int tid, int nt) {
+    if (tid == 0) s.floor_[0] = (double)np * 2.220446049250313e-16 * s.red[nt];
+  });
+  const double fl = s.floor_[0];
+  ex.run([&](int tid, int nt) {
+    double r = 0;
+    for (long long e = e0 + tid; e < e1; e += nt) {
+      const int i = (int)(e / np), j = (int)(e - (long long)i * np);
+      if (j > i) {
+        const double q = bj_ratio(A[e], A[(long long)i * np + i].x, A[(long long)j * np + j].x, a.tol, fl);
+        r = q > r ? q : r;
+      }
+    }
+    s.red[tid] = r;
+  });
+  bj_reduce_max(ex, s);
 }
 
 }  // namespace itp

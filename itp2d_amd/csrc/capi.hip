@@ -34,6 +34,9 @@ struct itp_ctx {
   size_t ev_used = 0;
   double ms[ITP_NUM_PHASES] = {};
   long long cnt[ITP_NUM_PHASES] = {};
+  long long eig_calls = 0, eig_sweeps = 0;
+  int eig_inner = 1;   // element sweeps per block sub-problem and round (upper bound)
+  int eig_grid = 64;   // workgroups of the eigensolver (its grid barrier cost grows with them)
 };
 
 struct itp_grid {
@@ -67,6 +70,7 @@ struct itp_stateset {
   cplx<double>* bjQ = nullptr;     // block Jacobi rotations and diagonalised sub-blocks
   cplx<double>* bjD = nullptr;
   double* bjslots = nullptr;
+  int* bjident = nullptr;
   cplx<double>* C = nullptr;
   double* lam = nullptr;
   int* perm = nullptr;
@@ -333,8 +337,8 @@ int eigen_stage(itp_stateset* s, size_t* bad_n, double* bad_lambda, bool scale) 
   const int N = (int)s->N;
   Phase pe(c, ITP_PHASE_EIGENSOLVE);
   HIPCK(c, hipMemcpyAsync(s->V[0], s->Vinit, (size_t)s->np * s->np * 16, hipMemcpyDeviceToDevice, c->stream));
-  HIPCK(c, launch_block_jacobi(s->A[0], s->A[1], s->V[0], s->V[1], s->bjQ, s->bjD, s->bjslots, s->np,
-                               kMaxSweeps, 8, 1e-14, s->info, c->num_cus, c->stream));
+  HIPCK(c, launch_block_jacobi(s->A[0], s->V[0], s->bjQ, s->bjD, s->bjident, s->bjslots, s->np, kMaxSweeps,
+                               c->eig_inner, 1e-14, s->info, c->eig_grid, c->stream));
   HIPCK(c, launch_eig_finish(s->A[0], s->V[0], s->np, N, s->lam, s->perm, s->C, s->nc, scale ? 1 : 0, c->stream));
   double info[4];
   int bad = 0;
@@ -344,6 +348,8 @@ int eigen_stage(itp_stateset* s, size_t* bad_n, double* bad_lambda, bool scale) 
   HIPCK(c, hipMemcpyAsync(lam.data(), s->lam, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCK(c, hipStreamSynchronize(c->stream));
   pe.stop();
+  c->eig_calls++;
+  c->eig_sweeps += (long long)info[0];
   if (bad) return set_err(c, ITP_E_EIGENSOLVER, "overlap matrix has non-finite entries");
   if (info[1] != 1.0)
     return set_err(c, ITP_E_EIGENSOLVER, "Jacobi eigensolver did not converge in " + std::to_string((int)info[0]) + " sweeps");
@@ -484,10 +490,31 @@ int itp_ctx_phase_times(itp_ctx* c, double* ms, long long* counts) {
   }
   return ITP_OK;
 }
+int itp_ctx_eig_stats(itp_ctx* c, long long* calls, long long* sweeps) {
+  if (!c) return ITP_E_INVALID;
+  if (calls) *calls = c->eig_calls;
+  if (sweeps) *sweeps = c->eig_sweeps;
+  return ITP_OK;
+}
+
+int itp_ctx_set_eig_inner(itp_ctx* c, int inner) {
+  if (!c || inner < 1 || inner > 64) return ITP_E_INVALID;
+  c->eig_inner = inner;
+  return ITP_OK;
+}
+
+int itp_ctx_set_eig_grid(itp_ctx* c, int workgroups) {
+  if (!c || workgroups < 1 || workgroups > 4096) return ITP_E_INVALID;
+  c->eig_grid = workgroups;
+  return ITP_OK;
+}
+
 int itp_ctx_reset_phase_times(itp_ctx* c) {
   if (!c) return ITP_E_INVALID;
   flush_events(c);
   for (int i = 0; i < ITP_NUM_PHASES; i++) { c->ms[i] = 0; c->cnt[i] = 0; }
+  c->eig_calls = 0;
+  c->eig_sweeps = 0;
   return ITP_OK;
 }
 
@@ -648,7 +675,8 @@ int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out
       const size_t qbytes = (size_t)(s->np / JS) * JS2 * 16;
       if ((rc = dalloc(c, (void**)&s->bjQ, qbytes))) return rc;
       if ((rc = dalloc(c, (void**)&s->bjD, qbytes))) return rc;
-      if ((rc = dalloc(c, (void**)&s->bjslots, 3 * kMaxSweeps * sizeof(double)))) return rc;
+      if ((rc = dalloc(c, (void**)&s->bjslots, 3 * (kMaxSweeps + 1) * sizeof(double)))) return rc;
+      if ((rc = dalloc(c, (void**)&s->bjident, (size_t)(s->np / JS) * sizeof(int)))) return rc;
     }
     {
       std::vector<cplx<double>> I((size_t)s->np * s->np, mk<double>(0.0, 0.0));
@@ -674,7 +702,7 @@ void itp_stateset_destroy(itp_stateset* s) {
   if (!s) return;
   (void)hipDeviceSynchronize();  // no parent access: parents may already be gone
   void* ptrs[] = {s->buf[0], s->buf[1], s->part, s->A[0], s->A[1], s->V[0], s->V[1], s->Vinit, s->bjQ, s->bjD,
-                  s->bjslots, s->C,
+                  s->bjslots, s->bjident, s->C,
                   s->lam, s->perm, s->info, s->bad, s->epart, s->dE, s->dmean};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);

@@ -336,50 +336,41 @@ hipError_t launch_jacobi(cplx<double>* A0, cplx<double>* A1, cplx<double>* V0, c
 __global__ __launch_bounds__(256) void block_jacobi_kernel(BlockJacobiArgs a) {
   cg::grid_group grid = cg::this_grid();
   extern __shared__ __align__(16) char bjl[];
-  BJLds s = bj_carve(bjl);
+  const int nt = (int)blockDim.x;
+  BJLds s = bj_carve(bjl, nt);
   GpuExec ex;
   const int np = a.np, nb = np / JB, P = np / JS;
   const int items = bj_items(np);
-  const long long gtid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long gthreads = (long long)gridDim.x * blockDim.x;
-  int cur = 0, sweeps = 0;
+  const long long tot = (long long)np * np;
+  const long long per = (tot + gridDim.x - 1) / gridDim.x;
+  const long long e0 = (long long)blockIdx.x * per;
+  const long long e1 = e0 + per < tot ? e0 + per : tot;
+  int sweeps = 0;
   bool conv = false;
-  for (int sw = 0; sw < a.max_sweeps; sw++) {
-    unsigned long long* slot = reinterpret_cast<unsigned long long*>(a.slots + 3 * sw);
+  for (int chk = 0;; chk++) {
+    // whole-matrix convergence check
+    unsigned long long* slot = reinterpret_cast<unsigned long long*>(a.slots + chk);
+    bj_check_rows(ex, a, e0, e1, s);
+    if (threadIdx.x == 0) atomicMax(slot, dbits(s.red[nt]));
+    grid.sync();
+    const double worst = bitsd(__hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    if (worst < 1.0) { conv = true; break; }
+    if (chk == a.max_sweeps) break;
+    // one sweep of nb-1 rounds
+    bj_reset_positions(ex, s, nb);
     for (int r = 0; r < nb - 1; r++) {
-      const cplx<double>* A = a.A[cur];
-      cplx<double>* An = a.A[cur ^ 1];
-      const cplx<double>* V = a.V[cur];
-      cplx<double>* Vn = a.V[cur ^ 1];
-      for (int k = blockIdx.x; k < P; k += gridDim.x) {
-        double o3[3];
-        bj_inner(ex, a, A, k, s, o3);
-        if (threadIdx.x == 0) {
-          atomicMax(slot + 0, dbits(o3[0]));
-          atomicMax(slot + 1, dbits(o3[1]));
-          atomicMax(slot + 2, dbits(o3[2]));
-        }
-      }
+      for (int k = blockIdx.x; k < P; k += gridDim.x) bj_inner(ex, a, k, s.pos2blk[2 * k], s.pos2blk[2 * k + 1], s);
       grid.sync();
-      for (int t = blockIdx.x; t < items; t += gridDim.x) bj_apply(ex, a, A, An, V, Vn, t, s);
+      for (int t = blockIdx.x; t < items; t += gridDim.x) bj_apply(ex, a, t, s);
       grid.sync();
-      cur ^= 1;
+      bj_next_positions(ex, s, nb);
     }
     sweeps++;
-    const double mo = bitsd(__hip_atomic_load(slot + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    const double mb = bitsd(__hip_atomic_load(slot + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    const double md = bitsd(__hip_atomic_load(slot + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    if (mo < a.tol || mb <= 1e-15 * md) { conv = true; break; }
-  }
-  if (cur == 1) {
-    const long long tot = (long long)np * np;
-    for (long long i = gtid; i < tot; i += gthreads) a.V[0][i] = a.V[1][i];
-    for (long long i = gtid; i < np; i += gthreads) a.A[0][i * np + i] = a.A[1][i * np + i];
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     a.info[0] = sweeps;
     a.info[1] = conv ? 1.0 : 0.0;
-    a.info[2] = cur;
+    a.info[2] = 0;
   }
 }
 
@@ -387,16 +378,15 @@ __global__ void fill_u64_kernel(unsigned long long* p, long long n) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = 0ull;
 }
 
-hipError_t launch_block_jacobi(cplx<double>* A0, cplx<double>* A1, cplx<double>* V0, cplx<double>* V1,
-                               cplx<double>* Q, cplx<double>* D, double* slots, int np, int max_sweeps,
-                               int max_inner, double tol, double* info, int num_cus, hipStream_t st) {
-  if (np % JS) return hipErrorInvalidValue;
+hipError_t launch_block_jacobi(cplx<double>* A, cplx<double>* V, cplx<double>* Q, cplx<double>* D, int* ident,
+                               double* slots, int np, int max_sweeps, int max_inner, double tol, double* info,
+                               int grid_wgs, hipStream_t st) {
+  if (np % JS || np / JB > kMaxBlocks) return hipErrorInvalidValue;
   BlockJacobiArgs a;
-  a.A[0] = A0; a.A[1] = A1; a.V[0] = V0; a.V[1] = V1;
-  a.Q = Q; a.D = D; a.slots = slots; a.info = info;
+  a.A = A; a.V = V; a.Q = Q; a.D = D; a.ident = ident; a.slots = slots; a.info = info;
   a.np = np; a.max_sweeps = max_sweeps; a.max_inner = max_inner; a.tol = tol;
   hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(256), 0, st, reinterpret_cast<unsigned long long*>(slots),
-                     (long long)3 * max_sweeps);
+                     (long long)(max_sweeps + 1));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const size_t lds = bj_lds_bytes(256);
@@ -404,9 +394,11 @@ hipError_t launch_block_jacobi(cplx<double>* A0, cplx<double>* A1, cplx<double>*
   e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, block_jacobi_kernel, 256, lds);
   if (e != hipSuccess) return e;
   if (per_cu < 1) return hipErrorInvalidValue;
-  const int want = std::max(np / JS, bj_items(np));
-  int grid = std::min<long long>((long long)per_cu * num_cus, std::min(want, num_cus));
-  if (grid < 1) grid = 1;
+  int dev = 0, ncu = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return hipErrorInvalidValue;
+  int grid = std::min(grid_wgs, per_cu * ncu);
+  grid = std::max(1, std::min(grid, bj_items(np)));
   void* args[] = {&a};
   e = hipLaunchCooperativeKernel((const void*)block_jacobi_kernel, dim3(grid), dim3(256), args, (unsigned)lds, st);
   if (e != hipSuccess) return e;

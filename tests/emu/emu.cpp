@@ -194,48 +194,57 @@ int emu_hamiltonian(int sx, int sy, double dx, int bt, double B, const double* V
   });
 }
 
-// Block Jacobi eigensolver phases (block_jacobi.hpp) run in order on the CPU.
+// Block Jacobi eigensolver phases (block_jacobi.hpp) run in kernel order on
+// the CPU, one emulated workgroup at a time (32 workgroups).
 // A: n x n complex column-major (upper triangle read). Outputs ascending
 // eigenvalues w and eigenvectors (columns) in A. Returns sweeps done, or -1.
-int emu_block_jacobi(int n, double* Ain, double* w, int max_inner) {
+int emu_block_jacobi_dbg(int n, double* Ain, double* w, int max_inner, int max_sweeps, double* Aout, double* Vout);
+int emu_block_jacobi(int n, double* Ain, double* w, int max_inner) { return emu_block_jacobi_dbg(n, Ain, w, max_inner, 40, nullptr, nullptr); }
+int emu_block_jacobi_dbg(int n, double* Ain, double* w, int max_inner, int max_sweeps, double* Aout, double* Vout) {
   const int np = (n + 63) / 64 * 64;
-  std::vector<cplx<double>> A0((size_t)np * np, mk<double>(0.0, 0.0)), A1(A0.size()), V0(A0.size()), V1(A0.size());
+  std::vector<cplx<double>> A((size_t)np * np, mk<double>(0.0, 0.0)), V(A.size(), mk<double>(0.0, 0.0));
   const cplx<double>* a = reinterpret_cast<const cplx<double>*>(Ain);
   for (int i = 0; i < n; i++)
-    for (int j = 0; j < n; j++) A0[(size_t)i * np + j] = (i <= j) ? a[(size_t)j * n + i] : conj(a[(size_t)i * n + j]);
-  for (int i = 0; i < n; i++) A0[(size_t)i * np + i].y = 0;
-  for (int i = 0; i < np; i++) V0[(size_t)i * np + i] = mk<double>(1.0, 0.0);
-  const int P = np / JS, nb = np / JB;
+    for (int j = 0; j < n; j++) A[(size_t)i * np + j] = (i <= j) ? a[(size_t)j * n + i] : conj(a[(size_t)i * n + j]);
+  for (int i = 0; i < n; i++) A[(size_t)i * np + i].y = 0;
+  for (int i = 0; i < np; i++) V[(size_t)i * np + i] = mk<double>(1.0, 0.0);
+  const int P = np / JS, nb = np / JB, G = 32;
   std::vector<cplx<double>> Q((size_t)P * JS2), D(Q.size());
-  std::vector<double> slots(3 * 64, 0.0);
+  std::vector<int> ident(P, 1);
   BlockJacobiArgs args;
-  args.A[0] = A0.data(); args.A[1] = A1.data(); args.V[0] = V0.data(); args.V[1] = V1.data();
-  args.Q = Q.data(); args.D = D.data(); args.slots = slots.data(); args.info = nullptr;
-  args.np = np; args.max_sweeps = 40; args.max_inner = max_inner; args.tol = 1e-14;
-  std::vector<char> lds(bj_lds_bytes(256));
-  BJLds ls = bj_carve(lds.data());
+  args.A = A.data(); args.V = V.data(); args.Q = Q.data(); args.D = D.data(); args.ident = ident.data();
+  args.slots = nullptr; args.info = nullptr;
+  args.np = np; args.max_sweeps = max_sweeps; args.max_inner = max_inner; args.tol = 1e-14;
+  // one LDS image per emulated workgroup (each keeps its own position table)
+  std::vector<std::vector<char>> lds(G, std::vector<char>(bj_lds_bytes(256)));
+  std::vector<BJLds> ls;
+  for (int g = 0; g < G; g++) ls.push_back(bj_carve(lds[g].data(), 256));
   CpuExec ex;
-  int cur = 0, sweeps = 0;
+  const long long tot = (long long)np * np, per = (tot + G - 1) / G;
+  int sweeps = 0;
   bool conv = false;
-  for (int sw = 0; sw < args.max_sweeps && !conv; sw++) {
-    double mx[3] = {0, 0, 0};
+  for (int chk = 0;; chk++) {
+    double worst = 0;
+    for (int g = 0; g < G; g++) {
+      const long long e0 = g * per, e1 = std::min(tot, e0 + per);
+      bj_check_rows(ex, args, e0, e1, ls[g]);
+      worst = std::max(worst, ls[g].red[256]);
+    }
+    if (worst < 1.0) { conv = true; break; }
+    if (chk == args.max_sweeps) break;
+    for (int g = 0; g < G; g++) bj_reset_positions(ex, ls[g], nb);
     for (int r = 0; r < nb - 1; r++) {
-      const cplx<double>* A = args.A[cur];
-      cplx<double>* An = args.A[cur ^ 1];
-      for (int k = 0; k < P; k++) {
-        double o3[3];
-        bj_inner(ex, args, A, k, ls, o3);
-        for (int q = 0; q < 3; q++) mx[q] = std::max(mx[q], o3[q]);
-      }
-      for (int t = 0; t < bj_items(np); t++) bj_apply(ex, args, A, An, args.V[cur], args.V[cur ^ 1], t, ls);
-      cur ^= 1;
+      for (int g = 0; g < G; g++)
+        for (int k = g; k < P; k += G) bj_inner(ex, args, k, ls[g].pos2blk[2 * k], ls[g].pos2blk[2 * k + 1], ls[g]);
+      for (int g = 0; g < G; g++)
+        for (int t = g; t < bj_items(np); t += G) bj_apply(ex, args, t, ls[g]);
+      for (int g = 0; g < G; g++) bj_next_positions(ex, ls[g], nb);
     }
     sweeps++;
-    if (mx[0] < args.tol || mx[1] <= 1e-15 * mx[2]) conv = true;
   }
+  if (Aout) std::copy((double*)A.data(), (double*)A.data() + 2 * A.size(), Aout);
+  if (Vout) std::copy((double*)V.data(), (double*)V.data() + 2 * V.size(), Vout);
   if (!conv) return -1;
-  const cplx<double>* A = args.A[cur];
-  const cplx<double>* V = args.V[cur];
   std::vector<int> idx(n);
   for (int i = 0; i < n; i++) idx[i] = i;
   std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return A[(size_t)x * np + x].x < A[(size_t)y * np + y].x; });
