@@ -62,6 +62,19 @@ def cpu_baseline(cfg, V, per_solve_iters, per_solve_converged, sample_steps):
                       f"{per_solve_iters} iterations and {per_solve_converged} converged states"}
 
 
+def copy_probe(mib):
+    """Three device-to-device copies of a known size: the per-dispatch FETCH_SIZE and WRITE_SIZE
+    they read give the factor that converts those counters to bytes on this device."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4, "dispatches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,11 +84,20 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--cpu-sample-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--iterations", type=int, default=0,
+                    help="profiling mode: run this many ITP iterations of one solve and print the phase times "
+                         "instead of the benchmark line")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="profiling mode: run 3 plain device copies of this many MiB (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters against a known byte count) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the single-GPU throughput is the reported value)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe_mib:
+        copy_probe(args.copy_probe_mib)
+        return
 
     import itp2d_amd as I
     from itp2d_amd import workloads as W
@@ -87,6 +109,16 @@ def main():
     ctx = I.Context(0)
     log(f"workload {args.workload}: {cfg['sx']}x{cfg['sy']}, N={N}, h={cfg['halforder']}, B={cfg['B']}, "
         f"{args.precision}; CUs={ctx.num_cus}")
+
+    if args.iterations:
+        s = I.ITPSystem(ctx, cfg, V)
+        ctx.enable_timing(True)
+        ctx.reset_phase_times()
+        for _ in range(args.iterations):
+            s.step()
+        ctx.synchronize()
+        print(json.dumps({"iterations": args.iterations, "phase_ms": ctx.phase_times()}), flush=True)
+        return
 
     for w in range(args.warmup):
         s = I.ITPSystem(ctx, cfg, V)
