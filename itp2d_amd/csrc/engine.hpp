@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "line_ops.hpp"
+#include "reg_sweep.hpp"
 
 namespace itp {
 
@@ -272,12 +273,29 @@ struct LineGeom {
   const cplx<double>* tw_y = nullptr;
   const cplx<double>* mk_y = nullptr;
   int L_row = 4, L_col = 4;
+  bool use_reg = true;  // register-resident engine where the line length allows it
 };
 
 inline int pick_L(int lines, int pref) {
   int L = pref;
   while (L > 1 && lines % L) L /= 2;
   return L;
+}
+
+// Engine and lines per workgroup of a sweep orientation (0 rows, 1 columns).
+inline bool sweep_uses_reg(const GridInfo& g, const LineGeom& lg, int orient) {
+  const int n = orient == 0 ? g.sx : g.sy;
+  const int lines = orient == 0 ? g.sy : g.sx;
+  return lg.use_reg && reg_supported(n) && lines % reg_lines_per_wg(n) == 0;
+}
+inline int sweep_lines_per_block(const GridInfo& g, const LineGeom& lg, int orient) {
+  const int n = orient == 0 ? g.sx : g.sy;
+  if (sweep_uses_reg(g, lg, orient)) return reg_lines_per_wg(n);
+  return orient == 0 ? pick_L(g.sy, lg.L_row) : pick_L(g.sx, lg.L_col);
+}
+inline int sweep_blocks(const GridInfo& g, const LineGeom& lg, int orient) {
+  const int lines = orient == 0 ? g.sy : g.sx;
+  return lines / sweep_lines_per_block(g, lg, orient);
 }
 
 // Fill SweepArgs for a sweep over nfields fields.
@@ -289,8 +307,11 @@ SweepArgs<R> make_args(const GridInfo& g, const LineGeom& lg, const Sweep<R>& s,
   a.orient = s.orient;
   a.sx = g.sx; a.sy = g.sy;
   a.fstride = fstride;
-  if (s.orient == 0) { a.L = pick_L(g.sy, lg.L_row); a.plan = lg.plan_x; a.tw = lg.tw_x; a.mk = lg.mk_x; nblk = g.sy / a.L; }
-  else { a.L = pick_L(g.sx, lg.L_col); a.plan = lg.plan_y; a.tw = lg.tw_y; a.mk = lg.mk_y; nblk = g.sx / a.L; }
+  if (s.orient == 0) { a.plan = lg.plan_x; a.tw = lg.tw_x; a.mk = lg.mk_x; }
+  else { a.plan = lg.plan_y; a.tw = lg.tw_y; a.mk = lg.mk_y; }
+  a.reg = sweep_uses_reg(g, lg, s.orient) ? 1 : 0;
+  a.L = sweep_lines_per_block(g, lg, s.orient);
+  nblk = sweep_blocks(g, lg, s.orient);
   if ((int)s.ops.size() > kMaxOps) throw std::runtime_error("too many line ops in one sweep");
   a.nops = (int)s.ops.size();
   for (int i = 0; i < a.nops; i++) a.ops[i] = s.ops[i];
@@ -441,7 +462,7 @@ int hamiltonian_program(const GridInfo& g, const LineGeom& lg, Branch br, const 
     c.src = in; c.add1 = w1; c.vtab = V; c.vsrc = V ? in : nullptr;
     c.dotsrc = dotsrc; c.partials = partials; c.dst = out;
     prog.push_back(c);
-    return g.sx / pick_L(g.sx, lg.L_col);
+    return sweep_blocks(g, lg, 1);
   }
   const bool per = (br == BR_B0P);
   Sweep<R> r1; r1.orient = 0;
@@ -459,7 +480,7 @@ int hamiltonian_program(const GridInfo& g, const LineGeom& lg, Branch br, const 
   r2.src = w1; r2.vtab = V; r2.vsrc = V ? in : nullptr;
   r2.dotsrc = dotsrc; r2.partials = partials; r2.dst = out;
   prog.push_back(r2);
-  return g.sy / pick_L(g.sy, lg.L_row);
+  return sweep_blocks(g, lg, 0);
 }
 
 }  // namespace itp

@@ -18,6 +18,7 @@
 #include "dense_ops.hpp"
 #include "gpu.hpp"
 #include "line_ops.hpp"
+#include "reg_sweep.hpp"
 
 namespace itp {
 
@@ -40,8 +41,59 @@ __global__ __launch_bounds__(256) void sweep_kernel(SweepArgs<R> a) {
   sweep_program<R>(ex, a, s, (int)blockIdx.y, (int)blockIdx.x);
 }
 
+// Per-thread state of the register engine lives in the exec object, so the
+// same pass code runs per thread on the GPU and per emulated thread on the CPU.
+// The thread index is passed through opaque() in every pass: addresses and
+// table loads derived from it then stay inside their pass instead of being
+// hoisted out of the op loop, which would pin registers for the whole kernel.
+template <class S>
+struct GpuRegExec {
+  S st;
+  template <class F>
+  __device__ void run(F&& f) {
+    f(opaque((int)threadIdx.x), st);
+    __syncthreads();
+  }
+  template <class F>
+  __device__ void local(F&& f) {
+    f(opaque((int)threadIdx.x), st);
+  }
+};
+
+template <class R, int N, int ORIENT>
+__global__ __launch_bounds__(256) void reg_sweep_kernel(SweepArgs<R> a) {
+  extern __shared__ __align__(16) char lds_raw[];
+  GpuRegExec<RegState<R, RegShape<N>::Q>> ex;
+  reg_sweep_program<R, N, ORIENT>(ex, a, reinterpret_cast<cplx<R>*>(lds_raw), (int)blockIdx.y, (int)blockIdx.x);
+}
+
+template <class R>
+hipError_t launch_reg_sweep(const SweepArgs<R>& a, int nblk, int nfields, hipStream_t st) {
+  const int n = a.plan.n;
+  const size_t lds = reg_lds_bytes<R>(n);
+  if (lds > 160 * 1024 || a.L != reg_lines_per_wg(n)) return hipErrorInvalidValue;
+  const int lines = a.orient == 0 ? a.sy : a.sx;
+  if ((a.orient == 0 ? a.sx : a.sy) != n || nblk * a.L != lines) return hipErrorInvalidValue;
+  hipError_t err = hipSuccess;
+  const bool ok = reg_dispatch(n, a.orient, [&](auto nc, auto oc) {
+    constexpr int N = decltype(nc)::value, O = decltype(oc)::value;
+    static bool attr_set = false;
+    if (!attr_set) {
+      (void)hipFuncSetAttribute((const void*)reg_sweep_kernel<R, N, O>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+      attr_set = true;
+    }
+    hipLaunchKernelGGL((reg_sweep_kernel<R, N, O>), dim3(nblk, nfields), dim3(kRegThreads), lds, st, a);
+    err = hipGetLastError();
+  });
+  return ok ? err : hipErrorInvalidValue;
+}
+
 template <class R>
 hipError_t launch_sweep(const SweepArgs<R>& a, int nblk, int nfields, hipStream_t st) {
+  if (nblk <= 0 || nfields <= 0) return hipSuccess;
+  if (nfields > 65535) return hipErrorInvalidValue;
+  if (a.reg) return launch_reg_sweep<R>(a, nblk, nfields, st);
   const size_t lds = sweep_lds_bytes<R>(a.L, a.plan.n, 256);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
   static bool attr_set = false;

@@ -121,6 +121,7 @@ struct SweepArgs {
   const cplx<R>* dotsrc;    // partial[field][block] = sum conj(dotsrc) * result (may be null)
   cplx<double>* partials;
   int nblk;                 // line blocks per field (partials stride)
+  int reg;                  // 1: register-resident engine (reg_sweep.hpp), L = its lines per workgroup
 };
 
 // LDS layout for a sweep: three line buffers of L*(n+1) values, the two

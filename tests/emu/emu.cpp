@@ -19,9 +19,42 @@ struct CpuExec {
   template <class F> void run(F&& f) { for (int t = 0; t < nthr; t++) f(t, nthr); }
 };
 
+// Register engine: per-thread register state kept per emulated thread.
+template <class S>
+struct CpuRegExec {
+  std::vector<S> st;
+  CpuRegExec() : st(kRegThreads) {}
+  template <class F> void run(F&& f) { for (int t = 0; t < kRegThreads; t++) f(t, st[t]); }
+  template <class F> void local(F&& f) { run(f); }
+};
+
+bool g_use_reg = true;
+
 struct CpuBackend {
   template <class R>
   void sweep(const SweepArgs<R>& a, int nblk, int nfields) {
+    if (a.reg) {
+      const int n = a.plan.n;
+      const int lines = a.orient == 0 ? a.sy : a.sx;
+      if (a.L != reg_lines_per_wg(n) || nblk * a.L != lines || (a.orient == 0 ? a.sx : a.sy) != n)
+        throw std::runtime_error("register sweep shape");
+      const bool ok = reg_dispatch(n, a.orient, [&](auto nc, auto oc) {
+        constexpr int N = decltype(nc)::value, O = decltype(oc)::value;
+        const size_t bytes = reg_lds_bytes<R>(N);
+#pragma omp parallel
+        {
+          std::vector<cplx<R>> lds(bytes / sizeof(cplx<R>) + 1);
+#pragma omp for collapse(2) schedule(static)
+          for (int f = 0; f < nfields; f++)
+            for (int b = 0; b < nblk; b++) {
+              CpuRegExec<RegState<R, RegShape<N>::Q>> ex;
+              reg_sweep_program<R, N, O>(ex, a, lds.data(), f, b);
+            }
+        }
+      });
+      if (!ok) throw std::runtime_error("register sweep length");
+      return;
+    }
     const size_t bytes = sweep_lds_bytes<R>(a.L, a.plan.n, 256);
 #pragma omp parallel
     {
@@ -44,6 +77,7 @@ struct Env {
   Env(int sx, int sy, double dx, int bt) {
     g = make_grid(sx, sy, dx, bt);
     if (!make_plan(sx, lg.plan_x) || !make_plan(sy, lg.plan_y)) throw std::runtime_error("plan");
+    lg.use_reg = g_use_reg;
     twiddle_tables(sx, twx, mkx);
     twiddle_tables(sy, twy, mky);
     lg.tw_x = twx.data(); lg.mk_x = mkx.data(); lg.tw_y = twy.data(); lg.mk_y = mky.data();
@@ -66,6 +100,10 @@ void from_r(const std::vector<cplx<R>>& v, double* d) {
 }  // namespace
 
 extern "C" {
+
+// 1: sweeps use the register engine where the line length allows it (default);
+// 0: always the LDS-resident engine.
+void emu_set_reg(int on) { g_use_reg = on != 0; }
 
 int emu_plan(int n, int* radix, int* nst) {
   LinePlan p;
@@ -104,6 +142,23 @@ int emu_expkinetic(int sx, int sy, double dx, int bt, double B, double eps, doub
     std::vector<cplx<double>> w(f.size());
     std::vector<Sweep<double>> prog;
     expkinetic_program<double>(branch_of(B, bt), k, f.data(), w.data(), prog);
+    run_program<double>(be, e.g, e.lg, prog, nfields);
+    from_r<double>(f, data);
+    return 0;
+  });
+}
+
+// ExpPotential alone (exppotential.hpp:56-65): one row sweep of elementwise ops.
+int emu_exppotential(int sx, int sy, double dx, int bt, const double* V, double eps, double coeff, double pref,
+                     double* data, int nfields) {
+  return guard([&] {
+    Env e(sx, sy, dx, bt);
+    CpuBackend be;
+    std::vector<double> vals;
+    if (V) exppot_values(e.g, V, eps, coeff, pref, vals);
+    auto f = to_r<double>(data, (size_t)nfields * e.g.M);
+    std::vector<Sweep<double>> prog;
+    exppotential_program<double>(V ? vals.data() : nullptr, pref, f.data(), prog);
     run_program<double>(be, e.g, e.lg, prog, nfields);
     from_r<double>(f, data);
     return 0;

@@ -25,7 +25,15 @@ def lib():
                                          C.c_int, _dp, _dp, C.c_int]
         _lib.emu_plan.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         _lib.emu_block_jacobi.argtypes = [C.c_int, _dp, _dp, C.c_int]
+        _lib.emu_set_reg.argtypes = [C.c_int]
+        _lib.emu_exppotential.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, _dp, C.c_double, C.c_double,
+                                          C.c_double, _dp, C.c_int]
     return _lib
+
+
+def set_reg(on):
+    """Select the register-resident sweep engine (True, default) or the LDS-resident one."""
+    lib().emu_set_reg(1 if on else 0)
 
 
 def _p(a):
@@ -51,6 +59,14 @@ def expkinetic(fields, dx, bt, B, e, coeff=-1.0, pref=1.0):
     f = np.ascontiguousarray(fields, dtype=np.complex128).copy()
     n, sy, sx = f.shape
     assert lib().emu_expkinetic(sx, sy, dx, bt, B, e, coeff, pref, _p(f.view(np.float64)), n) == 0
+    return f
+
+
+def exppotential(fields, dx, bt, V, e, coeff=-0.5, pref=1.0):
+    f = np.ascontiguousarray(fields, dtype=np.complex128).copy()
+    n, sy, sx = f.shape
+    Vc = None if V is None else np.ascontiguousarray(V, dtype=np.float64)
+    assert lib().emu_exppotential(sx, sy, dx, bt, _p(Vc), e, coeff, pref, _p(f.view(np.float64)), n) == 0
     return f
 
 

@@ -14,14 +14,23 @@ def rel(a, b):
     return np.abs(a - b).max() / np.abs(b).max()
 
 
+@pytest.fixture(params=["reg", "lds"])
+def engine(request):
+    """Run a test on the register-resident sweep engine and on the LDS-resident one."""
+    E.set_reg(request.param == "reg")
+    yield request.param
+    E.set_reg(True)
+
+
 def test_radix_plans_even():
     for n in [8, 16, 40, 64, 128, 192, 256, 512, 1024]:
         p = E.plan(n)
         assert p is not None and len(p) % 2 == 0 and int(np.prod(p)) == n and max(p) <= 8
 
 
-@pytest.mark.parametrize("sx,sy", [(16, 16), (64, 64), (40, 40), (192, 64), (64, 128)])
-def test_transforms(sx, sy):
+@pytest.mark.parametrize("sx,sy", [(16, 16), (64, 64), (40, 40), (192, 64), (64, 128), (32, 256), (256, 32),
+                                   (512, 64), (1024, 32)])
+def test_transforms(sx, sy, engine):
     rng = np.random.default_rng(sx * sy)
     f = rng.standard_normal((2, sy, sx)) + 1j * rng.standard_normal((2, sy, sx))
     for k, name in enumerate(O.TRANSFORMS):
@@ -43,10 +52,12 @@ BRANCHES = [(0, 0.0), (1, 0.0), (0, 1.3), (1, 1.3)]
 
 
 @pytest.mark.parametrize("bt,B", BRANCHES)
-@pytest.mark.parametrize("sx,sy", [(64, 64), (40, 40), (64, 128)])
-def test_operators(bt, B, sx, sy):
+@pytest.mark.parametrize("sx,sy", [(64, 64), (40, 40), (64, 128), (256, 32)])
+def test_operators(bt, B, sx, sy, engine):
     dx, V, f = _setup(sx, sy)
     assert rel(E.expkinetic(f, dx, bt, B, 0.07, -1.0, 0.9), O.apply_expkinetic(f, dx, bt, B, 0.07, -1.0, 0.9)) < 1e-14
+    assert rel(E.exppotential(f, dx, bt, V, 0.07, -0.5, 0.8), O.apply_exppotential(f, dx, V, 0.07, -0.5, 0.8)) < 1e-14
+    assert rel(E.exppotential(f, dx, bt, None, 0.07, -0.5, 0.8), O.apply_exppotential(f, dx, None, 0.07, -0.5, 0.8)) < 1e-14
     for ex in (1, 2, 3):
         a = E.split(f, dx, bt, B, V, 0.05, 0.7, ex)
         b = O.apply_secondorder(f, dx, bt, B, V, 0.05, 0.7, ex)
@@ -57,8 +68,9 @@ def test_operators(bt, B, sx, sy):
 
 
 @pytest.mark.parametrize("bt,B", BRANCHES)
-def test_hamiltonian_and_energies(bt, B):
-    dx, V, f = _setup(64, 64)
+@pytest.mark.parametrize("sx,sy", [(64, 64), (256, 32)])
+def test_hamiltonian_and_energies(bt, B, sx, sy, engine):
+    dx, V, f = _setup(sx, sy)
     hf, Ee, se = E.hamiltonian(f, dx, bt, B, V)
     assert rel(hf, O.apply_hamiltonian(f, dx, bt, B, V)) < 1e-14
     Eo, so, _, _ = O.energies(f, dx, bt, B, V)
@@ -70,7 +82,7 @@ def test_hamiltonian_and_energies(bt, B):
 
 
 @pytest.mark.parametrize("bt,B", BRANCHES)
-def test_fp32_propagation_tolerance(bt, B):
+def test_fp32_propagation_tolerance(bt, B, engine):
     """complex-FP32 fields: one MPS step within FP32 rounding of the FP64 result."""
     dx, V, f = _setup(64, 64)
     a = E.split(f, dx, bt, B, V, 0.05, halforder=2, fp32=True)
