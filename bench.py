@@ -130,6 +130,7 @@ def main():
     systems = [I.ITPSystem(ctx, cfg, V) for _ in range(args.steps)]
     ctx.enable_timing(True)
     ctx.reset_phase_times()
+    eig0, oa0 = ctx.eig_stats(), ctx.oa_stats()
     ctx.synchronize()
     t0 = time.perf_counter()
     for k, s in enumerate(systems):
@@ -138,6 +139,7 @@ def main():
     wall = time.perf_counter() - t0
     phases = ctx.phase_times()
     ctx.enable_timing(False)
+    eig1, oa1 = ctx.eig_stats(), ctx.oa_stats()
 
     iters = [s.total_steps for s in systems]
     conv = [s.num_converged for s in systems]
@@ -201,6 +203,8 @@ def main():
             "E_lowest": [round(x, 10) for x in E0[:4].tolist()],
             "phase_ms": {k: round(v[0], 3) for k, v in phases.items()},
             "phase_calls": {k: v[1] for k, v in phases.items()},
+            "eigensolver": {"calls": eig1[0] - eig0[0], "jacobi_sweeps": eig1[1] - eig0[1],
+                            "refined": oa1[0] - oa0[0], "fallbacks": oa1[1] - oa0[1]},
         },
         "roofline": roofline,
         "roofline_other": others,

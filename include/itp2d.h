@@ -171,6 +171,14 @@ int itp_ctx_eig_stats(itp_ctx* ctx, long long* calls, long long* sweeps);
 int itp_ctx_set_eig_inner(itp_ctx* ctx, int inner);
 /* Workgroups of the cooperative eigensolver launch (default 64). */
 int itp_ctx_set_eig_grid(itp_ctx* ctx, int workgroups);
+/* Eigensolver of the overlap matrix (replaces zheev, eigensolver.hpp:80-84):
+ * 0 (default) block Jacobi sweeps until every unconverged off-diagonal element
+ * is below a quarter of its diagonal gap, then the refinement iteration
+ * X <- X + X E on n x n matrix products (falls back to plain Jacobi if it does
+ * not converge); 1 block Jacobi only. */
+int itp_ctx_set_eig_mode(itp_ctx* ctx, int mode);
+/* Calls finished by the refinement iteration and calls that fell back to Jacobi. */
+int itp_ctx_oa_stats(itp_ctx* ctx, long long* refined, long long* fallbacks);
 
 #ifdef __cplusplus
 }

@@ -83,6 +83,8 @@ def lib():
             "itp_ctx_eig_stats": [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
             "itp_ctx_set_eig_inner": [_vp, C.c_int],
             "itp_ctx_set_eig_grid": [_vp, C.c_int],
+            "itp_ctx_set_eig_mode": [_vp, C.c_int],
+            "itp_ctx_oa_stats": [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
             "itp_grid_create": [_vp, _sz, _sz, C.c_double, C.c_int, C.POINTER(_vp)],
             "itp_grid_destroy": [_vp],
             "itp_grid_kvalues": [_vp, _dp, _dp, _dp, _dp],
@@ -205,6 +207,16 @@ class Context:
 
     def set_eig_grid(self, workgroups):
         self.check(lib().itp_ctx_set_eig_grid(self.h, int(workgroups)), "set_eig_grid")
+
+    def set_eig_mode(self, mode):
+        """0: Jacobi handing off to the refinement iteration (default); 1: Jacobi only."""
+        self.check(lib().itp_ctx_set_eig_mode(self.h, int(mode)), "set_eig_mode")
+
+    def oa_stats(self):
+        """(calls finished by the refinement iteration, calls that fell back to Jacobi)."""
+        a, b = C.c_longlong(0), C.c_longlong(0)
+        lib().itp_ctx_oa_stats(self.h, C.byref(a), C.byref(b))
+        return a.value, b.value
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

@@ -33,12 +33,14 @@ struct BlockJacobiArgs {
   cplx<double>* Q;       // [nb/2][JS][JS] rotations of the round
   cplx<double>* D;       // [nb/2][JS][JS] diagonalised sub-blocks of the round
   int* ident;            // [nb/2] 1 if Q_k is the identity
-  double* slots;         // [max_sweeps + 1]: largest element ratio of each whole-matrix check (atomic max)
-  double* info;          // [0] sweeps, [1] converged
+  double* slots;         // [2 (max_sweeps + 1)]: per whole-matrix check, the largest element ratio,
+                         // then (second half) the largest coupling-to-gap ratio (atomic max)
+  double* info;          // [0] sweeps, [1] 1 converged / 2 handed off / 0 neither
   int np;
   int max_sweeps;
   int max_inner;
   double tol;
+  double handoff;        // stop early once every unconverged |a_ij| < handoff |a_ii - a_jj| (0: never)
 };
 
 // LDS for one workgroup: 4 JS x JS complex matrices + rotation parameters +
@@ -101,16 +103,37 @@ ITP_HD double bj_ratio(cplx<double> aij, double aii, double ajj, double tol, dou
   return den > 0 ? ab / den : (ab > 0 ? 1e300 : 0.0);
 }
 
-// Block reduction of per-thread maxima red[tid] into red[nt] (one thread, fixed order).
+// Tree reduction of the per-thread maxima v[0..nt) (nt a power of two; v is
+// overwritten) into *out.
+template <class Exec>
+ITP_HD void bj_tree_max(Exec& ex, double* v, double* out) {
+  int half = 0;
+  ex.run([&](int tid, int nt) { (void)tid; half = nt / 2; });
+  for (; half >= 1; half /= 2) {
+    const int h = half;
+    ex.run([&](int tid, int nt) {
+      (void)nt;
+      if (tid < h) v[tid] = v[tid + h] > v[tid] ? v[tid + h] : v[tid];
+      if (h == 1 && tid == 0) *out = v[0];
+    });
+  }
+}
+
+// Block reduction of per-thread maxima red[tid] into red[nt].
 template <class Exec>
 ITP_HD void bj_reduce_max(Exec& ex, const BJLds& s) {
-  ex.run([&](int tid, int nt) {
-    if (tid == 0) {
-      double m = 0;
-      for (int t = 0; t < nt; t++) m = s.red[t] > m ? s.red[t] : m;
-      s.red[nt] = m;
-    }
-  });
+  int nt0 = 0;
+  ex.run([&](int tid, int nt) { (void)tid; nt0 = nt; });
+  bj_tree_max(ex, s.red, s.red + nt0);
+}
+
+// Coupling-to-gap ratio of an unconverged off-diagonal element,
+// |a_ij| / |a_ii - a_jj|: small for every pair once the matrix is close
+// enough to diagonal for first-order (perturbative) eigenvector corrections.
+ITP_HD double bj_gap_ratio(cplx<double> aij, double aii, double ajj) {
+  const double ab = std::hypot(aij.x, aij.y);
+  const double gap = std::fabs(aii - ajj);
+  return gap > 0 ? ab / gap : 1e300;
 }
 
 // Phase 1 for block pair k (blocks b1, b2): writes Q_k, D_k and ident[k].
@@ -339,7 +362,8 @@ ITP_HD void bj_reset_positions(Exec& ex, const BJLds& s, int nb) {
 
 // Whole-matrix check, elements [e0, e1) of this workgroup's slice: every
 // workgroup first derives the same floor from the whole diagonal, then the
-// largest element ratio of its slice lands in red[nt].
+// largest element ratio of its slice lands in red[nt] and the largest
+// coupling-to-gap ratio of its unconverged elements in red[nt + 1].
 template <class Exec>
 ITP_HD void bj_check_rows(Exec& ex, const BlockJacobiArgs& a, long long e0, long long e1, const BJLds& s) {
   const int np = a.np;
@@ -357,17 +381,28 @@ ITP_HD void bj_check_rows(Exec& ex, const BlockJacobiArgs& a, long long e0, long
     if (tid == 0) s.floor_[0] = (double)np * 2.220446049250313e-16 * s.red[nt];
   });
   const double fl = s.floor_[0];
+  double* rho = s.red + 0;
+  ex.run([&](int tid, int nt) { rho = s.red + nt + 2; });
   ex.run([&](int tid, int nt) {
-    double r = 0;
+    double r = 0, g = 0;
     for (long long e = e0 + tid; e < e1; e += nt) {
       const int i = (int)(e / np), j = (int)(e - (long long)i * np);
       if (j > i) {
-        const double q = bj_ratio(A[e], A[(long long)i * np + i].x, A[(long long)j * np + j].x, a.tol, fl);
+        const double aii = A[(long long)i * np + i].x, ajj = A[(long long)j * np + j].x;
+        const double q = bj_ratio(A[e], aii, ajj, a.tol, fl);
         r = q > r ? q : r;
+        if (q >= 1.0) {
+          const double gr = bj_gap_ratio(A[e], aii, ajj);
+          g = gr > g ? gr : g;
+        }
       }
     }
     s.red[tid] = r;
+    rho[tid] = g;
   });
+  int nt0 = 0;
+  ex.run([&](int tid, int nt) { (void)tid; nt0 = nt; });
+  bj_tree_max(ex, rho, s.red + nt0 + 1);
   bj_reduce_max(ex, s);
 }
 

@@ -37,6 +37,8 @@ struct itp_ctx {
   long long eig_calls = 0, eig_sweeps = 0;
   int eig_inner = 1;   // element sweeps per block sub-problem and round (upper bound)
   int eig_grid = 64;   // workgroups of the eigensolver (its grid barrier cost grows with them)
+  int eig_mode = 0;    // 0: Jacobi handing off to the refinement iteration, 1: Jacobi only
+  long long oa_done = 0, oa_fallback = 0;
 };
 
 struct itp_grid {
@@ -71,6 +73,10 @@ struct itp_stateset {
   cplx<double>* bjD = nullptr;
   double* bjslots = nullptr;
   int* bjident = nullptr;
+  cplx<double>* oa_work = nullptr;  // refinement iteration: Y, S, G, E (np x np each)
+  double* oa_lam = nullptr;
+  double* oa_slots = nullptr;
+  int* oa_state = nullptr;
   cplx<double>* C = nullptr;
   double* lam = nullptr;
   int* perm = nullptr;
@@ -108,6 +114,10 @@ struct itp_hamiltonian {
 namespace {
 
 constexpr int kMaxSweeps = 40;
+// The Jacobi sweeps hand the matrix to the refinement iteration once every
+// unconverged off-diagonal element is below this fraction of its diagonal gap.
+constexpr double kHandoff = 0.25;
+constexpr int kOAMaxIter = 8;
 
 int set_err(itp_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -336,22 +346,47 @@ int eigen_stage(itp_stateset* s, size_t* bad_n, double* bad_lambda, bool scale) 
   itp_ctx* c = s->grid->ctx;
   const int N = (int)s->N;
   Phase pe(c, ITP_PHASE_EIGENSOLVE);
-  HIPCK(c, hipMemcpyAsync(s->V[0], s->Vinit, (size_t)s->np * s->np * 16, hipMemcpyDeviceToDevice, c->stream));
+  const size_t np2 = (size_t)s->np * s->np * 16;
+  const bool hybrid = c->eig_mode == 0;
+  // Jacobi sweeps on A (in place, rotations into V); in hybrid mode they stop
+  // as soon as the refinement iteration can take over, which works on the
+  // untouched copy A[1] starting from the Jacobi eigenvectors.
+  HIPCK(c, hipMemcpyAsync(s->V[0], s->Vinit, np2, hipMemcpyDeviceToDevice, c->stream));
+  if (hybrid) HIPCK(c, hipMemcpyAsync(s->A[1], s->A[0], np2, hipMemcpyDeviceToDevice, c->stream));
   HIPCK(c, launch_block_jacobi(s->A[0], s->V[0], s->bjQ, s->bjD, s->bjident, s->bjslots, s->np, kMaxSweeps,
-                               c->eig_inner, 1e-14, s->info, c->eig_grid, c->stream));
+                               c->eig_inner, 1e-14, hybrid ? kHandoff : 0.0, s->info, c->eig_grid, c->stream));
+  if (hybrid)
+    HIPCK(c, launch_oa(s->A[1], s->V[0], s->V[1], s->oa_work, s->oa_lam, s->oa_slots, s->oa_state, s->info, s->np, N,
+                       kOAMaxIter, s->A[0], c->stream));
   HIPCK(c, launch_eig_finish(s->A[0], s->V[0], s->np, N, s->lam, s->perm, s->C, s->nc, scale ? 1 : 0, c->stream));
   double info[4];
+  int state[2] = {3, 0};
   int bad = 0;
   std::vector<double> lam(N);
   HIPCK(c, hipMemcpyAsync(info, s->info, sizeof(info), hipMemcpyDeviceToHost, c->stream));
+  if (hybrid) HIPCK(c, hipMemcpyAsync(state, s->oa_state, sizeof(state), hipMemcpyDeviceToHost, c->stream));
   HIPCK(c, hipMemcpyAsync(&bad, s->bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCK(c, hipMemcpyAsync(lam.data(), s->lam, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCK(c, hipStreamSynchronize(c->stream));
-  pe.stop();
   c->eig_calls++;
   c->eig_sweeps += (long long)info[0];
   if (bad) return set_err(c, ITP_E_EIGENSOLVER, "overlap matrix has non-finite entries");
-  if (info[1] != 1.0)
+  if (info[1] == 2.0 && state[0] == 1) c->oa_done++;
+  if (info[1] == 2.0 && state[0] != 1) {
+    // the refinement did not converge: plain Jacobi from the saved matrix
+    c->oa_fallback++;
+    HIPCK(c, hipMemcpyAsync(s->A[0], s->A[1], np2, hipMemcpyDeviceToDevice, c->stream));
+    HIPCK(c, hipMemcpyAsync(s->V[0], s->Vinit, np2, hipMemcpyDeviceToDevice, c->stream));
+    HIPCK(c, launch_block_jacobi(s->A[0], s->V[0], s->bjQ, s->bjD, s->bjident, s->bjslots, s->np, kMaxSweeps,
+                                 c->eig_inner, 1e-14, 0.0, s->info, c->eig_grid, c->stream));
+    HIPCK(c, launch_eig_finish(s->A[0], s->V[0], s->np, N, s->lam, s->perm, s->C, s->nc, scale ? 1 : 0, c->stream));
+    HIPCK(c, hipMemcpyAsync(info, s->info, sizeof(info), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipMemcpyAsync(lam.data(), s->lam, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    c->eig_sweeps += (long long)info[0];
+  }
+  pe.stop();
+  if (info[1] != 1.0 && !(info[1] == 2.0 && state[0] == 1))
     return set_err(c, ITP_E_EIGENSOLVER, "Jacobi eigensolver did not converge in " + std::to_string((int)info[0]) + " sweeps");
   for (int n = 0; n < N; n++) {
     const double ev = lam[n];
@@ -500,6 +535,19 @@ int itp_ctx_eig_stats(itp_ctx* c, long long* calls, long long* sweeps) {
 int itp_ctx_set_eig_inner(itp_ctx* c, int inner) {
   if (!c || inner < 1 || inner > 64) return ITP_E_INVALID;
   c->eig_inner = inner;
+  return ITP_OK;
+}
+
+int itp_ctx_set_eig_mode(itp_ctx* c, int mode) {
+  if (!c || mode < 0 || mode > 1) return ITP_E_INVALID;
+  c->eig_mode = mode;
+  return ITP_OK;
+}
+
+int itp_ctx_oa_stats(itp_ctx* c, long long* refined, long long* fallbacks) {
+  if (!c) return ITP_E_INVALID;
+  if (refined) *refined = c->oa_done;
+  if (fallbacks) *fallbacks = c->oa_fallback;
   return ITP_OK;
 }
 
@@ -677,6 +725,10 @@ int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out
       if ((rc = dalloc(c, (void**)&s->bjD, qbytes))) return rc;
       if ((rc = dalloc(c, (void**)&s->bjslots, 3 * (kMaxSweeps + 1) * sizeof(double)))) return rc;
       if ((rc = dalloc(c, (void**)&s->bjident, (size_t)(s->np / JS) * sizeof(int)))) return rc;
+      if ((rc = dalloc(c, (void**)&s->oa_work, 4 * np2))) return rc;
+      if ((rc = dalloc(c, (void**)&s->oa_lam, (size_t)s->np * sizeof(double)))) return rc;
+      if ((rc = dalloc(c, (void**)&s->oa_slots, kOAMaxIter * sizeof(double)))) return rc;
+      if ((rc = dalloc(c, (void**)&s->oa_state, 2 * sizeof(int)))) return rc;
     }
     {
       std::vector<cplx<double>> I((size_t)s->np * s->np, mk<double>(0.0, 0.0));
@@ -702,7 +754,7 @@ void itp_stateset_destroy(itp_stateset* s) {
   if (!s) return;
   (void)hipDeviceSynchronize();  // no parent access: parents may already be gone
   void* ptrs[] = {s->buf[0], s->buf[1], s->part, s->A[0], s->A[1], s->V[0], s->V[1], s->Vinit, s->bjQ, s->bjD,
-                  s->bjslots, s->bjident, s->C,
+                  s->bjslots, s->bjident, s->oa_work, s->oa_lam, s->oa_slots, s->oa_state, s->C,
                   s->lam, s->perm, s->info, s->bad, s->epart, s->dE, s->dmean};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);

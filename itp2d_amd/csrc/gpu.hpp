@@ -13,12 +13,12 @@ hipError_t launch_convert(const void* in, void* out, long long n, int to_f32, hi
 template <class R>
 hipError_t launch_gram(const cplx<R>* psi, long long M, int N, int nc, cplx<double>* part, int nchunk,
                        long long kchunk, double dx, cplx<double>* S, int np, int* bad, hipStream_t st);
-size_t jacobi_lds_bytes(int np);
-hipError_t launch_jacobi(cplx<double>* A0, cplx<double>* A1, cplx<double>* V0, cplx<double>* V1, int np,
-                         int max_sweeps, double tol, double* info, int num_cus, hipStream_t st);
 hipError_t launch_block_jacobi(cplx<double>* A, cplx<double>* V, cplx<double>* Q, cplx<double>* D, int* ident,
-                               double* slots, int np, int max_sweeps, int max_inner, double tol, double* info,
-                               int grid_wgs, hipStream_t st);
+                               double* slots, int np, int max_sweeps, int max_inner, double tol, double handoff,
+                               double* info, int grid_wgs, hipStream_t st);
+hipError_t launch_oa(const cplx<double>* A0, cplx<double>* X0, cplx<double>* X1, cplx<double>* work, double* lam,
+                     double* slots, int* state, const double* info, int np, int N, int max_iter, cplx<double>* Aout,
+                     hipStream_t st);
 hipError_t launch_eig_finish(const cplx<double>* A, const cplx<double>* V, int np, int N, double* lam_sorted,
                              int* perm, cplx<double>* C, int nc, int scale, hipStream_t st);
 template <class R>

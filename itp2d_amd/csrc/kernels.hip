@@ -8,7 +8,8 @@
 //                    64x64 tiles of state pairs, split over the grid points
 //  gram_reduce       sums the split-K partials, scales by dx^2, fills the
 //                    Hermitian matrix for the eigensolver
-//  jacobi_kernel     cooperative parallel two-sided Jacobi on the overlap matrix
+//  block_jacobi      cooperative block two-sided Jacobi on the overlap matrix
+//  zgemm / oa_*      refinement of the Jacobi eigenvectors by n x n MFMA products
 //  eig_sort / eig_coeffs  ascending eigenvalues, C = U Lambda^(-1/2)
 //  rotate_kernel     Psi' = Psi C on FP64 MFMA
 #include <hip/hip_runtime.h>
@@ -244,140 +245,6 @@ template hipError_t launch_gram<float>(const cplx<float>*, long long, int, int, 
                                        double, cplx<double>*, int, int*, hipStream_t);
 
 // ---------------------------------------------------------------------------
-// Jacobi eigensolver, one cooperative launch for all sweeps. Every workgroup
-// computes all rotation parameters of a round itself (they only read the
-// 2x2 diagonal blocks), so the only grid-wide barrier is the one per round.
-// The convergence measure (largest relative off-diagonal element met in a
-// sweep) is therefore identical in every workgroup, and all stop together.
-// ---------------------------------------------------------------------------
-struct JacobiArgs {
-  cplx<double>* A[2];
-  cplx<double>* V[2];
-  int np;
-  int max_sweeps;
-  double tol;
-  double* info;  // [0] sweeps, [1] converged, [2] final buffer, [3] last sweep's largest relative
-                 // off-diagonal, [4] stop flag written by block 0
-};
-
-__global__ __launch_bounds__(256) void jacobi_kernel(JacobiArgs a) {
-  cg::grid_group grid = cg::this_grid();
-  extern __shared__ __align__(16) char jl[];
-  const int np = a.np, m = np / 2;
-  cplx<double>* sj = reinterpret_cast<cplx<double>*>(jl);
-  double* cs = reinterpret_cast<double*>(sj + m);
-  double* red = cs + m;  // [3 * blockDim + 3]
-  const int nt = (int)blockDim.x;
-  const long long gtid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long gthreads = (long long)gridDim.x * blockDim.x;
-  const long long nblock_items = (long long)m * m;
-  const long long nitems = nblock_items + (long long)np * m;
-  int cur = 0, sweeps = 0;
-  bool conv = false;
-  double wg_off = 0;
-  for (int sw = 0; sw < a.max_sweeps; sw++) {
-    double thr_off = 0, thr_b = 0, thr_d = 0;
-    for (int r = 0; r < np - 1; r++) {
-      const cplx<double>* A = a.A[cur];
-      cplx<double>* An = a.A[cur ^ 1];
-      const cplx<double>* V = a.V[cur];
-      cplx<double>* Vn = a.V[cur ^ 1];
-      for (int k = threadIdx.x; k < m; k += nt) {
-        const int p = 2 * k, q = p + 1;
-        double c, off;
-        cplx<double> s;
-        const double app = A[(long long)p * np + p].x, aqq = A[(long long)q * np + q].x;
-        const cplx<double> b = A[(long long)p * np + q];
-        jacobi_rotation(app, aqq, b, c, s, off);
-        cs[k] = c;
-        sj[k] = s;
-        thr_off = off > thr_off ? off : thr_off;
-        const double ab = std::hypot(b.x, b.y);
-        thr_b = ab > thr_b ? ab : thr_b;
-        const double dd = std::fabs(app) > std::fabs(aqq) ? std::fabs(app) : std::fabs(aqq);
-        thr_d = dd > thr_d ? dd : thr_d;
-      }
-      __syncthreads();
-      for (long long it = gtid; it < nitems; it += gthreads) {
-        if (it < nblock_items) {
-          const int kI = (int)(it / m), kJ = (int)(it - (long long)kI * m);
-          if (kJ >= kI) jacobi_block(A, An, np, m, kI, kJ, cs, sj);
-        } else {
-          const long long i2 = it - nblock_items;
-          const int c = (int)(i2 / m), k = (int)(i2 - (long long)c * m);
-          jacobi_vrow(V, Vn, np, m, c, k, cs, sj);
-        }
-      }
-      grid.sync();
-      cur ^= 1;
-    }
-    sweeps++;
-    // block 0 decides whether the sweep met the tolerance and broadcasts it
-    if (blockIdx.x == 0) {
-      red[threadIdx.x] = thr_off;
-      red[nt + threadIdx.x] = thr_b;
-      red[2 * nt + threadIdx.x] = thr_d;
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        double mo = 0, mb = 0, md = 0;
-        for (int t = 0; t < nt; t++) {
-          mo = red[t] > mo ? red[t] : mo;
-          mb = red[nt + t] > mb ? red[nt + t] : mb;
-          md = red[2 * nt + t] > md ? red[2 * nt + t] : md;
-        }
-        // relative off-diagonal below tol, or all off-diagonals at the
-        // rounding level of the largest diagonal element
-        const bool done = (mo < a.tol) || (mb <= 1e-15 * md);
-        a.info[4] = done ? 1.0 : 0.0;
-        a.info[3] = mo;
-      }
-    }
-    grid.sync();
-    conv = __hip_atomic_load(&a.info[4], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0.0;
-    if (conv) break;
-    grid.sync();  // everyone has read the flag before block 0 may overwrite it
-  }
-  // leave the result in buffer 0 (the diagonal of A and all of V)
-  if (cur == 1) {
-    const long long tot = (long long)np * np;
-    for (long long i = gtid; i < tot; i += gthreads) a.V[0][i] = a.V[1][i];
-    for (long long i = gtid; i < np; i += gthreads) a.A[0][i * np + i] = a.A[1][i * np + i];
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    a.info[0] = sweeps;
-    a.info[1] = conv ? 1.0 : 0.0;
-    a.info[2] = cur;
-  }
-  (void)wg_off;
-}
-
-size_t jacobi_lds_bytes(int np) { return (size_t)(np / 2) * (sizeof(cplx<double>) + sizeof(double)) + (3 * 256 + 3) * sizeof(double); }
-
-hipError_t launch_jacobi(cplx<double>* A0, cplx<double>* A1, cplx<double>* V0, cplx<double>* V1, int np,
-                         int max_sweeps, double tol, double* info, int num_cus, hipStream_t st) {
-  JacobiArgs a;
-  a.A[0] = A0; a.A[1] = A1; a.V[0] = V0; a.V[1] = V1;
-  a.np = np; a.max_sweeps = max_sweeps; a.tol = tol; a.info = info;
-  const size_t lds = jacobi_lds_bytes(np);
-  if (lds > 160 * 1024) return hipErrorInvalidValue;
-  (void)hipFuncSetAttribute((const void*)jacobi_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-  int per_cu = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, jacobi_kernel, 256, lds);
-  if (e != hipSuccess) return e;
-  if (per_cu < 1) return hipErrorInvalidValue;
-  const long long m = np / 2;
-  const long long items = m * m + (long long)np * m;
-  long long want = (items + 255) / 256;
-  long long cap = (long long)per_cu * num_cus;
-  if (cap > (long long)num_cus) cap = (long long)num_cus;  // barrier cost grows with workgroups
-  int grid = (int)std::max<long long>(1, std::min(want, cap));
-  void* args[] = {&a};
-  e = hipLaunchCooperativeKernel((const void*)jacobi_kernel, dim3(grid), dim3(256), args, (unsigned)lds, st);
-  if (e != hipSuccess) return e;
-  return hipGetLastError();
-}
-
-// ---------------------------------------------------------------------------
 // Block Jacobi (block_jacobi.hpp), one cooperative launch for all sweeps:
 // per round, phase 1 (one workgroup per block pair, sub-problem in LDS) and
 // phase 2 (super-tile updates of A and V), each followed by a grid barrier.
@@ -398,15 +265,21 @@ __global__ __launch_bounds__(256) void block_jacobi_kernel(BlockJacobiArgs a) {
   const long long e0 = (long long)blockIdx.x * per;
   const long long e1 = e0 + per < tot ? e0 + per : tot;
   int sweeps = 0;
-  bool conv = false;
+  bool conv = false, handed = false;
   for (int chk = 0;; chk++) {
     // whole-matrix convergence check
     unsigned long long* slot = reinterpret_cast<unsigned long long*>(a.slots + chk);
+    unsigned long long* gslot = reinterpret_cast<unsigned long long*>(a.slots + a.max_sweeps + 1 + chk);
     bj_check_rows(ex, a, e0, e1, s);
-    if (threadIdx.x == 0) atomicMax(slot, dbits(s.red[nt]));
+    if (threadIdx.x == 0) {
+      atomicMax(slot, dbits(s.red[nt]));
+      atomicMax(gslot, dbits(s.red[nt + 1]));
+    }
     grid.sync();
     const double worst = bitsd(__hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    const double rho = bitsd(__hip_atomic_load(gslot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     if (worst < 1.0) { conv = true; break; }
+    if (a.handoff > 0 && rho < a.handoff) { handed = true; break; }
     if (chk == a.max_sweeps) break;
     // one sweep of nb-1 rounds
     bj_reset_positions(ex, s, nb);
@@ -421,7 +294,7 @@ __global__ __launch_bounds__(256) void block_jacobi_kernel(BlockJacobiArgs a) {
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     a.info[0] = sweeps;
-    a.info[1] = conv ? 1.0 : 0.0;
+    a.info[1] = conv ? 1.0 : (handed ? 2.0 : 0.0);
     a.info[2] = 0;
   }
 }
@@ -431,14 +304,14 @@ __global__ void fill_u64_kernel(unsigned long long* p, long long n) {
 }
 
 hipError_t launch_block_jacobi(cplx<double>* A, cplx<double>* V, cplx<double>* Q, cplx<double>* D, int* ident,
-                               double* slots, int np, int max_sweeps, int max_inner, double tol, double* info,
-                               int grid_wgs, hipStream_t st) {
+                               double* slots, int np, int max_sweeps, int max_inner, double tol, double handoff,
+                               double* info, int grid_wgs, hipStream_t st) {
   if (np % JS || np / JB > kMaxBlocks) return hipErrorInvalidValue;
   BlockJacobiArgs a;
   a.A = A; a.V = V; a.Q = Q; a.D = D; a.ident = ident; a.slots = slots; a.info = info;
-  a.np = np; a.max_sweeps = max_sweeps; a.max_inner = max_inner; a.tol = tol;
+  a.np = np; a.max_sweeps = max_sweeps; a.max_inner = max_inner; a.tol = tol; a.handoff = handoff;
   hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(256), 0, st, reinterpret_cast<unsigned long long*>(slots),
-                     (long long)(max_sweeps + 1));
+                     (long long)(2 * (max_sweeps + 1)));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const size_t lds = bj_lds_bytes(256);
@@ -454,6 +327,168 @@ hipError_t launch_block_jacobi(cplx<double>* A, cplx<double>* V, cplx<double>* Q
   void* args[] = {&a};
   e = hipLaunchCooperativeKernel((const void*)block_jacobi_kernel, dim3(grid), dim3(256), args, (unsigned)lds, st);
   if (e != hipSuccess) return e;
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Refinement of an approximate eigenvector matrix X of the Hermitian A
+// (Ogita and Aishima, "Iterative refinement for symmetric eigenvalue
+// decomposition", Japan J. Indust. Appl. Math. 35 (2018), here for complex
+// Hermitian A): with S = X^H A X, G = X^H X, R = I - G,
+//   lambda_i = S_ii / G_ii,
+//   E_ij = (S_ij + lambda_j R_ij) / (lambda_j - lambda_i)   (i != j),
+//   E_ii = R_ii / 2,
+//   X <- X + X E,
+// which converges quadratically once every |S_ij| is small against its gap.
+// Every iteration is four n x n complex products on FP64 MFMA. The control
+// words live on the device, so the iterations queue without host round trips:
+//   state[0]: 0 running, 1 done, 2 failed, 3 not needed; state[1]: X buffer
+//   holding the result.
+// ---------------------------------------------------------------------------
+
+// C = op(A) B (+ Cadd), n x n complex, row-major with leading dimension n
+// (n a multiple of 32). op(A) = A^H if ah. 32 x 32 output tile per workgroup,
+// one 16 x 16 quarter per wavefront.
+__global__ __launch_bounds__(256) void zgemm_kernel(const cplx<double>* A, int ah, const cplx<double>* B,
+                                                    cplx<double>* C, const cplx<double>* Cadd, int n,
+                                                    const int* state) {
+  if (state && state[0] != 0) return;
+  __shared__ cplx<double> As[32][17];
+  __shared__ cplx<double> Bs[16][33];
+  const int i0 = blockIdx.y * 32, j0 = blockIdx.x * 32;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wi = w >> 1, wj = w & 1;
+  const int li = lane & 15, lk = lane >> 4;
+  d4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+  for (int k0 = 0; k0 < n; k0 += 16) {
+    for (int q = 0; q < 2; q++) {
+      const int e = tid + 256 * q;
+      // A tile: 32 rows of op(A) x 16 k
+      {
+        const int r = e >> 4, kk = e & 15;
+        As[r][kk] = ah ? conj(A[(long long)(k0 + kk) * n + i0 + r]) : A[(long long)(i0 + r) * n + k0 + kk];
+      }
+      // B tile: 16 k x 32 columns
+      {
+        const int kk = e >> 5, cidx = e & 31;
+        Bs[kk][cidx] = B[(long long)(k0 + kk) * n + j0 + cidx];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < 4; ks++) {
+      const cplx<double> av = As[wi * 16 + li][ks * 4 + lk];
+      const cplx<double> bv = Bs[ks * 4 + lk][wj * 16 + li];
+      cr = __builtin_amdgcn_mfma_f64_16x16x4f64(av.x, bv.x, cr, 0, 0, 0);
+      cr = __builtin_amdgcn_mfma_f64_16x16x4f64(-av.y, bv.y, cr, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f64_16x16x4f64(av.x, bv.y, ci, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f64_16x16x4f64(av.y, bv.x, ci, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  for (int v = 0; v < 4; v++) {
+    const long long o = (long long)(i0 + wi * 16 + lk + 4 * v) * n + j0 + wj * 16 + li;
+    cplx<double> r = mk<double>(cr[v], ci[v]);
+    if (Cadd) r = Cadd[o] + r;
+    C[o] = r;
+  }
+}
+
+// E from S and G; indices >= N (padding) stay fixed. lam[i] = S_ii / G_ii.
+// The largest |E_ij| (i != j) goes to *slot (atomic max of the bit pattern).
+__global__ void oa_update_kernel(const cplx<double>* S, const cplx<double>* G, cplx<double>* E, double* lam,
+                                 int np, int N, unsigned long long* slot, const int* state) {
+  if (state[0] != 0) return;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= np) return;
+  const long long o = (long long)i * np + j;
+  cplx<double> e = mk<double>(0.0, 0.0);
+  double m = 0;
+  if (i < N && j < N) {
+    const double li = S[(long long)i * np + i].x / G[(long long)i * np + i].x;
+    if (i == j) {
+      e = mk<double>(0.5 * (1.0 - G[o].x), 0.0);
+      lam[i] = li;
+    } else {
+      const double lj = S[(long long)j * np + j].x / G[(long long)j * np + j].x;
+      // S and G are Hermitian; their two computed triangles differ by rounding.
+      // Using the Hermitian part keeps that difference out of E's Hermitian
+      // part (which must be exactly R/2), where divided by a small gap it
+      // would stall the iteration at eps |A| / gap.
+      const long long ot = (long long)j * np + i;
+      const cplx<double> sij = (S[o] + conj(S[ot])) * 0.5;
+      const cplx<double> gij = (G[o] + conj(G[ot])) * 0.5;
+      const cplx<double> num = sij - gij * lj;  // S_ij + lambda_j R_ij with R_ij = -G_ij
+      const double gap = lj - li;
+      e = mk<double>(num.x / gap, num.y / gap);
+      m = std::hypot(e.x, e.y);
+      if (!(m <= 1e300)) m = 1e300;  // a zero gap or a NaN fails the iteration
+    }
+  }
+  E[o] = e;
+  // wave maximum, then one atomic per wavefront
+  for (int off = 32; off >= 1; off >>= 1) {
+    const double t = __shfl_xor(m, off);
+    m = t > m ? t : m;
+  }
+  if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(slot, dbits(m));
+}
+
+// After iteration k: done once the correction fell below 1e-8 (the update
+// just applied leaves errors of its square), failed if it is not small.
+__global__ void oa_decide_kernel(const unsigned long long* slot, int k, int* state) {
+  if (state[0] != 0) return;
+  const double m = bitsd(*slot);
+  if (!(m < 0.5)) { state[0] = 2; return; }
+  if (m < 1e-8) { state[0] = 1; state[1] = (k + 1) & 1; }
+}
+
+// Starts the refinement if the Jacobi stage handed off (info[1] == 2).
+__global__ void oa_init_kernel(const double* info, int* state) {
+  state[0] = info[1] == 2.0 ? 0 : 3;
+  state[1] = 0;
+}
+
+// Result into the places the eigenvalue sort reads: X -> V (if it ended in
+// the second buffer), lambda -> diagonal of A.
+__global__ void oa_finalize_kernel(const int* state, const cplx<double>* X1, cplx<double>* V, cplx<double>* A,
+                                   const double* lam, int np, int N) {
+  if (state[0] != 1) return;
+  const long long n2 = (long long)np * np;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n2; e += (long long)gridDim.x * blockDim.x) {
+    if (state[1] == 1) V[e] = X1[e];
+    const int i = (int)(e / np), j = (int)(e - (long long)i * np);
+    if (i == j && i < N) A[e] = mk<double>(lam[i], 0.0);
+  }
+}
+
+hipError_t launch_oa(const cplx<double>* A0, cplx<double>* X0, cplx<double>* X1, cplx<double>* work, double* lam,
+                     double* slots, int* state, const double* info, int np, int N, int max_iter, cplx<double>* Aout,
+                     hipStream_t st) {
+  if (np % 32) return hipErrorInvalidValue;
+  const long long n2 = (long long)np * np;
+  cplx<double>* Y = work;
+  cplx<double>* S = work + n2;
+  cplx<double>* G = work + 2 * n2;
+  cplx<double>* E = work + 3 * n2;
+  hipLaunchKernelGGL(oa_init_kernel, dim3(1), dim3(1), 0, st, info, state);
+  hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<unsigned long long*>(slots),
+                     (long long)max_iter);
+  const dim3 gg(np / 32, np / 32);
+  for (int k = 0; k < max_iter; k++) {
+    cplx<double>* X = (k & 1) ? X1 : X0;
+    cplx<double>* Xn = (k & 1) ? X0 : X1;
+    unsigned long long* slot = reinterpret_cast<unsigned long long*>(slots) + k;
+    hipLaunchKernelGGL(zgemm_kernel, gg, dim3(256), 0, st, A0, 0, X, Y, nullptr, np, state);
+    hipLaunchKernelGGL(zgemm_kernel, gg, dim3(256), 0, st, X, 1, Y, S, nullptr, np, state);
+    hipLaunchKernelGGL(zgemm_kernel, gg, dim3(256), 0, st, X, 1, X, G, nullptr, np, state);
+    hipLaunchKernelGGL(oa_update_kernel, dim3((np + 255) / 256, np), dim3(256), 0, st, S, G, E, lam, np, N, slot,
+                       state);
+    hipLaunchKernelGGL(zgemm_kernel, gg, dim3(256), 0, st, X, 0, E, Xn, X, np, state);
+    hipLaunchKernelGGL(oa_decide_kernel, dim3(1), dim3(1), 0, st, slot, k, state);
+  }
+  hipLaunchKernelGGL(oa_finalize_kernel, dim3(256), dim3(256), 0, st, state, X1, X0, Aout, lam, np, N);
   return hipGetLastError();
 }
 

@@ -269,7 +269,7 @@ int emu_block_jacobi_dbg(int n, double* Ain, double* w, int max_inner, int max_s
   BlockJacobiArgs args;
   args.A = A.data(); args.V = V.data(); args.Q = Q.data(); args.D = D.data(); args.ident = ident.data();
   args.slots = nullptr; args.info = nullptr;
-  args.np = np; args.max_sweeps = max_sweeps; args.max_inner = max_inner; args.tol = 1e-14;
+  args.np = np; args.max_sweeps = max_sweeps; args.max_inner = max_inner; args.tol = 1e-14; args.handoff = 0;
   // one LDS image per emulated workgroup (each keeps its own position table)
   std::vector<std::vector<char>> lds(G, std::vector<char>(bj_lds_bytes(256)));
   std::vector<BJLds> ls;

@@ -137,6 +137,34 @@ def test_eigh(ctx, n):
     assert np.abs(V @ np.diag(w) @ V.conj().T - A).max() < 1e-12 * np.abs(A).max()
 
 
+@pytest.mark.parametrize("n,cluster", [(100, False), (300, False), (120, True)])
+def test_eigh_refinement(ctx, n, cluster):
+    """Nearly diagonal Hermitian matrices like the late overlap matrices of an
+    ITP run: the Jacobi stage hands them to the refinement iteration at once
+    (an exactly degenerate cluster keeps them in Jacobi). Eigenpairs against LAPACK."""
+    say(f"eigh refinement n={n} cluster={cluster}")
+    rng = np.random.default_rng(1000 + n)
+    d = np.exp(-0.1 * np.sort(rng.uniform(1, 30, n)))
+    if cluster:
+        d[10:14] = d[10]
+    X = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    A = np.diag(d) + 1e-5 * (X + X.conj().T) * np.sqrt(np.outer(d, d))
+    before = ctx.oa_stats()
+    w, V = I.eigh(ctx, A)
+    after = ctx.oa_stats()
+    assert after[1] == before[1]  # no fallback
+    if not cluster:
+        assert after[0] == before[0] + 1
+    wl = np.linalg.eigvalsh(A)
+    assert np.abs(w - wl).max() < 1e-14 * np.abs(wl).max()
+    assert np.abs(A @ V - V * w).max() < 1e-13 * np.abs(A).max()
+    # what orthonormalization needs: Psi V Lambda^(-1/2) is an orthonormal set
+    VAV = V.conj().T @ A @ V
+    assert np.abs(VAV - np.diag(np.diag(VAV))).max() / np.sqrt(np.outer(w, w)).min() < 1e-13
+    assert np.abs(np.real(np.diag(VAV)) / w - 1).max() < 1e-13
+    assert np.abs(V.conj().T @ V - np.eye(n)).max() < 1e-13
+
+
 def test_zero_state_raises_nonpositive(ctx):
     sx = sy = 16
     dx = 0.5
@@ -241,3 +269,26 @@ def test_c2_full_size_steps(ctx):
     A = st.reshape(cfg["N"], -1)
     G = (A.conj() @ A.T) * dx * dx
     assert np.abs(G - np.eye(cfg["N"])).max() < 1e-11
+
+
+def test_c2_solve_refinement_matches_jacobi(ctx):
+    """The full C2 solve with the Jacobi-to-refinement hand-off against the same
+    solve with Jacobi alone: identical step and converged-state counts, energies
+    within 1e-10."""
+    say("C2 full solve, eigensolver modes")
+    cfg, V = W.config("c2_dirichlet256")
+    res = []
+    try:
+        for mode in (1, 0):
+            ctx.set_eig_mode(mode)
+            before = ctx.oa_stats()
+            s = I.ITPSystem(ctx, cfg, V)
+            s.run()
+            res.append((s.total_steps, s.num_converged, s.sorted()[0], ctx.oa_stats()[0] - before[0]))
+            del s
+    finally:
+        ctx.set_eig_mode(0)
+    (st1, c1, E1, _), (st0, c0, E0, refined) = res
+    assert st0 == st1 and c0 == c1
+    assert np.abs(E0 - E1).max() < 1e-10 * np.abs(E1).max()
+    assert refined > 0
