@@ -61,8 +61,10 @@ struct GpuRegExec {
   }
 };
 
+// At least two waves per SIMD: the line exchanges are latency-bound, and the
+// LDS of two workgroups per CU allows it.
 template <class R, int N, int ORIENT>
-__global__ __launch_bounds__(256) void reg_sweep_kernel(SweepArgs<R> a) {
+__global__ __launch_bounds__(256, 2) void reg_sweep_kernel(SweepArgs<R> a) {
   extern __shared__ __align__(16) char lds_raw[];
   GpuRegExec<RegState<R, RegShape<N>::Q>> ex;
   reg_sweep_program<R, N, ORIENT>(ex, a, reinterpret_cast<cplx<R>*>(lds_raw), (int)blockIdx.y, (int)blockIdx.x);

@@ -7,11 +7,11 @@
 // elements each (n = P*Q), thread t holding positions t + P*q. FFTs are
 // Stockham stages of radix 16/8/4/2 computed in registers; only between two
 // stages does data go through LDS (one write, one read), so a length-256
-// FFT costs one LDS exchange. The Makhoul pre- and post-permutations of the
-// real-to-real kinds are LDS exchanges as well, except that a real-to-real
-// forward transform at the start of the program gathers its permuted input
-// straight from global memory, and an inverse one at the end scatters its
-// output straight to global memory.
+// FFT costs one LDS exchange. The Makhoul pre- and post-steps of the
+// real-to-real kinds are gathers: a post-step is left pending and folded into
+// the next exchange (or the store), so a real-to-real transform costs one
+// exchange besides its FFT; a forward one at the start of the program gathers
+// its permuted input straight from global memory.
 //
 // Thread layout: row sweeps put the P threads of a line on consecutive lanes
 // (global accesses run along the row); column sweeps put consecutive lines
@@ -225,7 +225,102 @@ ITP_HD void reg_fft(Exec& ex, cplx<R>* lds, const cplx<double>* tw) {
   }
 }
 
-// Write registers A to LDS in natural order, then A[q] = f(t, l, pos, line).
+// Makhoul input permutation of DCT-II / DST-II: position m of the FFT input
+// holds x_{j(m)}; DST-II also negates odd j.
+ITP_HD int r2f_src(int n, int m) { return (m < n / 2) ? 2 * m : 2 * (n - 1 - m) + 1; }
+// Makhoul output permutation of DCT-III / DST-III: FFT output position m
+// lands at x_{j(m)} (the same map); DST-III negates odd j.
+ITP_HD int r2i_dst(int n, int m) { return (m < n / 2) ? 2 * m : 2 * (n - 1 - m) + 1; }
+
+ITP_HD bool lop_elementwise(int c) { return c == LOP_MUL_TAB || c == LOP_MUL_POS || c == LOP_SCALE; }
+ITP_HD bool lop_r2f(int c) { return c == LOP_DCT2 || c == LOP_DST2; }
+ITP_HD bool lop_r2i(int c) { return c == LOP_DCT3 || c == LOP_DST3; }
+
+// ---------------------------------------------------------------------------
+// Pending post-steps. The last step of a real-to-real transform is a gather
+// (a pair combination for DCT-II / DST-II, a permutation for DCT-III /
+// DST-III), and so is the first step of the next one: rather than spend an
+// LDS exchange on each, the registers keep the FFT output L and a note of how
+// the logical line values P(p) derive from it. The next exchange reads P(p)
+// straight from L.
+//   PK_NONE      P(p) = L[p]
+//   PK_R2F_DCT   P(p) = f(p) (w_k L[k] + conj(w_k) L[(n-k) mod n]), k = p    (REDFT10 post)
+//   PK_R2F_DST   the same with k = n-1-p                                      (RODFT10 post)
+//   PK_PERM_DCT  P(p) = L[m(p)], m(p) = p/2 (p even), n-1-(p-1)/2 (odd)      (REDFT01 post)
+//   PK_PERM_DST  P(p) = (-1)^p L[m(p)]                                        (RODFT01 post)
+// f(p) is at most one pending elementwise factor; elementwise ops met with
+// nothing or a permutation pending are applied to the registers at once.
+// The kind is a compile-time tag at every use, so the unrolled element loops
+// carry no branches on it.
+// ---------------------------------------------------------------------------
+enum RegPend { PK_NONE = 0, PK_R2F_DCT, PK_R2F_DST, PK_PERM_DCT, PK_PERM_DST };
+
+template <int K, bool E>
+struct PendTag {
+  static constexpr int kind = K;
+  static constexpr bool elem = E;
+};
+
+struct RegPending {
+  int kind;
+  int el;  // index of the pending elementwise op, or -1
+};
+
+ITP_HD int perm_src(int n, int p) { return (p & 1) ? (n - 1 - (p >> 1)) : (p >> 1); }
+
+// Factor of an elementwise op at global offset g and line position pos
+// (MUL_TAB: t0[g], MUL_POS: t0[pos], SCALE: s).
+template <class R>
+ITP_HD R elem_factor(const LineOp& op, long long g, int pos) {
+  if (op.code == LOP_MUL_TAB) return R(op.t0[g]);
+  if (op.code == LOP_MUL_POS) return R(op.t0[pos]);
+  return R(op.s);
+}
+
+// P(p) for a pending kind known at compile time. fg, fp: the pending factor's
+// table strides in (global offset, position); ft its table.
+template <class TAG, class R, int N>
+ITP_HD cplx<R> pend_value(const cplx<R>* line, int p, long long g, const cplx<double>* mkt, const double* ft,
+                          long long fsg, long long fsp) {
+  constexpr int n = N;
+  constexpr int K = TAG::kind;
+  cplx<R> v;
+  if constexpr (K == PK_R2F_DCT || K == PK_R2F_DST) {
+    const int k = K == PK_R2F_DST ? n - 1 - p : p;
+    const cplx<double> wd = mkt[opaque(k)];
+    const cplx<R> w = mk<R>(R(wd.x), R(wd.y));
+    v = w * line[lpad(k)] + conj(w) * line[lpad((n - k) & (n - 1))];
+    if constexpr (TAG::elem) v = v * R(ft[g * fsg + p * fsp]);
+  } else if constexpr (K == PK_PERM_DCT || K == PK_PERM_DST) {
+    v = line[lpad(perm_src(n, p))];
+    if (K == PK_PERM_DST && (p & 1)) v = -v;
+  } else {
+    v = line[lpad(p)];
+  }
+  return v;
+}
+
+// Calls f(PendTag<kind, elem>{}) for the runtime pending state.
+template <class F>
+ITP_HD void with_pend(const RegPending& pd, F&& f) {
+  const bool e = pd.el >= 0;
+  switch (pd.kind) {
+    case PK_R2F_DCT:
+      if (e) f(PendTag<PK_R2F_DCT, true>{});
+      else f(PendTag<PK_R2F_DCT, false>{});
+      break;
+    case PK_R2F_DST:
+      if (e) f(PendTag<PK_R2F_DST, true>{});
+      else f(PendTag<PK_R2F_DST, false>{});
+      break;
+    case PK_PERM_DCT: f(PendTag<PK_PERM_DCT, false>{}); break;
+    case PK_PERM_DST: f(PendTag<PK_PERM_DST, false>{}); break;
+    default: f(PendTag<PK_NONE, false>{}); break;
+  }
+}
+
+// Write registers A (the pending L) to LDS in natural order, then
+// A[q] = f(t, l, position t + P q, line). Two barriers.
 template <class G, int WHICH, class R, class Exec, class F>
 ITP_HD void reg_exchange(Exec& ex, cplx<R>* lds, F&& f) {
   ex.run([&](int tid, auto& st) {
@@ -244,85 +339,6 @@ ITP_HD void reg_exchange(Exec& ex, cplx<R>* lds, F&& f) {
   });
 }
 
-// Makhoul input permutation of DCT-II / DST-II: position m of the FFT input
-// holds x_{j(m)}; DST-II also negates odd j.
-ITP_HD int r2f_src(int n, int m) { return (m < n / 2) ? 2 * m : 2 * (n - 1 - m) + 1; }
-// Makhoul output permutation of DCT-III / DST-III: FFT output position m
-// lands at x_{j(m)} (the same map); DST-III negates odd j.
-ITP_HD int r2i_dst(int n, int m) { return (m < n / 2) ? 2 * m : 2 * (n - 1 - m) + 1; }
-
-ITP_HD bool lop_elementwise(int c) { return c == LOP_MUL_TAB || c == LOP_MUL_POS || c == LOP_SCALE; }
-ITP_HD bool lop_r2f(int c) { return c == LOP_DCT2 || c == LOP_DST2; }
-ITP_HD bool lop_r2i(int c) { return c == LOP_DCT3 || c == LOP_DST3; }
-
-// Post step of DCT-II / DST-II: Y_k = w_k V_k + conj(w_k) V_{(n-k) mod n},
-// DST-II output reversed.
-template <class G, bool DST, int WHICH, class R, class Exec>
-ITP_HD void reg_r2f_post(Exec& ex, cplx<R>* lds, const cplx<double>* mkt) {
-  constexpr int n = G::n;
-  reg_exchange<G, WHICH>(ex, lds, [&](int, int, int p, const cplx<R>* line) {
-    const int k = DST ? n - 1 - p : p;
-    const cplx<double> wd = mkt[opaque(k)];
-    const cplx<R> w = mk<R>(R(wd.x), R(wd.y));
-    return w * line[lpad(k)] + conj(w) * line[lpad((n - k) & (n - 1))];
-  });
-}
-
-// Pre step of DCT-III / DST-III: U_0 = X_0, U_k = w_k (X_k + i X_{n-k});
-// DST-III reads X reversed.
-template <class G, bool DST, int WHICH, class R, class Exec>
-ITP_HD void reg_r2i_pre(Exec& ex, cplx<R>* lds, const cplx<double>* mkt) {
-  constexpr int n = G::n;
-  reg_exchange<G, WHICH>(ex, lds, [&](int, int, int k, const cplx<R>* line) {
-    if (k == 0) return line[lpad(DST ? n - 1 : 0)];
-    const cplx<R> xk = line[lpad(DST ? n - 1 - k : k)];
-    const cplx<R> xnk = line[lpad(DST ? k - 1 : n - k)];
-    const cplx<double> wd = mkt[opaque(k)];
-    return mk<R>(R(wd.x), R(wd.y)) * (xk + mul_i(xnk));
-  });
-}
-
-template <class G, int CODE, int WHICH, class R, class Exec>
-ITP_HD void reg_r2r(Exec& ex, cplx<R>* lds, const cplx<double>* tw, const cplx<double>* mkt) {
-  constexpr int n = G::n;
-  constexpr bool DST = (CODE == LOP_DST2 || CODE == LOP_DST3);
-  if constexpr (CODE == LOP_DCT2 || CODE == LOP_DST2) {
-    reg_exchange<G, WHICH>(ex, lds, [&](int, int, int m, const cplx<R>* line) {
-      const int j = r2f_src(n, m);
-      const cplx<R> v = line[lpad(j)];
-      return (DST && (j & 1)) ? -v : v;
-    });
-    reg_fft<G, false, WHICH>(ex, lds, tw);
-    reg_r2f_post<G, DST, WHICH>(ex, lds, mkt);
-  } else {
-    reg_r2i_pre<G, DST, WHICH>(ex, lds, mkt);
-    reg_fft<G, false, WHICH>(ex, lds, tw);
-    reg_exchange<G, WHICH>(ex, lds, [&](int, int, int j, const cplx<R>* line) {
-      const int m = (j & 1) ? (n - 1 - (j >> 1)) : (j >> 1);
-      const cplx<R> v = line[lpad(m)];
-      return (DST && (j & 1)) ? -v : v;
-    });
-  }
-}
-
-template <class G, int WHICH, class R, class Exec>
-ITP_HD void reg_r2r_dispatch(Exec& ex, int code, cplx<R>* lds, const cplx<double>* tw, const cplx<double>* mkt) {
-  switch (code) {
-    case LOP_DST2: reg_r2r<G, LOP_DST2, WHICH>(ex, lds, tw, mkt); break;
-    case LOP_DST3: reg_r2r<G, LOP_DST3, WHICH>(ex, lds, tw, mkt); break;
-    case LOP_DCT2: reg_r2r<G, LOP_DCT2, WHICH>(ex, lds, tw, mkt); break;
-    default: reg_r2r<G, LOP_DCT3, WHICH>(ex, lds, tw, mkt); break;
-  }
-}
-
-// Elementwise op at global line `line` and position `pos`.
-template <class R>
-ITP_HD cplx<R> reg_elem(const LineOp& op, cplx<R> v, long long g, int pos) {
-  if (op.code == LOP_MUL_TAB) return v * R(op.t0[g]);
-  if (op.code == LOP_MUL_POS) return v * R(op.t0[pos]);
-  return v * R(op.s);
-}
-
 // The whole sweep for line block blk of field `field`.
 template <class R, int N, int ORIENT, class Exec>
 ITP_HD void reg_sweep_program(Exec& ex, const SweepArgs<R>& a, cplx<R>* lds, int field, int blk) {
@@ -332,70 +348,151 @@ ITP_HD void reg_sweep_program(Exec& ex, const SweepArgs<R>& a, cplx<R>* lds, int
   const int sx = a.sx;
   const int line0 = blk * G::LPW;
   const long long fo = (long long)field * a.fstride;
+  const LineOp* ops = a.ops;
+  const cplx<double>* mkt = a.mk;
   auto goff = [&](int l, int pos) -> long long {
     return ORIENT == 0 ? (long long)(line0 + l) * sx + pos : (long long)pos * sx + (line0 + l);
   };
   // leading elementwise ops, then possibly a forward r2r whose input
   // permutation is taken from global memory
   int lead = 0;
-  while (lead < nops && lop_elementwise(a.ops[lead].code)) lead++;
-  const bool fold_in = lead < nops && lop_r2f(a.ops[lead].code);
-  const bool in_dst = fold_in && a.ops[lead].code == LOP_DST2;
-  const int start = fold_in ? lead + 1 : lead;
-  // trailing elementwise ops (not already taken by the load), and possibly an
-  // inverse r2r before them whose output permutation goes straight to global
-  // memory
-  int trail = nops;
-  while (trail > start && lop_elementwise(a.ops[trail - 1].code)) trail--;
-  const bool fold_out = trail > start && lop_r2i(a.ops[trail - 1].code);
-  const bool out_dst = fold_out && a.ops[trail - 1].code == LOP_DST3;
+  while (lead < nops && lop_elementwise(ops[lead].code)) lead++;
+  const bool fold_in = lead < nops && lop_r2f(ops[lead].code);
+  const bool in_dst = fold_in && ops[lead].code == LOP_DST2;
 
   // load
   {
     const cplx<R>* src = a.src + fo;
-    ex.local([&](int tid, auto& st) {
-      const int t = G::t_of(tid), l = G::l_of(tid);
+    auto load = [&](auto fi) {
+      constexpr bool FI = decltype(fi)::value;
+      ex.local([&](int tid, auto& st) {
+        const int t = G::t_of(tid), l = G::l_of(tid);
 #pragma unroll
-      for (int q = 0; q < Q; q++) {
-        const int m = t + P * q;
-        const int pos = fold_in ? r2f_src(n, m) : m;
-        const long long g = goff(l, pos);
-        cplx<R> v = src[g];
-        for (int i = 0; i < lead; i++) v = reg_elem<R>(a.ops[i], v, g, pos);
-        if (in_dst && (pos & 1)) v = -v;
-        st.v[q] = v;
-      }
-    });
-  }
-  if (fold_in) {
-    reg_fft<G, false, 0>(ex, lds, a.tw);
-    if (in_dst) reg_r2f_post<G, true, 0>(ex, lds, a.mk);
-    else reg_r2f_post<G, false, 0>(ex, lds, a.mk);
-  }
-  const int stop = fold_out ? trail - 1 : trail;
-  for (int o = start; o < stop; o++) {
-    const LineOp op = a.ops[o];
-    switch (op.code) {
-      case LOP_FFT_FWD: reg_fft<G, false, 0>(ex, lds, a.tw); break;
-      case LOP_FFT_INV: reg_fft<G, true, 0>(ex, lds, a.tw); break;
-      case LOP_DST2: case LOP_DST3: case LOP_DCT2: case LOP_DCT3:
-        reg_r2r_dispatch<G, 0>(ex, op.code, lds, a.tw, a.mk);
-        break;
-      case LOP_MUL_TAB: case LOP_MUL_POS: case LOP_SCALE:
+        for (int q = 0; q < Q; q++) {
+          const int m = t + P * q;
+          const int pos = FI ? r2f_src(n, m) : m;
+          cplx<R> v = src[goff(l, pos)];
+          if (FI && in_dst && (pos & 1)) v = -v;
+          st.v[q] = v;
+        }
+      });
+      for (int i = 0; i < lead; i++) {
+        const LineOp op = ops[i];
         ex.local([&](int tid, auto& st) {
           const int t = G::t_of(tid), l = G::l_of(tid);
 #pragma unroll
           for (int q = 0; q < Q; q++) {
-            const int pos = t + P * q;
-            st.v[q] = reg_elem<R>(op, st.v[q], goff(l, pos), pos);
+            const int m = t + P * q;
+            const int pos = FI ? r2f_src(n, m) : m;
+            st.v[q] = st.v[q] * elem_factor<R>(op, goff(l, pos), pos);
           }
         });
-        break;
-      case LOP_XSPLIT: {
-        // rows: w[x] = i X2[y][x-1] v[x-1] (w[0] = 0), v *= X, then
-        // v = RODFT01(v) + REDFT01(w) (expkinetic.cpp:176-184, kinetic.cpp:128-134)
-        const double* X = op.t0;
-        const double* X2 = op.t1;
+      }
+    };
+    if (fold_in) load(std::integral_constant<bool, true>{});
+    else load(std::integral_constant<bool, false>{});
+  }
+  RegPending pd = {PK_NONE, -1};
+  int o = lead;
+  if (fold_in) {
+    reg_fft<G, false, 0>(ex, lds, a.tw);
+    pd.kind = in_dst ? PK_R2F_DST : PK_R2F_DCT;
+    o = lead + 1;
+  }
+  // the pending factor's table and strides
+  auto ftab = [&](const RegPending& pc, const double*& ft, long long& fsg, long long& fsp) {
+    ft = nullptr; fsg = 0; fsp = 0;
+    if (pc.el >= 0) {
+      const LineOp& e = ops[pc.el];
+      ft = e.t0;
+      fsg = e.code == LOP_MUL_TAB ? 1 : 0;
+      fsp = e.code == LOP_MUL_POS ? 1 : 0;
+    }
+  };
+  // bring registers to natural order (one exchange if a post-step is pending)
+  auto materialize = [&]() {
+    if (pd.kind == PK_NONE) return;
+    const RegPending pc = pd;
+    const double* ft; long long fsg, fsp;
+    ftab(pc, ft, fsg, fsp);
+    with_pend(pc, [&](auto tag) {
+      reg_exchange<G, 0>(ex, lds, [&](int, int l, int p, const cplx<R>* line) {
+        return pend_value<decltype(tag), R, N>(line, p, goff(l, p), mkt, ft, fsg, fsp);
+      });
+    });
+    pd.kind = PK_NONE;
+    pd.el = -1;
+  };
+  for (; o < nops; o++) {
+    const LineOp op = ops[o];
+    const int code = op.code;
+    if (lop_elementwise(code)) {
+      if ((pd.kind == PK_R2F_DCT || pd.kind == PK_R2F_DST) && op.code != LOP_SCALE) {
+        if (pd.el >= 0) materialize();  // at most one pending factor
+        else { pd.el = o; continue; }
+      }
+      // nothing or a permutation pending: apply at the logical position now
+      const bool perm = pd.kind == PK_PERM_DCT || pd.kind == PK_PERM_DST;
+      ex.local([&](int tid, auto& st) {
+        const int t = G::t_of(tid), l = G::l_of(tid);
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+          const int m = t + P * q;
+          const int pos = perm ? r2i_dst(n, m) : m;
+          st.v[q] = st.v[q] * elem_factor<R>(op, goff(l, pos), pos);
+        }
+      });
+      continue;
+    }
+    if (code == LOP_FFT_FWD || code == LOP_FFT_INV) {
+      materialize();
+      if (code == LOP_FFT_FWD) reg_fft<G, false, 0>(ex, lds, a.tw);
+      else reg_fft<G, true, 0>(ex, lds, a.tw);
+      continue;
+    }
+    const RegPending pc = pd;
+    const double* ft; long long fsg, fsp;
+    ftab(pc, ft, fsg, fsp);
+    if (lop_r2f(code)) {
+      // DCT-II / DST-II: FFT input m holds +-P(j(m)) (Makhoul's reordering)
+      const bool dst = code == LOP_DST2;
+      with_pend(pc, [&](auto tag) {
+        using T = decltype(tag);
+        reg_exchange<G, 0>(ex, lds, [&](int, int l, int m, const cplx<R>* line) {
+          const int j = r2f_src(n, m);
+          const cplx<R> v = pend_value<T, R, N>(line, j, goff(l, j), mkt, ft, fsg, fsp);
+          return (dst && (j & 1)) ? -v : v;
+        });
+      });
+      reg_fft<G, false, 0>(ex, lds, a.tw);
+      pd.kind = dst ? PK_R2F_DST : PK_R2F_DCT;
+    } else if (lop_r2i(code)) {
+      // DCT-III / DST-III: U_0 = X_0, U_k = w_k (X_k + i X_{n-k}); DST-III reads X reversed
+      const bool dst = code == LOP_DST3;
+      with_pend(pc, [&](auto tag) {
+        using T = decltype(tag);
+        reg_exchange<G, 0>(ex, lds, [&](int, int l, int k, const cplx<R>* line) {
+          const int pa = dst ? n - 1 - k : k;
+          const int pb = k == 0 ? pa : (dst ? k - 1 : n - k);
+          const cplx<R> xa = pend_value<T, R, N>(line, pa, goff(l, pa), mkt, ft, fsg, fsp);
+          const cplx<R> xb = pend_value<T, R, N>(line, pb, goff(l, pb), mkt, ft, fsg, fsp);
+          const cplx<double> wd = mkt[opaque(k)];
+          const cplx<R> u = mk<R>(R(wd.x), R(wd.y)) * (xa + mul_i(xb));
+          return k == 0 ? xa : u;
+        });
+      });
+      reg_fft<G, false, 0>(ex, lds, a.tw);
+      pd.kind = dst ? PK_PERM_DST : PK_PERM_DCT;
+    } else if (code == LOP_XSPLIT) {
+      // rows: with b = P, v_x = X[y][x] b_x and w_x = i X2[y][x-1] b_{x-1} (w_0 = 0),
+      // the result RODFT01(v) + REDFT01(w) (expkinetic.cpp:176-184,
+      // kinetic.cpp:128-134). Both pre-steps need only b_{n-1-k} and b_{k-1}:
+      //   U^v_k = w_k (v_{n-1-k} + i v_{k-1}),  U^v_0 = v_{n-1}
+      //   U^w_k = w_k (w_k + i w_{n-k}),        U^w_0 = 0
+      const double* X = op.t0;
+      const double* X2 = op.t1;
+      with_pend(pc, [&](auto tag) {
+        using T = decltype(tag);
         ex.run([&](int tid, auto& st) {
           const int t = G::t_of(tid), l = G::l_of(tid);
           cplx<R>* line = lds + l * G::LD;
@@ -408,28 +505,40 @@ ITP_HD void reg_sweep_program(Exec& ex, const SweepArgs<R>& a, cplx<R>* lds, int
           const long long row = (long long)(line0 + l) * sx;
 #pragma unroll
           for (int q = 0; q < Q; q++) {
-            const int x = t + P * q;
-            cplx<R> wv = mk<R>(R(0), R(0));
-            if (x > 0) wv = mul_i(line[lpad(x - 1)] * R(X2[row + x - 1]));
-            st.w[q] = wv;
-            st.v[q] = st.v[q] * R(X[row + x]);
+            const int k = t + P * q;
+            const int pa = n - 1 - k;
+            const int pb = k == 0 ? pa : k - 1;
+            const cplx<R> bpa = pend_value<T, R, N>(line, pa, goff(l, pa), mkt, ft, fsg, fsp);
+            const cplx<R> bb = pend_value<T, R, N>(line, pb, goff(l, pb), mkt, ft, fsg, fsp);
+            const cplx<R> ba = bpa * R(X[row + pa]);
+            const cplx<double> wd = mkt[opaque(k)];
+            const cplx<R> w = mk<R>(R(wd.x), R(wd.y));
+            const cplx<R> uv = w * (ba + mul_i(bb * R(X[row + pb])));
+            const cplx<R> uw = w * (mul_i(bb * R(X2[row + pb])) - bpa * R(X2[row + pa]));
+            st.v[q] = k == 0 ? ba : uv;
+            st.w[q] = k == 0 ? mk<R>(R(0), R(0)) : uw;
           }
         });
-        reg_r2r<G, LOP_DST3, 0>(ex, lds, a.tw, a.mk);
-        reg_r2r<G, LOP_DCT3, 1>(ex, lds, a.tw, a.mk);
-        ex.local([&](int, auto& st) {
+      });
+      reg_fft<G, false, 0>(ex, lds, a.tw);
+      reg_fft<G, false, 1>(ex, lds, a.tw);
+      // combine before the shared output permutation: DST-III's (-1)^j sign
+      // falls on the FFT outputs m >= n/2
+      ex.local([&](int tid, auto& st) {
+        const int t = G::t_of(tid);
 #pragma unroll
-          for (int q = 0; q < Q; q++) st.v[q] = st.v[q] + st.w[q];
-        });
-      } break;
-      default: break;
+        for (int q = 0; q < Q; q++) {
+          const int m = t + P * q;
+          st.v[q] = (m < n / 2) ? st.w[q] + st.v[q] : st.w[q] - st.v[q];
+        }
+      });
+      pd.kind = PK_PERM_DCT;
     }
+    pd.el = -1;
   }
-  if (fold_out) {
-    if (out_dst) reg_r2i_pre<G, true, 0>(ex, lds, a.mk);
-    else reg_r2i_pre<G, false, 0>(ex, lds, a.mk);
-    reg_fft<G, false, 0>(ex, lds, a.tw);
-  }
+  // a pending pair combination needs one more exchange; a pending
+  // permutation is applied by the store itself
+  if (pd.kind == PK_R2F_DCT || pd.kind == PK_R2F_DST) materialize();
   // store with epilogue; the dot partial of each thread goes to LDS slot tid
   // (every LDS read of the program ended at a barrier)
   {
@@ -439,28 +548,32 @@ ITP_HD void reg_sweep_program(Exec& ex, const SweepArgs<R>& a, cplx<R>* lds, int
     const cplx<R>* vsrc = a.vsrc ? a.vsrc + fo : nullptr;
     const cplx<R>* dsrc = a.dotsrc ? a.dotsrc + fo : nullptr;
     cplx<double>* red = reinterpret_cast<cplx<double>*>(lds);
-    ex.run([&](int tid, auto& st) {
-      const int t = G::t_of(tid), l = G::l_of(tid);
-      cplx<double> acc = mk<double>(0.0, 0.0);
+    auto store = [&](auto tag) {
+      constexpr int K = decltype(tag)::kind;
+      constexpr bool PERM = K == PK_PERM_DCT || K == PK_PERM_DST;
+      ex.run([&](int tid, auto& st) {
+        const int t = G::t_of(tid), l = G::l_of(tid);
+        cplx<double> acc = mk<double>(0.0, 0.0);
 #pragma unroll
-      for (int q = 0; q < Q; q++) {
-        const int m = t + P * q;
-        const int pos = fold_out ? r2i_dst(n, m) : m;
-        const long long g = goff(l, pos);
-        cplx<R> r = st.v[q];
-        if (out_dst && (pos & 1)) r = -r;
-        for (int i = trail; i < nops; i++) r = reg_elem<R>(a.ops[i], r, g, pos);
-        if (add1) r = add1[g] + r;
-        if (vtab) r = r + vsrc[g] * R(vtab[g]);
-        if (dsrc) {
-          const cplx<R> d = dsrc[g];
-          acc.x += double(d.x) * double(r.x) + double(d.y) * double(r.y);
-          acc.y += double(d.x) * double(r.y) - double(d.y) * double(r.x);
+        for (int q = 0; q < Q; q++) {
+          const int m = t + P * q;
+          const int pos = PERM ? r2i_dst(n, m) : m;
+          const long long g = goff(l, pos);
+          cplx<R> r = st.v[q];
+          if (K == PK_PERM_DST && (pos & 1)) r = -r;
+          if (add1) r = add1[g] + r;
+          if (vtab) r = r + vsrc[g] * R(vtab[g]);
+          if (dsrc) {
+            const cplx<R> d = dsrc[g];
+            acc.x += double(d.x) * double(r.x) + double(d.y) * double(r.y);
+            acc.y += double(d.x) * double(r.y) - double(d.y) * double(r.x);
+          }
+          if (dst) dst[g] = r;
         }
-        if (dst) dst[g] = r;
-      }
-      if (dsrc) red[tid] = acc;
-    });
+        if (dsrc) red[tid] = acc;
+      });
+    };
+    with_pend(pd, store);
     if (a.dotsrc) {
       ex.run([&](int tid, auto&) {
         constexpr int groups = 16, per = kRegThreads / groups;
