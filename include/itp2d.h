@@ -173,10 +173,13 @@ int itp_ctx_set_eig_inner(itp_ctx* ctx, int inner);
 int itp_ctx_set_eig_grid(itp_ctx* ctx, int workgroups);
 /* Eigensolver of the overlap matrix (replaces zheev, eigensolver.hpp:80-84):
  * 0 (default) block Jacobi sweeps until every unconverged off-diagonal element
- * is below a quarter of its diagonal gap, then the refinement iteration
+ * is below half of its diagonal gap, then the refinement iteration
  * X <- X + X E on n x n matrix products (falls back to plain Jacobi if it does
  * not converge); 1 block Jacobi only. */
 int itp_ctx_set_eig_mode(itp_ctx* ctx, int mode);
+/* Hand-off threshold of mode 0: the Jacobi sweeps stop once every unconverged
+ * |a_ij| is below ratio * |a_ii - a_jj| (default 0.5). */
+int itp_ctx_set_eig_handoff(itp_ctx* ctx, double ratio);
 /* Calls finished by the refinement iteration and calls that fell back to Jacobi. */
 int itp_ctx_oa_stats(itp_ctx* ctx, long long* refined, long long* fallbacks);
 

@@ -84,6 +84,7 @@ def lib():
             "itp_ctx_set_eig_inner": [_vp, C.c_int],
             "itp_ctx_set_eig_grid": [_vp, C.c_int],
             "itp_ctx_set_eig_mode": [_vp, C.c_int],
+            "itp_ctx_set_eig_handoff": [_vp, C.c_double],
             "itp_ctx_oa_stats": [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
             "itp_grid_create": [_vp, _sz, _sz, C.c_double, C.c_int, C.POINTER(_vp)],
             "itp_grid_destroy": [_vp],
@@ -211,6 +212,10 @@ class Context:
     def set_eig_mode(self, mode):
         """0: Jacobi handing off to the refinement iteration (default); 1: Jacobi only."""
         self.check(lib().itp_ctx_set_eig_mode(self.h, int(mode)), "set_eig_mode")
+
+    def set_eig_handoff(self, ratio):
+        """Coupling-to-gap ratio below which the Jacobi sweeps hand off to the refinement."""
+        self.check(lib().itp_ctx_set_eig_handoff(self.h, float(ratio)), "set_eig_handoff")
 
     def oa_stats(self):
         """(calls finished by the refinement iteration, calls that fell back to Jacobi)."""

@@ -38,6 +38,7 @@ struct itp_ctx {
   int eig_inner = 1;   // element sweeps per block sub-problem and round (upper bound)
   int eig_grid = 64;   // workgroups of the eigensolver (its grid barrier cost grows with them)
   int eig_mode = 0;    // 0: Jacobi handing off to the refinement iteration, 1: Jacobi only
+  double eig_handoff = 0.5;   // hand-off threshold on |a_ij| / |a_ii - a_jj|
   long long oa_done = 0, oa_fallback = 0;
 };
 
@@ -114,9 +115,6 @@ struct itp_hamiltonian {
 namespace {
 
 constexpr int kMaxSweeps = 40;
-// The Jacobi sweeps hand the matrix to the refinement iteration once every
-// unconverged off-diagonal element is below this fraction of its diagonal gap.
-constexpr double kHandoff = 0.25;
 constexpr int kOAMaxIter = 8;
 
 int set_err(itp_ctx* c, int code, const std::string& msg) {
@@ -354,7 +352,7 @@ int eigen_stage(itp_stateset* s, size_t* bad_n, double* bad_lambda, bool scale) 
   HIPCK(c, hipMemcpyAsync(s->V[0], s->Vinit, np2, hipMemcpyDeviceToDevice, c->stream));
   if (hybrid) HIPCK(c, hipMemcpyAsync(s->A[1], s->A[0], np2, hipMemcpyDeviceToDevice, c->stream));
   HIPCK(c, launch_block_jacobi(s->A[0], s->V[0], s->bjQ, s->bjD, s->bjident, s->bjslots, s->np, kMaxSweeps,
-                               c->eig_inner, 1e-14, hybrid ? kHandoff : 0.0, s->info, c->eig_grid, c->stream));
+                               c->eig_inner, 1e-14, hybrid ? c->eig_handoff : 0.0, s->info, c->eig_grid, c->stream));
   if (hybrid)
     HIPCK(c, launch_oa(s->A[1], s->V[0], s->V[1], s->oa_work, s->oa_lam, s->oa_slots, s->oa_state, s->info, s->np, N,
                        kOAMaxIter, s->A[0], c->stream));
@@ -541,6 +539,12 @@ int itp_ctx_set_eig_inner(itp_ctx* c, int inner) {
 int itp_ctx_set_eig_mode(itp_ctx* c, int mode) {
   if (!c || mode < 0 || mode > 1) return ITP_E_INVALID;
   c->eig_mode = mode;
+  return ITP_OK;
+}
+
+int itp_ctx_set_eig_handoff(itp_ctx* c, double ratio) {
+  if (!c || !(ratio > 0) || !(ratio < 1e3)) return ITP_E_INVALID;
+  c->eig_handoff = ratio;
   return ITP_OK;
 }
 
