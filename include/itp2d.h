@@ -133,6 +133,22 @@ int itp_energies(itp_hamiltonian* h, itp_stateset* s, double* E, double* sigma);
 /* Complex <H> and <H^2> per slot (interleaved re, im; each nullable). */
 int itp_energies_complex(itp_hamiltonian* h, itp_stateset* s, double* mean, double* meansq);
 
+/* ---- mixed-precision path (no counterpart in the reference; it completes
+ * the loop above for FP32 sets) ---- */
+/* Rayleigh-Ritz: orthonormalize (as itp_orthonormalize), W = H Psi, the
+ * H-matrix Psi^H W (split-BF16x3 MFMA for FP32 sets, FP64 MFMA for FP64 sets),
+ * its FP64 eigendecomposition, Psi <- Psi U. E[n] are the ascending Ritz values
+ * and sigma[n] = ||H psi_n - E_n psi_n|| the residual norms (the energy
+ * deviation of operators.cpp:40-50 for an eigenpair estimate). Slot n of the
+ * set then holds the Ritz vector of E[n]. E, sigma nullable. */
+int itp_rayleigh_ritz(itp_hamiltonian* h, itp_stateset* s, double* E, double* sigma);
+/* Copies the fields of src into dst (same grid and size), converting the
+ * precision if the two differ. */
+int itp_stateset_convert(itp_stateset* dst, const itp_stateset* src);
+/* Ascending overlap eigenvalues of the set's last orthonormalization
+ * (lambda_n of stateset.cpp:158-177); N doubles. */
+int itp_stateset_overlap_eigenvalues(const itp_stateset* s, double* lam);
+
 /* ---- single-operator hooks (Operator::operate on host fields, for parity
  * checks; fields are [n][sy][sx] interleaved complex double, in place) ---- */
 /* ExpKinetic(e, B, tr, bt, coeff, prefactor) (expkinetic.hpp:38-39) */

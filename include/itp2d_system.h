@@ -44,7 +44,22 @@ typedef struct itps_params {
   int recover;
   uint64_t seed;            /* initial-state noise seed, continued on recovery */
   itps_test timestep_test, final_test;
-  int precision;            /* ITP_FP64 or ITP_FP32 fields */
+  int precision;            /* ITP_FP64: the reference algorithm in FP64.
+                               ITP_FP32: the mixed path (no counterpart in the
+                               reference): FP32 fields, split-BF16x3 overlap,
+                               FP64 eigensolver, energies estimated from the
+                               overlap eigenvalues, E_n = -ln(lambda_n) / (2 eps),
+                               until their relative change per step is below
+                               mixed_tol for the wanted states, or its largest
+                               value has stopped falling for 6 steps (the FP32
+                               rounding floor); then FP64 steps
+                               of the reference algorithm on an FP64 copy until
+                               the configured tests pass (the reference's energies
+                               belong to the fixed point of the split propagator,
+                               so parity needs that fixed point). Uses the first
+                               eps for the FP32 phase. */
+  double mixed_tol;         /* 0: 2e-7 */
+  double accuracy;          /* error bound of itps_rayleigh_ritz; 0: 1e-8 */
 } itps_params;
 
 typedef struct itps_system itps_system;
@@ -71,6 +86,19 @@ int itps_states(itps_system* s, double* out);
 double itps_total_time(const itps_system* s);
 /* Number of orthonormalization failures recovered from. */
 int itps_recoveries(const itps_system* s);
+/* Steps of the mixed path's FP32 phase (included in itps_total_steps). */
+int itps_mixed_steps(const itps_system* s);
+/* Final-convergence flag of each sorted position (N ints). */
+int itps_converged(const itps_system* s, int* flags);
+/* Rayleigh-Ritz pass on the current states (itp_rayleigh_ritz): replaces the
+ * sorted energies and deviations by the Ritz values and residual norms, and the
+ * convergence flags by the error bound sigma^2 / gap <= accuracy |E| (Ritz
+ * values within 1e-7 relative form a cluster: summed squared residuals, gap to
+ * the other Ritz values). These are the exact discrete eigenvalues of the
+ * Hermitian part of H in the states' span, free of the split propagator's
+ * O(eps^(2h)) bias that the loop's own energies carry. No counterpart in the
+ * reference. */
+int itps_rayleigh_ritz(itps_system* s);
 
 #ifdef __cplusplus
 }

@@ -60,7 +60,8 @@ class _Params(C.Structure):
                 ("eps_values", _dp), ("n_eps", _sz), ("eps_divisor", C.c_double),
                 ("exhaust_eps", C.c_int), ("max_steps", C.c_int), ("min_time_step", C.c_double),
                 ("recover", C.c_int), ("seed", C.c_uint64), ("timestep_test", _Test),
-                ("final_test", _Test), ("precision", C.c_int)]
+                ("final_test", _Test), ("precision", C.c_int), ("mixed_tol", C.c_double),
+                ("accuracy", C.c_double)]
 
 
 def lib():
@@ -111,6 +112,9 @@ def lib():
             "itp_hamiltonian_create": [_vp, C.c_double, C.POINTER(_vp)],
             "itp_hamiltonian_destroy": [_vp],
             "itp_energies": [_vp, _vp, _dp, _dp],
+            "itp_rayleigh_ritz": [_vp, _vp, _dp, _dp],
+            "itp_stateset_convert": [_vp, _vp],
+            "itp_stateset_overlap_eigenvalues": [_vp, _dp],
             "itp_energies_complex": [_vp, _vp, _dp, _dp],
             "itp_apply_expkinetic_host": [_vp, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _sz],
             "itp_apply_exppotential_host": [_vp, C.c_double, C.c_double, C.c_double, _dp, _sz],
@@ -132,6 +136,9 @@ def lib():
             "itps_states": [_vp, _dp],
             "itps_total_time": [_vp],
             "itps_recoveries": [_vp],
+            "itps_converged": [_vp, C.POINTER(C.c_int)],
+            "itps_rayleigh_ritz": [_vp],
+            "itps_mixed_steps": [_vp],
         }
         for name, args in sig.items():
             getattr(L, name).argtypes = args
@@ -322,6 +329,15 @@ class StateSet:
         rc = lib().itp_orthonormalize(self.h, C.byref(n), C.byref(v))
         self.grid.ctx.check(rc, "orthonormalize", bad=(n.value, v.value))
 
+    def assign(self, other):
+        """Copy the fields of another set of the same grid and size (converting precision)."""
+        self.grid.ctx.check(lib().itp_stateset_convert(self.h, other.h), "convert")
+
+    def overlap_eigenvalues(self):
+        lam = np.zeros(self.N)
+        self.grid.ctx.check(lib().itp_stateset_overlap_eigenvalues(self.h, _p(lam)), "overlap_eigenvalues")
+        return lam
+
     def overlap(self):
         out = np.zeros(self.N * self.N, dtype=np.complex128)
         self.grid.ctx.check(lib().itp_overlap(self.h, _p(out.view(np.float64))), "overlap")
@@ -373,6 +389,13 @@ class Hamiltonian:
         self.pot.grid.ctx.check(lib().itp_energies(self.h, states.h, _p(E), _p(sd)), "energies")
         return E, sd
 
+    def rayleigh_ritz(self, states):
+        """Rayleigh-Ritz on the set: (ascending Ritz values, residual norms); slot n becomes Ritz vector n."""
+        E = np.zeros(states.N)
+        sd = np.zeros(states.N)
+        self.pot.grid.ctx.check(lib().itp_rayleigh_ritz(self.h, states.h, _p(E), _p(sd)), "rayleigh_ritz")
+        return E, sd
+
     def energies_complex(self, states):
         m = np.zeros(2 * states.N)
         q = np.zeros(2 * states.N)
@@ -419,6 +442,8 @@ def _params(cfg):
     p.exhaust_eps = int(cfg.get("exhaust_eps", 0))
     p.recover = int(cfg.get("recover", 0))
     p.precision = int(cfg.get("precision", FP64))
+    p.mixed_tol = float(cfg.get("mixed_tol", 0.0))
+    p.accuracy = float(cfg.get("accuracy", 0.0))
     p.eps_values = _p(eps) if eps.size else None
     p.n_eps = eps.size
     for name in ["timestep_test", "final_test"]:
@@ -473,6 +498,21 @@ class ITPSystem:
     @property
     def recoveries(self):
         return lib().itps_recoveries(self.h)
+
+    @property
+    def mixed_steps(self):
+        """Steps of the mixed path's FP32 phase."""
+        return lib().itps_mixed_steps(self.h)
+
+    def rayleigh_ritz(self):
+        """Rayleigh-Ritz pass on the current states (bias-free Ritz values and residual norms)."""
+        self.ctx.check(lib().itps_rayleigh_ritz(self.h), "itps_rayleigh_ritz")
+
+    def converged(self):
+        """Final-convergence flag of each sorted position."""
+        f = np.zeros(int(self.cfg["N"]), dtype=np.int32)
+        lib().itps_converged(self.h, f.ctypes.data_as(C.POINTER(C.c_int)))
+        return f.astype(bool)
 
     def sorted(self):
         N = self.cfg["N"]

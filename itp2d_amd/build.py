@@ -1,6 +1,6 @@
 """Builds the gfx950 library itp2d_amd/lib/libitp2d_amd.so with hipcc.
 
-Sources: csrc/kernels.hip (kernels), csrc/capi.hip (C ABI of include/itp2d.h),
+Sources: csrc/kernels.hip, csrc/mixed.hip (kernels), csrc/capi.hip (C ABI of include/itp2d.h),
 host/itpsystem.cpp (ITP driver, C ABI of include/itp2d_system.h).
 """
 import os
@@ -14,6 +14,7 @@ LIB = os.path.join(LIB_DIR, "libitp2d_amd.so")
 
 SOURCES = [
     os.path.join(HERE, "csrc", "kernels.hip"),
+    os.path.join(HERE, "csrc", "mixed.hip"),
     os.path.join(HERE, "csrc", "capi.hip"),
     os.path.join(HERE, "host", "itpsystem.cpp"),
 ]

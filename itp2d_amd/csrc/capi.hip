@@ -66,6 +66,10 @@ struct itp_stateset {
   // orthonormalisation workspace
   int np = 0, nc = 0, nchunk = 1;
   long long kchunk = 0;
+  int nchunk32 = 1;             // split-K of the bf16x3 Gram (FP32 sets)
+  long long kchunk32 = 0;
+  cplx<float>* C32 = nullptr;   // FP32 copy of the coefficients (FP32 sets)
+  std::vector<double> lam_host; // ascending overlap eigenvalues of the last orthonormalization
   cplx<double>* part = nullptr;
   cplx<double>* A[2] = {nullptr, nullptr};
   cplx<double>* V[2] = {nullptr, nullptr};
@@ -340,7 +344,7 @@ int energies_impl(itp_hamiltonian* h, itp_stateset* s, double* E, double* sigma,
 
 // Overlap + eigendecomposition + coefficients; returns after the host has
 // checked the eigenvalues (stateset.cpp:145-177).
-int eigen_stage(itp_stateset* s, size_t* bad_n, double* bad_lambda, bool scale) {
+int eigen_stage(itp_stateset* s, size_t* bad_n, double* bad_lambda, bool scale, bool check_positive = true) {
   itp_ctx* c = s->grid->ctx;
   const int N = (int)s->N;
   Phase pe(c, ITP_PHASE_EIGENSOLVE);
@@ -386,8 +390,13 @@ int eigen_stage(itp_stateset* s, size_t* bad_n, double* bad_lambda, bool scale) 
   pe.stop();
   if (info[1] != 1.0 && !(info[1] == 2.0 && state[0] == 1))
     return set_err(c, ITP_E_EIGENSOLVER, "Jacobi eigensolver did not converge in " + std::to_string((int)info[0]) + " sweeps");
+  s->lam_host = lam;
   for (int n = 0; n < N; n++) {
     const double ev = lam[n];
+    if (!check_positive) {
+      if (!std::isfinite(ev)) return set_err(c, ITP_E_EIGENSOLVER, "non-finite eigenvalue");
+      continue;
+    }
     if (ev <= 0) {
       if (bad_n) *bad_n = n;
       if (bad_lambda) *bad_lambda = ev;
@@ -409,25 +418,112 @@ int gram_impl(itp_stateset* s) {
   const GridInfo& g = s->grid->g;
   Phase po(c, ITP_PHASE_OVERLAP);
   HIPCK(c, hipMemsetAsync(s->bad, 0, sizeof(int), c->stream));
-  HIPCK(c, launch_gram<R>((const cplx<R>*)s->buf[s->cur], g.M, (int)s->N, s->nc, s->part, s->nchunk, s->kchunk, g.dx,
+  if (sizeof(R) == 4) {
+    const cplx<float>* psi = (const cplx<float>*)s->buf[s->cur];
+    HIPCK(c, launch_gram_bf16x3(psi, psi, g.M, (int)s->N, s->nc, s->part, s->nchunk32, s->kchunk32, g.dx, s->A[0],
+                                s->np, s->bad, c->stream));
+    po.stop();
+    return ITP_OK;
+  }
+  HIPCK(c, launch_gram<R>((const cplx<R>*)s->buf[s->cur], (const cplx<R>*)s->buf[s->cur], g.M, (int)s->N, s->nc,
+                          s->part, s->nchunk, s->kchunk, g.dx,
                           s->A[0], s->np, s->bad, c->stream));
   po.stop();
   return ITP_OK;
 }
 
 template <class R>
+int rotate_set(itp_stateset* s, const cplx<R>* in, cplx<R>* out);
+
+template <class R>
 int ortho_impl(itp_stateset* s, size_t* bad_n, double* bad_lambda) {
   itp_ctx* c = s->grid->ctx;
-  const GridInfo& g = s->grid->g;
   int rc = gram_impl<R>(s);
   if (rc) return rc;
   rc = eigen_stage(s, bad_n, bad_lambda, true);
   if (rc) return rc;
   Phase pl(c, ITP_PHASE_LINCOMB);
-  HIPCK(c, launch_rotate<R>((const cplx<R>*)s->buf[s->cur], (cplx<R>*)s->buf[s->cur ^ 1], s->C, s->nc, g.M,
-                            (int)s->N, c->stream));
+  rc = rotate_set<R>(s, (const cplx<R>*)s->buf[s->cur], (cplx<R>*)s->buf[s->cur ^ 1]);
   pl.stop();
+  if (rc) return rc;
   s->cur ^= 1;
+  return ITP_OK;
+}
+
+// out = in C for N fields of the set's precision: FP64 MFMA for FP64 sets,
+// FP32 MFMA (with an FP32 copy of C) for FP32 sets.
+template <class R>
+int rotate_set(itp_stateset* s, const cplx<R>* in, cplx<R>* out) {
+  itp_ctx* c = s->grid->ctx;
+  const GridInfo& g = s->grid->g;
+  if (sizeof(R) == 4) {
+    HIPCK(c, launch_convert(s->C, s->C32, (long long)s->nc * s->nc, 1, c->stream));
+    HIPCK(c, launch_rotate_f32((const cplx<float>*)in, (cplx<float>*)out, s->C32, s->nc, g.M, (int)s->N, c->stream));
+    return ITP_OK;
+  }
+  HIPCK(c, launch_rotate<R>(in, out, s->C, s->nc, g.M, (int)s->N, c->stream));
+  return ITP_OK;
+}
+
+// Rayleigh-Ritz on the set (the mixed path's final pass; no counterpart in the
+// reference): orthonormalize as above, W = H Psi, H-matrix Psi^H W, its
+// eigenvectors U and ascending eigenvalues E (the Ritz values), Psi <- Psi U,
+// and the residual norms sigma_n = ||H psi_n - E_n psi_n|| (the energy
+// deviation: for a normalised psi_n with E_n = <psi_n|H|psi_n> it equals
+// sqrt(<H^2> - <H>^2), operators.cpp:40-50).
+template <class R>
+int rr_impl(itp_hamiltonian* h, itp_stateset* s, double* E, double* sigma) {
+  itp_grid* gr = s->grid;
+  itp_ctx* c = gr->ctx;
+  const GridInfo& g = gr->g;
+  const int N = (int)s->N;
+  int rc = ortho_impl<R>(s, nullptr, nullptr);
+  if (rc) return rc;
+  const std::vector<double> overlap_lam = s->lam_host;
+  const size_t fbytes = s->N * (size_t)g.M * sizeof(cplx<R>);
+  if ((rc = ensure_ws(c, 2 * fbytes))) return rc;
+  cplx<R>* W = (cplx<R>*)c->ws;
+  cplx<R>* T = (cplx<R>*)((char*)c->ws + fbytes);
+  const cplx<R>* psi = (const cplx<R>*)s->buf[s->cur];
+  {
+    Phase ph(c, ITP_PHASE_ENERGIES);
+    const Branch br = branch_of(h->B, g.bt);
+    const double* V = h->pot->null ? nullptr : h->pot->dV;
+    const LineGeom& lg = gr->lg[sizeof(R) == 4];
+    std::vector<Sweep<R>> prog;
+    hamiltonian_program<R>(g, lg, br, h->d, V, psi, T, W, nullptr, nullptr, prog);
+    GpuBackend be;
+    be.st = c->stream;
+    run_program<R>(be, g, lg, prog, N);
+    if (be.err != hipSuccess) return set_err(c, ITP_E_HIP, std::string("rayleigh-ritz: ") + hipGetErrorString(be.err));
+  }
+  {
+    // the magnetic Dirichlet kinetic operator of the reference is not
+    // Hermitian (kinetic.cpp:99-142): use the Hermitian part of the H-matrix,
+    // whose expectation values are the energies Re<psi|H psi>
+    const int full = branch_of(h->B, g.bt) == BR_BD ? 1 : 0;
+    Phase po(c, ITP_PHASE_OVERLAP);
+    HIPCK(c, hipMemsetAsync(s->bad, 0, sizeof(int), c->stream));
+    if (sizeof(R) == 4)
+      HIPCK(c, launch_gram_bf16x3((const cplx<float>*)psi, (const cplx<float>*)W, g.M, N, s->nc, s->part, s->nchunk32,
+                                  s->kchunk32, g.dx, s->A[0], s->np, s->bad, c->stream, full));
+    else
+      HIPCK(c, launch_gram<R>(psi, W, g.M, N, s->nc, s->part, s->nchunk, s->kchunk, g.dx, s->A[0], s->np, s->bad,
+                              c->stream, full));
+  }
+  rc = eigen_stage(s, nullptr, nullptr, false, false);
+  s->lam_host = overlap_lam;
+  if (rc) return rc;
+  {
+    Phase pl(c, ITP_PHASE_LINCOMB);
+    if ((rc = rotate_set<R>(s, psi, (cplx<R>*)s->buf[s->cur ^ 1]))) return rc;
+    if ((rc = rotate_set<R>(s, W, T))) return rc;
+  }
+  s->cur ^= 1;
+  HIPCK(c, launch_residual<R>(T, (const cplx<R>*)s->buf[s->cur], s->lam, g.M, N, g.dx, s->dE + s->N, c->stream));
+  if (E) HIPCK(c, hipMemcpyAsync(E, s->lam, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (sigma) HIPCK(c, hipMemcpyAsync(sigma, s->dE + s->N, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipStreamSynchronize(c->stream));
   return ITP_OK;
 }
 
@@ -715,6 +811,10 @@ int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out
     s->kchunk = ((long long)M + nchunk - 1) / nchunk;
     s->kchunk = (s->kchunk + 15) / 16 * 16;
     s->nchunk = (int)(((long long)M + s->kchunk - 1) / s->kchunk);
+    // bf16x3 Gram: chunks of whole 512-element groups, no more chunks than above
+    s->kchunk32 = ((long long)M + nchunk - 1) / nchunk;
+    s->kchunk32 = (s->kchunk32 + 511) / 512 * 512;
+    s->nchunk32 = (int)(((long long)M + s->kchunk32 - 1) / s->kchunk32);
     const size_t np2 = (size_t)s->np * s->np * 16;
     if ((rc = dalloc(c, (void**)&s->part, (size_t)s->nchunk * per_chunk))) return rc;
     for (int b = 0; b < 2; b++) {
@@ -740,6 +840,7 @@ int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out
       HIPCK(c, hipMemcpy(s->Vinit, I.data(), np2, hipMemcpyHostToDevice));
     }
     if ((rc = dalloc(c, (void**)&s->C, (size_t)s->nc * s->nc * 16))) return rc;
+    if (precision == ITP_FP32 && (rc = dalloc(c, (void**)&s->C32, (size_t)s->nc * s->nc * 8))) return rc;
     HIPCK(c, hipMemset(s->C, 0, (size_t)s->nc * s->nc * 16));
     if ((rc = dalloc(c, (void**)&s->lam, N * sizeof(double)))) return rc;
     if ((rc = dalloc(c, (void**)&s->perm, N * sizeof(int)))) return rc;
@@ -758,7 +859,7 @@ void itp_stateset_destroy(itp_stateset* s) {
   if (!s) return;
   (void)hipDeviceSynchronize();  // no parent access: parents may already be gone
   void* ptrs[] = {s->buf[0], s->buf[1], s->part, s->A[0], s->A[1], s->V[0], s->V[1], s->Vinit, s->bjQ, s->bjD,
-                  s->bjslots, s->bjident, s->oa_work, s->oa_lam, s->oa_slots, s->oa_state, s->C,
+                  s->bjslots, s->bjident, s->oa_work, s->oa_lam, s->oa_slots, s->oa_state, s->C, s->C32,
                   s->lam, s->perm, s->info, s->bad, s->epart, s->dE, s->dmean};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -870,6 +971,35 @@ int itp_overlap(itp_stateset* s, double* out) {
       for (size_t j = 0; j < N; j++) o[j * N + i] = S[i * s->np + j];  // column-major
     return ITP_OK;
   });
+}
+
+int itp_rayleigh_ritz(itp_hamiltonian* h, itp_stateset* s, double* E, double* sigma) {
+  if (!h || !s || h->pot->grid != s->grid) return ITP_E_INVALID;
+  itp_ctx* c = s->grid->ctx;
+  return guarded(c, [&]() -> int {
+    return s->prec == ITP_FP64 ? rr_impl<double>(h, s, E, sigma) : rr_impl<float>(h, s, E, sigma);
+  });
+}
+
+int itp_stateset_convert(itp_stateset* dst, const itp_stateset* src) {
+  if (!dst || !src || dst->grid != src->grid || dst->N != src->N) return ITP_E_INVALID;
+  itp_ctx* c = dst->grid->ctx;
+  return guarded(c, [&]() -> int {
+    const long long n = (long long)src->N * src->grid->g.M;  // complex elements
+    if (dst->prec == src->prec)
+      HIPCK(c, hipMemcpyAsync(dst->buf[dst->cur], src->buf[src->cur], (size_t)n * src->esz, hipMemcpyDeviceToDevice,
+                              c->stream));
+    else
+      HIPCK(c, launch_convert(src->buf[src->cur], dst->buf[dst->cur], n, dst->prec == ITP_FP32 ? 1 : 0, c->stream));
+    return ITP_OK;
+  });
+}
+
+int itp_stateset_overlap_eigenvalues(const itp_stateset* s, double* lam) {
+  if (!s || !lam) return ITP_E_INVALID;
+  if (s->lam_host.size() != s->N) return ITP_E_INVALID;
+  std::copy(s->lam_host.begin(), s->lam_host.end(), lam);
+  return ITP_OK;
 }
 
 int itp_hamiltonian_create(itp_potential* pot, double B, itp_hamiltonian** out) {

@@ -9,10 +9,26 @@ template <class R>
 hipError_t launch_sweep(const SweepArgs<R>& a, int nblk, int nfields, hipStream_t st);
 hipError_t launch_energy_reduce(const cplx<double>* p1, const cplx<double>* p2, int nblk, int N, double dx,
                                 double* E, double* sigma, cplx<double>* mean, cplx<double>* msq, hipStream_t st);
+// n complex elements: cplx<double> -> cplx<float> (to_f32) or back
 hipError_t launch_convert(const void* in, void* out, long long n, int to_f32, hipStream_t st);
+// S = dx^2 Psi^H Phi from the upper 64 x 64 tile pairs, mirrored (Hermitian
+// when Phi = Psi or Phi = H Psi with H Hermitian); with full, from all tile
+// pairs, and S is the Hermitian part (G + G^H) / 2.
 template <class R>
-hipError_t launch_gram(const cplx<R>* psi, long long M, int N, int nc, cplx<double>* part, int nchunk,
-                       long long kchunk, double dx, cplx<double>* S, int np, int* bad, hipStream_t st);
+hipError_t launch_gram(const cplx<R>* psi, const cplx<R>* phi, long long M, int N, int nc, cplx<double>* part,
+                       int nchunk, long long kchunk, double dx, cplx<double>* S, int np, int* bad, hipStream_t st,
+                       int full = 0);
+hipError_t launch_gram_reduce(const cplx<double>* part, int nchunk, int nc, int N, int np, double dx, cplx<double>* S,
+                              int* bad, hipStream_t st, int full = 0);
+// mixed path (mixed.hip)
+hipError_t launch_gram_bf16x3(const cplx<float>* A, const cplx<float>* B, long long M, int N, int nc,
+                              cplx<double>* part, int nchunk, long long kchunk, double dx, cplx<double>* S, int np,
+                              int* bad, hipStream_t st, int full = 0);
+hipError_t launch_rotate_f32(const cplx<float>* in, cplx<float>* out, const cplx<float>* C, int nc, long long M, int N,
+                             hipStream_t st);
+template <class R>
+hipError_t launch_residual(const cplx<R>* W, const cplx<R>* Psi, const double* E, long long M, int N, double dx,
+                           double* sigma, hipStream_t st);
 hipError_t launch_block_jacobi(cplx<double>* A, cplx<double>* V, cplx<double>* Q, cplx<double>* D, int* ident,
                                double* slots, int np, int max_sweeps, int max_inner, double tol, double handoff,
                                double* info, int grid_wgs, hipStream_t st);

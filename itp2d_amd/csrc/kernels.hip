@@ -4,7 +4,7 @@
 //                    line_ops.hpp (LDS-resident FFT / DST / DCT passes fused
 //                    with the multiplier tables and the epilogue)
 //  energy_reduce     E and sigma per state from the sweep partial sums
-//  gram_kernel       overlap S = Psi^H Psi on FP64 MFMA (v_mfma_f64_16x16x4f64),
+//  gram_kernel       overlap S = Psi^H Phi on FP64 MFMA (v_mfma_f64_16x16x4f64),
 //                    64x64 tiles of state pairs, split over the grid points
 //  gram_reduce       sums the split-K partials, scales by dx^2, fills the
 //                    Hermitian matrix for the eigensolver
@@ -157,11 +157,17 @@ hipError_t launch_convert(const void* in, void* out, long long n, int to_f32, hi
 // Re += a.x b.x + a.y b.y, Im += a.x b.y - a.y b.x.
 // ---------------------------------------------------------------------------
 template <class R>
-__global__ __launch_bounds__(256) void gram_kernel(const cplx<R>* psi, long long M, int N, int nb, long long kchunk,
-                                                   cplx<double>* part, int nc) {
-  int bi = 0, rem = blockIdx.x;
-  while (rem >= nb - bi) { rem -= nb - bi; bi++; }
-  const int bj = bi + rem;
+__global__ __launch_bounds__(256) void gram_kernel(const cplx<R>* psi, const cplx<R>* phi, long long M, int N, int nb,
+                                                   long long kchunk, cplx<double>* part, int nc, int full) {
+  int bi = 0, bj = 0;
+  if (full) {
+    bi = blockIdx.x / nb;
+    bj = blockIdx.x - bi * nb;
+  } else {
+    int rem = blockIdx.x;
+    while (rem >= nb - bi) { rem -= nb - bi; bi++; }
+    bj = bi + rem;
+  }
   __shared__ cplx<double> As[64][17];
   __shared__ cplx<double> Bs[64][17];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wi = wave >> 1, wj = wave & 1;
@@ -181,7 +187,7 @@ __global__ __launch_bounds__(256) void gram_kernel(const cplx<R>* psi, long long
       cplx<double> va = mk<double>(0.0, 0.0), vb = va;
       if (t < k1) {
         if (ia < N) { const cplx<R> x = psi[(long long)ia * M + t]; va = mk<double>(double(x.x), double(x.y)); }
-        if (ib < N) { const cplx<R> x = psi[(long long)ib * M + t]; vb = mk<double>(double(x.x), double(x.y)); }
+        if (ib < N) { const cplx<R> x = phi[(long long)ib * M + t]; vb = mk<double>(double(x.x), double(x.y)); }
       }
       As[s][k] = va;
       Bs[s][k] = vb;
@@ -213,12 +219,17 @@ __global__ __launch_bounds__(256) void gram_kernel(const cplx<R>* psi, long long
 }
 
 __global__ void gram_reduce_kernel(const cplx<double>* part, int nchunk, int nc, int N, int np, double dx,
-                                   cplx<double>* S, int* bad) {
+                                   cplx<double>* S, int* bad, int full) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
   if (j >= N || i > j) return;
   cplx<double> s = mk<double>(0.0, 0.0);
   for (int z = 0; z < nchunk; z++) s = s + part[((long long)z * nc + i) * nc + j];
+  if (full) {  // Hermitian part (G + G^H) / 2 of a matrix computed in full
+    cplx<double> t = mk<double>(0.0, 0.0);
+    for (int z = 0; z < nchunk; z++) t = t + part[((long long)z * nc + j) * nc + i];
+    s = (s + conj(t)) * 0.5;
+  }
   s = (s * dx) * dx;
   if (!(std::isfinite(s.x) && std::isfinite(s.y))) atomicOr(bad, 1);
   if (i == j) {
@@ -230,21 +241,28 @@ __global__ void gram_reduce_kernel(const cplx<double>* part, int nchunk, int nc,
 }
 
 template <class R>
-hipError_t launch_gram(const cplx<R>* psi, long long M, int N, int nc, cplx<double>* part, int nchunk,
-                       long long kchunk, double dx, cplx<double>* S, int np, int* bad, hipStream_t st) {
+hipError_t launch_gram(const cplx<R>* psi, const cplx<R>* phi, long long M, int N, int nc, cplx<double>* part,
+                       int nchunk, long long kchunk, double dx, cplx<double>* S, int np, int* bad, hipStream_t st,
+                       int full) {
   const int nb = nc / 64;
-  const int npairs = nb * (nb + 1) / 2;
-  hipLaunchKernelGGL(gram_kernel<R>, dim3(npairs, nchunk), dim3(256), 0, st, psi, M, N, nb, kchunk, part, nc);
+  const int npairs = full ? nb * nb : nb * (nb + 1) / 2;
+  hipLaunchKernelGGL(gram_kernel<R>, dim3(npairs, nchunk), dim3(256), 0, st, psi, phi, M, N, nb, kchunk, part, nc,
+                     full);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
+  return launch_gram_reduce(part, nchunk, nc, N, np, dx, S, bad, st, full);
+}
+
+hipError_t launch_gram_reduce(const cplx<double>* part, int nchunk, int nc, int N, int np, double dx, cplx<double>* S,
+                              int* bad, hipStream_t st, int full) {
   hipLaunchKernelGGL(gram_reduce_kernel, dim3((N + 127) / 128, N), dim3(128), 0, st, part, nchunk, nc, N, np, dx, S,
-                     bad);
+                     bad, full);
   return hipGetLastError();
 }
-template hipError_t launch_gram<double>(const cplx<double>*, long long, int, int, cplx<double>*, int, long long,
-                                        double, cplx<double>*, int, int*, hipStream_t);
-template hipError_t launch_gram<float>(const cplx<float>*, long long, int, int, cplx<double>*, int, long long,
-                                       double, cplx<double>*, int, int*, hipStream_t);
+template hipError_t launch_gram<double>(const cplx<double>*, const cplx<double>*, long long, int, int, cplx<double>*,
+                                        int, long long, double, cplx<double>*, int, int*, hipStream_t, int);
+template hipError_t launch_gram<float>(const cplx<float>*, const cplx<float>*, long long, int, int, cplx<double>*, int,
+                                       long long, double, cplx<double>*, int, int*, hipStream_t, int);
 
 // ---------------------------------------------------------------------------
 // Block Jacobi (block_jacobi.hpp), one cooperative launch for all sweeps:

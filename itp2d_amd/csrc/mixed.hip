@@ -1,0 +1,301 @@
+// gfx950 kernels of the mixed-precision path (complex FP32 fields).
+//
+//  gram_bf16x3_kernel  G = A^H B for 64 x 64 tiles of state pairs (I <= J, or
+//                      all pairs for a non-Hermitian product), split over the
+//                      grid points. Each FP32 value is split
+//                      into three bfloat16 parts x = x1 + x2 + x3; the six
+//                      products x_p y_q with p + q <= 4 (1-based) run on
+//                      v_mfma_f32_16x16x32_bf16 and carry FP32 accuracy.
+//                      The complex product is two real dot products over the
+//                      interleaved (re, im) sequence: Re = sum a.b,
+//                      Im = sum a.b' with b' = (b_im, -b_re) per element.
+//                      MFMA partial sums are FP32 over one 64-real step, then
+//                      added in FP64; the chunk partials are reduced in FP64.
+//  rotate_f32_kernel   Psi_out = Psi_in C on v_mfma_f32_16x16x4_f32.
+//  residual_kernel     sigma_n = ||W_n - E_n Psi_n|| dx (FP64 sums).
+//
+// Lane layouts (measured on gfx950): for both v_mfma_f32_16x16x4_f32 (KL = 1)
+// and v_mfma_f32_16x16x32_bf16 (KL = 8 elements per lane), A[i][k] comes from
+// lane i + 16 (k / KL), element k % KL; B[k][j] from lane j + 16 (k / KL),
+// element k % KL; D[i][j] lands in lane j + 16 (i / 4), register i % 4.
+#include <hip/hip_runtime.h>
+
+#include "gpu.hpp"
+#include "line_ops.hpp"
+
+namespace itp {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef double d4m __attribute__((ext_vector_type(4)));
+typedef __bf16 bf4v __attribute__((ext_vector_type(4)));
+
+constexpr int kG3Rows = 64;     // states per tile side
+constexpr int kG3KT = 64;       // reals per K step
+constexpr int kG3Pad = 8;       // bf16 row padding in LDS
+constexpr int kG3Sub = 64;      // reals per FP32 partial sum (one K step: 12 MFMA roundings)
+
+// x = x1 + x2 + x3 in bfloat16 (round to nearest each time)
+__device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l) {
+  h = (__bf16)x;
+  const float r1 = x - (float)h;
+  m = (__bf16)r1;
+  l = (__bf16)(r1 - (float)m);
+}
+
+// (b0, b1, b2, ...) -> (b1, -b0, b3, -b2, ...): the operand of the imaginary part
+__device__ __forceinline__ bf8 swapneg(bf8 b) {
+  bf8 r;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    r[2 * c] = b[2 * c + 1];
+    r[2 * c + 1] = -b[2 * c];
+  }
+  return r;
+}
+
+__global__ __launch_bounds__(256) void gram_bf16x3_kernel(const float* A, const float* B, long long M, int N, int nb,
+                                                          long long kchunk, cplx<double>* part, int nc, int full) {
+  __shared__ __align__(16) __bf16 As[3][kG3Rows][kG3KT + kG3Pad];
+  __shared__ __align__(16) __bf16 Bs[3][kG3Rows][kG3KT + kG3Pad];
+  // tile pair from blockIdx.x: all of them if full, else I <= J
+  int I = 0, J = 0;
+  if (full) {
+    I = blockIdx.x / nb;
+    J = blockIdx.x - I * nb;
+  } else {
+    int rem = blockIdx.x;
+    while (rem >= nb - I) { rem -= nb - I; I++; }
+    J = I + rem;
+  }
+  const int chunk = blockIdx.y;
+  const long long K2 = 2 * M;  // reals per field
+  const long long k_begin = (long long)chunk * 2 * kchunk;
+  const long long k_end = k_begin + 2 * kchunk < K2 ? k_begin + 2 * kchunk : K2;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wi = w >> 1, wj = w & 1;
+  const int i0 = I * kG3Rows, j0 = J * kG3Rows;
+  f4 re[2][2], im[2][2];
+  d4m dre[2][2], dim[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      re[a][b] = f4{0, 0, 0, 0};
+      im[a][b] = f4{0, 0, 0, 0};
+      dre[a][b] = d4m{0, 0, 0, 0};
+      dim[a][b] = d4m{0, 0, 0, 0};
+    }
+  int sub = 0;
+  for (long long k0 = k_begin; k0 < k_end; k0 += kG3KT) {
+    // stage 64 rows x 64 reals of A and of B as three bf16 parts
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int idx = tid + 256 * j;
+      const int row = idx >> 4, c4 = idx & 15;
+      const long long kk = k0 + 4 * c4;
+      float4 va = make_float4(0.f, 0.f, 0.f, 0.f), vb = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (kk < k_end) {  // k_end - kk is a multiple of 4 (2 M and the chunk bounds are)
+        if (i0 + row < N) va = *reinterpret_cast<const float4*>(A + (long long)(i0 + row) * K2 + kk);
+        if (j0 + row < N) vb = *reinterpret_cast<const float4*>(B + (long long)(j0 + row) * K2 + kk);
+      }
+      const float xa[4] = {va.x, va.y, va.z, va.w}, xb[4] = {vb.x, vb.y, vb.z, vb.w};
+      bf4v ah, am, al, bh, bm, bl;
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        __bf16 h, m, l;
+        split3(xa[e], h, m, l);
+        ah[e] = h; am[e] = m; al[e] = l;
+        split3(xb[e], h, m, l);
+        bh[e] = h; bm[e] = m; bl[e] = l;
+      }
+      *reinterpret_cast<bf4v*>(&As[0][row][4 * c4]) = ah;
+      *reinterpret_cast<bf4v*>(&As[1][row][4 * c4]) = am;
+      *reinterpret_cast<bf4v*>(&As[2][row][4 * c4]) = al;
+      *reinterpret_cast<bf4v*>(&Bs[0][row][4 * c4]) = bh;
+      *reinterpret_cast<bf4v*>(&Bs[1][row][4 * c4]) = bm;
+      *reinterpret_cast<bf4v*>(&Bs[2][row][4 * c4]) = bl;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < kG3KT / 32; ks++) {
+      const int kcol = ks * 32 + 8 * (lane >> 4);
+      bf8 a[3][2], b[3][2], bp[3][2];
+#pragma unroll
+      for (int s = 0; s < 3; s++)
+#pragma unroll
+        for (int x = 0; x < 2; x++) {
+          a[s][x] = *reinterpret_cast<const bf8*>(&As[s][wi * 32 + x * 16 + (lane & 15)][kcol]);
+          b[s][x] = *reinterpret_cast<const bf8*>(&Bs[s][wj * 32 + x * 16 + (lane & 15)][kcol]);
+          bp[s][x] = swapneg(b[s][x]);
+        }
+#pragma unroll
+      for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++) {
+          // split products with p + q <= 4 (1-based): (1,1) (1,2) (2,1) (2,2) (1,3) (3,1)
+          re[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][x], b[0][y], re[x][y], 0, 0, 0);
+          re[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], b[2][y], re[x][y], 0, 0, 0);
+          re[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][x], b[1][y], re[x][y], 0, 0, 0);
+          re[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][x], b[0][y], re[x][y], 0, 0, 0);
+          re[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], b[1][y], re[x][y], 0, 0, 0);
+          re[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], b[0][y], re[x][y], 0, 0, 0);
+          im[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][x], bp[0][y], im[x][y], 0, 0, 0);
+          im[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], bp[2][y], im[x][y], 0, 0, 0);
+          im[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][x], bp[1][y], im[x][y], 0, 0, 0);
+          im[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][x], bp[0][y], im[x][y], 0, 0, 0);
+          im[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], bp[1][y], im[x][y], 0, 0, 0);
+          im[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], bp[0][y], im[x][y], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+    // FP32 partial sums span at most kG3Sub reals, then move to FP64
+    if (++sub == kG3Sub / kG3KT) {
+      sub = 0;
+#pragma unroll
+      for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            dre[x][y][v] += (double)re[x][y][v];
+            dim[x][y][v] += (double)im[x][y][v];
+            re[x][y][v] = 0.f;
+            im[x][y][v] = 0.f;
+          }
+    }
+  }
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int i = i0 + wi * 32 + x * 16 + 4 * (lane >> 4) + v;
+        const int j = j0 + wj * 32 + y * 16 + (lane & 15);
+        part[((long long)chunk * nc + i) * nc + j] =
+            mk<double>(dre[x][y][v] + (double)re[x][y][v], dim[x][y][v] + (double)im[x][y][v]);
+      }
+}
+
+hipError_t launch_gram_bf16x3(const cplx<float>* A, const cplx<float>* B, long long M, int N, int nc,
+                              cplx<double>* part, int nchunk, long long kchunk, double dx, cplx<double>* S, int np,
+                              int* bad, hipStream_t st, int full) {
+  if (nc % kG3Rows || (2 * kchunk) % kG3Sub || M % 2) return hipErrorInvalidValue;
+  const int nb = nc / kG3Rows;
+  const int npairs = full ? nb * nb : nb * (nb + 1) / 2;
+  hipLaunchKernelGGL(gram_bf16x3_kernel, dim3(npairs, nchunk), dim3(256), 0, st, reinterpret_cast<const float*>(A),
+                     reinterpret_cast<const float*>(B), M, N, nb, kchunk, part, nc, full);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_gram_reduce(part, nchunk, nc, N, np, dx, S, bad, st, full);
+}
+
+// ---------------------------------------------------------------------------
+// Psi_out[r][t] = sum_c C[c][r] Psi_in[c][t] for complex FP32 fields and an
+// FP32 copy of the coefficients, on v_mfma_f32_16x16x4_f32: 64 output states
+// x 64 grid points per workgroup, one 32 x 32 quarter per wavefront.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void rotate_f32_kernel(const cplx<float>* in, cplx<float>* out, const cplx<float>* C,
+                                                         int nc, long long M, int N) {
+  __shared__ cplx<float> Cs[16][65];
+  __shared__ cplx<float> Ps[16][65];
+  const long long t0 = (long long)blockIdx.x * 64;
+  const int r0 = blockIdx.y * 64;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wr = wave >> 1, wt = wave & 1;
+  const int li = lane & 15, lk = lane >> 4;
+  f4 accr[2][2], acci[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      accr[a][b] = f4{0, 0, 0, 0};
+      acci[a][b] = f4{0, 0, 0, 0};
+    }
+  for (int c0 = 0; c0 < N; c0 += 16) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int e = tid + 256 * q;
+      const int kc = e >> 6, x = e & 63;
+      const int c = c0 + kc;
+      Cs[kc][x] = (c < N) ? C[(long long)c * nc + r0 + x] : mk<float>(0.f, 0.f);
+      Ps[kc][x] = (c < N && t0 + x < M) ? in[(long long)c * M + t0 + x] : mk<float>(0.f, 0.f);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) {
+      cplx<float> a[2], b[2];
+#pragma unroll
+      for (int mi = 0; mi < 2; mi++) a[mi] = Cs[kk * 4 + lk][wr * 32 + mi * 16 + li];
+#pragma unroll
+      for (int nj = 0; nj < 2; nj++) b[nj] = Ps[kk * 4 + lk][wt * 32 + nj * 16 + li];
+#pragma unroll
+      for (int mi = 0; mi < 2; mi++)
+#pragma unroll
+        for (int nj = 0; nj < 2; nj++) {
+          accr[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[mi].x, b[nj].x, accr[mi][nj], 0, 0, 0);
+          accr[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x4f32(-a[mi].y, b[nj].y, accr[mi][nj], 0, 0, 0);
+          acci[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[mi].x, b[nj].y, acci[mi][nj], 0, 0, 0);
+          acci[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[mi].y, b[nj].x, acci[mi][nj], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int mi = 0; mi < 2; mi++)
+#pragma unroll
+    for (int nj = 0; nj < 2; nj++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int r = r0 + wr * 32 + mi * 16 + 4 * lk + v;
+        const long long t = t0 + wt * 32 + nj * 16 + li;
+        if (r < N && t < M) out[(long long)r * M + t] = mk<float>(accr[mi][nj][v], acci[mi][nj][v]);
+      }
+}
+
+hipError_t launch_rotate_f32(const cplx<float>* in, cplx<float>* out, const cplx<float>* C, int nc, long long M, int N,
+                             hipStream_t st) {
+  const long long tiles = (M + 63) / 64;
+  if (tiles > 0x7fffffffLL || nc % 64) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(rotate_f32_kernel, dim3((unsigned)tiles, nc / 64), dim3(256), 0, st, in, out, C, nc, M, N);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// sigma_n = dx ||W_n - E_n Psi_n||: one workgroup per state, FP64 sums.
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ __launch_bounds__(256) void residual_kernel(const cplx<R>* W, const cplx<R>* Psi, const double* E,
+                                                       long long M, double dx, double* sigma) {
+  __shared__ double red[256];
+  const int n = blockIdx.x;
+  const double e = E[n];
+  const cplx<R>* w = W + (long long)n * M;
+  const cplx<R>* p = Psi + (long long)n * M;
+  double acc = 0;
+  for (long long i = threadIdx.x; i < M; i += 256) {
+    const double rx = double(w[i].x) - e * double(p[i].x);
+    const double ry = double(w[i].y) - e * double(p[i].y);
+    acc += rx * rx + ry * ry;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = 128; h >= 1; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sigma[n] = std::sqrt(red[0]) * dx;
+}
+
+template <class R>
+hipError_t launch_residual(const cplx<R>* W, const cplx<R>* Psi, const double* E, long long M, int N, double dx,
+                           double* sigma, hipStream_t st) {
+  hipLaunchKernelGGL(residual_kernel<R>, dim3(N), dim3(256), 0, st, W, Psi, E, M, dx, sigma);
+  return hipGetLastError();
+}
+template hipError_t launch_residual<double>(const cplx<double>*, const cplx<double>*, const double*, long long, int,
+                                            double, double*, hipStream_t);
+template hipError_t launch_residual<float>(const cplx<float>*, const cplx<float>*, const double*, long long, int,
+                                           double, double*, hipStream_t);
+
+}  // namespace itp

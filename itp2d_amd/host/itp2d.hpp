@@ -143,6 +143,13 @@ class StateSet {
     if (rc == ITP_E_NONNORMAL_EIG) throw NonNormalEigenvalue(n, v);
     tr_.context().check(rc, "orthonormalize");
   }
+  // Ascending overlap eigenvalues of the last orthonormalization.
+  void overlap_eigenvalues(std::vector<double>& lam) const {
+    lam.resize(N);
+    tr_.context().check(itp_stateset_overlap_eigenvalues(s_, lam.data()), "overlap_eigenvalues");
+  }
+  // Fields of other (same grid and size), converted to this set's precision.
+  void assign(const StateSet& other) { tr_.context().check(itp_stateset_convert(s_, other.get()), "convert"); }
   itp_stateset* get() const { return s_; }
   Transformer& transformer() const { return tr_; }
   const size_t N;
@@ -194,6 +201,13 @@ class Hamiltonian {
     E.resize(s.N);
     sigma.resize(s.N);
     pot_.transformer().context().check(itp_energies(h_, s.get(), E.data(), sigma.data()), "energies");
+  }
+  // Rayleigh-Ritz on the set: slot n becomes the Ritz vector of E[n]
+  // (ascending), sigma[n] its residual norm.
+  void rayleigh_ritz(StateSet& s, std::vector<double>& E, std::vector<double>& sigma) {
+    E.resize(s.N);
+    sigma.resize(s.N);
+    pot_.transformer().context().check(itp_rayleigh_ritz(h_, s.get(), E.data(), sigma.data()), "rayleigh_ritz");
   }
  private:
   Potential& pot_;

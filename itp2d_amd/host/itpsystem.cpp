@@ -33,6 +33,12 @@ class ITPSystem {
     if (!eps_values.empty()) { eps = eps_values.front(); eps_values.pop_front(); }
     else eps = 0.5;  // Parameters::default_initial_eps (parameters.cpp:49)
     T_.reset(new MultiProductSplit(p.halforder, *pot_, eps, p.B));
+    mixed = p.precision == ITP_FP32;
+    if (mixed) {
+      states64_.reset(new StateSet(*tr_, p.N, FP64));
+      if (!(params.mixed_tol > 0)) params.mixed_tol = 2e-7;
+      if (!(params.accuracy > 0)) params.accuracy = 1e-8;
+    }
     rng = p.seed;
     host_states.resize(p.N * p.sx * p.sy);
     if (init) std::memcpy(host_states.data(), init, host_states.size() * sizeof(comp));
@@ -50,6 +56,8 @@ class ITPSystem {
     total_step_counter++;
     T_->operate(*states_);          // propagate (itpsystem.cpp:243-256)
     orthonormalize();               // itpsystem.cpp:258-289
+    if (finished) return;
+    if (mixed) { mixed_energies_and_tests(); return; }
     calculate_energies();           // itpsystem.cpp:473-511
     check_timestep_convergence();
     if (all_ts) {
@@ -78,6 +86,9 @@ class ITPSystem {
   void download(comp* out) { states_->download(out); }
 
   itps_params params;
+  bool mixed = false, just_recovered = false;
+  int mixed_steps = 0, stall = 0;
+  double best_change = 1e300;
   std::vector<double> eps_list;
   double eps = 0;
   bool finished = false, error_flag = false;
@@ -131,6 +142,7 @@ class ITPSystem {
     } catch (NonPositiveEigenvalue&) {
       if (params.recover) {
         recoveries++;
+        just_recovered = true;
         change_time_step();
         init_noise();
         states_->upload(host_states.data());
@@ -151,13 +163,98 @@ class ITPSystem {
     esn.resize(params.N);
     for (size_t n = 0; n < params.N; n++) esn[n] = Esn_tuple(E[n], sd[n], n);
     std::sort(esn.begin(), esn.end());
-    if (energies.size() < size_t(total_step_counter)) energies.resize(total_step_counter);
-    if (stdevs.size() < size_t(total_step_counter)) stdevs.resize(total_step_counter);
-    auto& ne = energies[total_step_counter - 1];
-    auto& nd = stdevs[total_step_counter - 1];
-    ne.clear();
-    nd.clear();
-    for (auto& t : esn) { ne.push_back(std::get<0>(t)); nd.push_back(std::get<1>(t)); }
+    push_history();
+  }
+
+  // Mixed path: energies from the overlap eigenvalues of this step's
+  // orthonormalization, lambda_n = |T psi_n|^2 ~ exp(-2 eps E_n) for the
+  // Ritz-like states the canonical orthonormalization produces; slot r of the
+  // set holds the eigenvector of the r-th smallest lambda.
+  void mixed_energies_and_tests() {
+    if (just_recovered) {  // fresh noise: no energies this step
+      just_recovered = false;
+      return;
+    }
+    std::vector<double> lam;
+    states_->overlap_eigenvalues(lam);
+    esn.resize(params.N);
+    for (size_t r = 0; r < params.N; r++) esn[r] = Esn_tuple(-std::log(lam[r]) / (2 * eps), 0.0, r);
+    std::sort(esn.begin(), esn.end());
+    push_history();
+    mixed_steps++;
+    if (energies.size() < 2) return;
+    const auto& a = energies[energies.size() - 1];
+    const auto& b = energies[energies.size() - 2];
+    double worst = 0;
+    for (size_t n = params.ignore_lowest; n < params.ignore_lowest + params.n_wanted; n++) {
+      const double r = std::fabs((a[n] - b[n]) / b[n]);
+      worst = std::isfinite(r) ? std::max(worst, r) : 1e300;
+    }
+    // switch when every wanted state changes by less than mixed_tol, or when
+    // the largest change has not reached a new low for kStall steps (the
+    // FP32 rounding floor)
+    constexpr int kStall = 6;
+    if (worst < best_change) { best_change = worst; stall = 0; }
+    else stall++;
+    if (!(worst < params.mixed_tol) && stall < kStall) return;
+    // FP32 phase done: continue with FP64 steps of the reference algorithm on
+    // an FP64 copy of the states until its own tests pass. The energies the
+    // reference reports are those of the fixed point of the split propagator
+    // (they carry its O(eps^(2h)) splitting error), so parity needs that fixed
+    // point; from FP32-converged states its energies are within second-order
+    // terms, and the FP64 steps only confirm convergence.
+    states64_->assign(*states_);
+    std::swap(states_, states64_);
+    mixed = false;
+    energies.clear();
+    stdevs.clear();
+    step_counter = 0;
+    all_ts = false;
+    for (size_t n = 0; n < params.N; n++) { set_ts(n, false); set_fin(n, false); }
+  }
+
+ public:
+  // Rayleigh-Ritz pass on the current states (exact discrete eigenvalues of
+  // the Hermitian part of H in their span); converged states by the
+  // eigenvalue error bound sigma^2 / gap <= accuracy |E|.
+  void rayleigh_ritz_pass() {
+    std::vector<double> E, sd;
+    H_->rayleigh_ritz(*states_, E, sd);
+    esn.resize(params.N);
+    for (size_t n = 0; n < params.N; n++) esn[n] = Esn_tuple(E[n], sd[n], n);
+    std::sort(esn.begin(), esn.end());
+    push_history();
+    if (params.accuracy <= 0) params.accuracy = 1e-8;
+    // Ritz values closer than 1e-7 relative form a cluster (a degenerate or
+    // nearly degenerate level); a cluster's error bound uses its summed
+    // squared residuals and its distance to the other Ritz values.
+    const size_t N = params.N;
+    size_t b = 0;
+    while (b < N) {
+      size_t e = b + 1;
+      while (e < N && std::get<0>(esn[e]) - std::get<0>(esn[e - 1]) <= 1e-7 * std::fabs(std::get<0>(esn[e]))) e++;
+      double gap = 1e300, r2 = 0;
+      if (b > 0) gap = std::min(gap, std::get<0>(esn[b]) - std::get<0>(esn[b - 1]));
+      if (e < N) gap = std::min(gap, std::get<0>(esn[e]) - std::get<0>(esn[e - 1]));
+      for (size_t n = b; n < e; n++) r2 += std::get<1>(esn[n]) * std::get<1>(esn[n]);
+      for (size_t n = b; n < e; n++) {
+        const bool ok = gap > 0 && r2 / gap <= params.accuracy * std::fabs(std::get<0>(esn[n]));
+        set_fin(std::get<2>(esn[n]), ok);
+        set_ts(std::get<2>(esn[n]), ok);
+      }
+      b = e;
+    }
+  }
+
+  // converged flag of sorted position n (after the final stage)
+  bool converged_sorted(size_t n) const { return n < esn.size() && fin_conv[std::get<2>(esn[n])]; }
+
+ private:
+
+  void push_history() {
+    energies.emplace_back();
+    stdevs.emplace_back();
+    for (auto& t : esn) { energies.back().push_back(std::get<0>(t)); stdevs.back().push_back(std::get<1>(t)); }
   }
 
   // ConvergenceTest::test for sorted position n (convergence.cpp:75-200)
@@ -211,6 +308,7 @@ class ITPSystem {
   std::unique_ptr<Transformer> tr_;
   std::unique_ptr<Potential> pot_;
   std::unique_ptr<StateSet> states_;
+  std::unique_ptr<StateSet> states64_;  // mixed path: the other precision's set (FP64 for the final stage)
   std::unique_ptr<Hamiltonian> H_;
   std::unique_ptr<MultiProductSplit> T_;
   std::vector<comp> host_states;
@@ -292,5 +390,21 @@ int itps_states(itps_system* s, double* out) {
 }
 double itps_total_time(const itps_system* s) { return s ? s->sys->total_time : 0; }
 int itps_recoveries(const itps_system* s) { return s ? s->sys->recoveries : 0; }
+int itps_mixed_steps(const itps_system* s) { return s ? s->sys->mixed_steps : 0; }
+int itps_rayleigh_ritz(itps_system* s) {
+  if (!s) return ITP_E_INVALID;
+  try {
+    s->sys->rayleigh_ritz_pass();
+  } catch (const std::exception&) {
+    return ITP_E_HIP;
+  }
+  return ITP_OK;
+}
+
+int itps_converged(const itps_system* s, int* flags) {
+  if (!s || !flags) return ITP_E_INVALID;
+  for (size_t n = 0; n < s->sys->params.N; n++) flags[n] = s->sys->converged_sorted(n) ? 1 : 0;
+  return ITP_OK;
+}
 
 }  // extern "C"

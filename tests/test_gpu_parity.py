@@ -292,3 +292,127 @@ def test_c2_solve_refinement_matches_jacobi(ctx):
     assert st0 == st1 and c0 == c1
     assert np.abs(E0 - E1).max() < 1e-10 * np.abs(E1).max()
     assert refined > 0
+
+
+# ---------------------------------------------------------------------------
+# mixed-precision path
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("N,sx", [(7, 64), (70, 64), (130, 32)])
+def test_gram_bf16x3(ctx, N, sx):
+    """Split-BF16x3 overlap of an FP32 set against the FP64 overlap of the same
+    FP32-rounded fields: FP32-level agreement (1e-6 of the largest entry)."""
+    say(f"bf16x3 gram N={N} sx={sx}")
+    dx = 10.0 / sx
+    g = I.Grid(ctx, sx, sx, dx, 1)
+    f = rand_states(N, sx, sx, dx, 21).astype(np.complex64).astype(np.complex128)
+    S = I.StateSet(g, N, I.FP32)
+    S.upload(f)
+    G = S.overlap()
+    A = f.reshape(N, -1)
+    Gr = (A.conj() @ A.T) * dx * dx
+    assert np.abs(G - Gr).max() < 1e-6 * np.abs(Gr).max()
+    assert np.abs(G - G.conj().T).max() == 0
+
+
+def test_orthonormalize_fp32(ctx):
+    """FP32 set: bf16x3 overlap, FP64 eigensolver, FP32-MFMA rotation give a
+    set orthonormal to FP32 accuracy spanning the same space."""
+    say("orthonormalize fp32")
+    sx, N = 64, 40
+    dx = 12.0 / sx
+    g = I.Grid(ctx, sx, sx, dx, 0)
+    f = rand_states(N, sx, sx, dx, 31)
+    S = I.StateSet(g, N, I.FP32)
+    S.upload(f)
+    S.orthonormalize()
+    out = S.download()
+    A = out.reshape(N, -1)
+    G = (A.conj() @ A.T) * dx * dx
+    assert np.abs(G - np.eye(N)).max() < 2e-5
+    # same span: the input lies in the span of the output
+    B = f.reshape(N, -1)
+    proj = (A.conj() @ B.T) * dx * dx
+    resid = B - proj.T @ A
+    assert np.abs(resid).max() < 2e-5 * np.abs(B).max()
+    lam = S.overlap_eigenvalues()
+    assert np.all(np.diff(lam) >= 0)
+
+
+@pytest.mark.parametrize("bt,B", BRANCHES)
+def test_rayleigh_ritz_fp64(ctx, bt, B):
+    """FP64 Rayleigh-Ritz against the oracle: Ritz values of the H-matrix built
+    from the oracle's H applied to the orthonormalized set, and residual norms."""
+    say(f"rayleigh-ritz bt={bt} B={B}")
+    sx, N = 48, 12
+    dx = 12.0 / sx
+    g = I.Grid(ctx, sx, sx, dx, bt)
+    V = W.harmonic(sx, sx, dx)
+    pot = I.Potential(g, V)
+    H = I.Hamiltonian(pot, B)
+    f = rand_states(N, sx, sx, dx, 41)
+    S = I.StateSet(g, N)
+    S.upload(f)
+    E, sd = H.rayleigh_ritz(S)
+    out = S.download()
+    # reference: orthonormal basis of the same span, H-matrix with the oracle's H
+    Q, _ = np.linalg.qr(f.reshape(N, -1).T)
+    Qf = (Q.T / dx).reshape(N, sx, sx)
+    HQ = O.apply_hamiltonian(Qf, dx, bt, B, V)
+    Hm = (Qf.reshape(N, -1).conj() @ HQ.reshape(N, -1).T) * dx * dx
+    El = np.linalg.eigvalsh(0.5 * (Hm + Hm.conj().T))
+    assert np.abs(E - El).max() < 1e-11 * np.abs(El).max()
+    Ho = O.apply_hamiltonian(out, dx, bt, B, V)
+    r = np.sqrt((np.abs(Ho - E[:, None, None] * out) ** 2).sum(axis=(1, 2))) * dx
+    assert np.abs(sd - r).max() < 1e-9 * np.abs(E).max()
+    G = (out.reshape(N, -1).conj() @ out.reshape(N, -1).T) * dx * dx
+    assert np.abs(G - np.eye(N)).max() < 1e-12
+
+
+@pytest.mark.parametrize("name", ["c1_ho64", "c2_small"])
+def test_mixed_itp_matches_fp64(ctx, name):
+    """The mixed path (FP32 ITP, FP64 Rayleigh-Ritz pass) against the FP64 golden
+    run: every state it counts as converged has its energy within 1e-8."""
+    say(f"mixed ITP {name}")
+    cfg, V = W.config(name)
+    ref = np.load(os.path.join(GOLD, f"itp_{name}.npz"))
+    s = I.ITPSystem(ctx, dict(cfg, precision=I.FP32), V)
+    s.run()
+    assert not s.error_flag
+    E, sd, _ = s.sorted()
+    nw = cfg["n_wanted"]
+    conv = np.where(s.converged()[:nw])[0]
+    assert len(conv) >= nw // 2
+    Er = ref["E"][:nw]
+    err = np.abs(E[:nw][conv] - Er[conv]) / np.abs(Er[conv])
+    assert err.max() < 1e-8
+    if name == "c1_ho64":
+        # the Rayleigh-Ritz pass removes the splitting bias: the states its
+        # error bound accepts are within 1e-8 of the analytic levels (the
+        # discrete spectrum agrees with them to ~1e-12 at this resolution)
+        s.rayleigh_ritz()
+        E2, sd2, _ = s.sorted()
+        ok = s.converged()[:16]
+        assert ok[:6].all()
+        ref_levels = W.ho_levels(16)
+        assert (np.abs(E2[:16] - ref_levels)[ok] <= 1e-8 * ref_levels[ok]).all()
+
+
+def test_mixed_c2_matches_fp64(ctx):
+    """Full-size C2 with the mixed path against the FP64 path: the converged
+    states' energies agree to 1e-8, and most wanted states converge."""
+    say("C2 full solve, mixed vs FP64")
+    cfg, V = W.config("c2_dirichlet256")
+    s64 = I.ITPSystem(ctx, cfg, V)
+    s64.run()
+    E64 = s64.sorted()[0]
+    del s64
+    s = I.ITPSystem(ctx, dict(cfg, precision=I.FP32), V)
+    s.run()
+    assert not s.error_flag
+    E, sd, _ = s.sorted()
+    nw = cfg["n_wanted"]
+    conv = np.where(s.converged()[:nw])[0]
+    assert len(conv) >= nw // 2
+    err = np.abs(E[conv] - E64[conv]) / np.abs(E64[conv])
+    assert err.max() < 1e-8
