@@ -9,7 +9,9 @@ solves / wall time of the timed solves.
 
 Default workload: the 256x256 Dirichlet grid, N = 500, oscillator + 40
 Gaussian impurities, B = 1, 4th-order multi-product split, eps = 0.05, on the
-plain FP64 GPU path.
+plain FP64 GPU path (value). The same workload also runs on the mixed-precision
+path ("mixed" in the output line): its converged states count only if their
+energies agree with the FP64 path's to 1e-8 relative.
 
 Prints one JSON line. Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -27,6 +29,18 @@ sys.path.insert(0, ROOT)
 METRIC = "converged states/s @1e-8 rel eigenvalue err, mixed vs FP64 GPU path; %HBM"
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFLOPS = 78.6
+
+
+def measured_traffic(workload, path):
+    """HBM bytes per sweep launch from the committed PMC summary of this
+    workload (FETCH_SIZE and WRITE_SIZE runs, copy-calibrated), or None."""
+    f = os.path.join(ROOT, "profiles", f"sweep_traffic_{workload}_{path}.md")
+    if not os.path.exists(f):
+        return None
+    for line in open(f):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
 
 
 def log(msg):
@@ -62,6 +76,47 @@ def cpu_baseline(cfg, V, per_solve_iters, per_solve_converged, sample_steps):
                       f"{per_solve_iters} iterations and {per_solve_converged} converged states"}
 
 
+def mixed_leg(I, ctx, cfg, V, args):
+    """The mixed-precision path on the same workload: timed solves, and the
+    converged states whose energies agree with the FP64 path's (one FP64 solve
+    of the same seeded input, run here as the reference) to 1e-8 relative."""
+    ref = I.ITPSystem(ctx, cfg, V)
+    ref.run()
+    E64 = ref.sorted()[0]
+    conv64 = ref.converged()
+    del ref
+    mcfg = dict(cfg, precision=I.FP32)
+    for w in range(args.warmup):
+        s = I.ITPSystem(ctx, mcfg, V)
+        s.run()
+        del s
+    systems = [I.ITPSystem(ctx, mcfg, V) for _ in range(args.steps)]
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for s in systems:
+        s.run()
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    good, worst = 0, 0.0
+    for s in systems:
+        E = s.sorted()[0]
+        c = s.converged()
+        err = np.abs(E - E64) / np.abs(E64)
+        ok = c & (err <= 1e-8)
+        good += int(ok.sum())
+        if c.any():
+            worst = max(worst, float(err[c].max()))
+    s0 = systems[0]
+    out = {"value": round(good / wall, 3), "unit": "converged states/s", "ms_per_step": round(1e3 * wall / len(systems), 3),
+           "steps_per_solve": s0.total_steps, "fp32_steps_per_solve": s0.mixed_steps,
+           "converged_per_solve": s0.num_converged, "converged_within_1e-8_of_fp64": good // len(systems),
+           "fp64_converged": int(conv64.sum()), "max_rel_err_vs_fp64": worst,
+           "dtype": "c32 fields and sweeps, bf16x3 MFMA overlap (fp32 accumulate, fp64 sums), fp32 MFMA rotation, "
+                    "fp64 eigensolver; fp64 final steps"}
+    log(f"mixed leg: {out}")
+    return out
+
+
 def copy_probe(mib):
     """Three device-to-device copies of a known size: the per-dispatch FETCH_SIZE and WRITE_SIZE
     they read give the factor that converts those counters to bytes on this device."""
@@ -84,6 +139,7 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--cpu-sample-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--iterations", type=int, default=0,
                     help="profiling mode: run this many ITP iterations of one solve and print the phase times "
                          "instead of the benchmark line")
@@ -119,6 +175,10 @@ def main():
         ctx.synchronize()
         print(json.dumps({"iterations": args.iterations, "phase_ms": ctx.phase_times()}), flush=True)
         return
+
+    mixed = None
+    if args.precision == "fp64" and not args.no_mixed:
+        mixed = mixed_leg(I, ctx, cfg, V, args)
 
     for w in range(args.warmup):
         s = I.ITPSystem(ctx, cfg, V)
@@ -173,9 +233,12 @@ def main():
     }
     dom = max(groups, key=lambda k: groups[k]["ms"])
     g = groups[dom]
+    traffic = measured_traffic(args.workload, args.precision) if dom == "sweep_kernel" else None
     roofline = {"kernel": dom, "bound": g["bound"], "achieved": round(g["achieved"], 3), "peak": g["peak"],
-                "unit": g["unit"], "frac": round(g["achieved"] / g["peak"], 4), "traffic": None,
+                "unit": g["unit"], "frac": round(g["achieved"] / g["peak"], 4), "traffic": traffic,
                 "avg_launch_ms": round(g["ms"] / max(g["launches"], 1), 5)}
+    if dom == "sweep_kernel":
+        roofline["algorithmic_bytes_per_launch"] = round(g["per_launch_bytes"])
     others = {k: {"achieved": round(v["achieved"], 3), "unit": v["unit"], "frac": round(v["achieved"] / v["peak"], 4),
                   "ms": round(v["ms"], 3)} for k, v in groups.items() if k != dom}
 
@@ -209,6 +272,9 @@ def main():
         "roofline": roofline,
         "roofline_other": others,
     }
+    if mixed is not None:
+        mixed["speedup_vs_fp64"] = round(mixed["value"] / value, 3) if value else None
+        line["mixed"] = mixed
     if not args.no_cpu_baseline:
         log("cpu baseline sample ...")
         line["cpu_baseline"] = cpu_baseline(W.config(args.workload)[0], V, iters[0], conv[0], args.cpu_sample_steps)
