@@ -63,11 +63,11 @@ struct GpuRegExec {
 
 // At least two waves per SIMD: the line exchanges are latency-bound, and the
 // LDS of two workgroups per CU allows it.
-template <class R, int N, int ORIENT>
+template <class R, int N, int ORIENT, bool XS>
 __global__ __launch_bounds__(256, 2) void reg_sweep_kernel(SweepArgs<R> a) {
   extern __shared__ __align__(16) char lds_raw[];
   GpuRegExec<RegState<R, RegShape<N>::Q>> ex;
-  reg_sweep_program<R, N, ORIENT>(ex, a, reinterpret_cast<cplx<R>*>(lds_raw), (int)blockIdx.y, (int)blockIdx.x);
+  reg_sweep_program<R, N, ORIENT, XS>(ex, a, reinterpret_cast<cplx<R>*>(lds_raw), (int)blockIdx.y, (int)blockIdx.x);
 }
 
 template <class R>
@@ -78,15 +78,17 @@ hipError_t launch_reg_sweep(const SweepArgs<R>& a, int nblk, int nfields, hipStr
   const int lines = a.orient == 0 ? a.sy : a.sx;
   if ((a.orient == 0 ? a.sx : a.sy) != n || nblk * a.L != lines) return hipErrorInvalidValue;
   hipError_t err = hipSuccess;
-  const bool ok = reg_dispatch(n, a.orient, [&](auto nc, auto oc) {
+  const bool xs = ops_have_xsplit(a.ops, a.nops);
+  const bool ok = reg_dispatch(n, a.orient, xs, [&](auto nc, auto oc, auto xc) {
     constexpr int N = decltype(nc)::value, O = decltype(oc)::value;
+    constexpr bool X = decltype(xc)::value;
     static bool attr_set = false;
     if (!attr_set) {
-      (void)hipFuncSetAttribute((const void*)reg_sweep_kernel<R, N, O>,
+      (void)hipFuncSetAttribute((const void*)reg_sweep_kernel<R, N, O, X>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
       attr_set = true;
     }
-    hipLaunchKernelGGL((reg_sweep_kernel<R, N, O>), dim3(nblk, nfields), dim3(kRegThreads), lds, st, a);
+    hipLaunchKernelGGL((reg_sweep_kernel<R, N, O, X>), dim3(nblk, nfields), dim3(kRegThreads), lds, st, a);
     err = hipGetLastError();
   });
   return ok ? err : hipErrorInvalidValue;

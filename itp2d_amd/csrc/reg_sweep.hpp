@@ -79,6 +79,14 @@ ITP_HD int opaque(int x) {
   return x;
 }
 
+// Scheduling fence: keeps the compiler from hoisting the loads of later
+// elements above it, which bounds the registers in flight in a gather loop.
+ITP_HD void sched_fence() {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __builtin_amdgcn_sched_barrier(0);
+#endif
+}
+
 template <class R, int Q>
 struct RegState {
   cplx<R> v[Q];
@@ -339,8 +347,10 @@ ITP_HD void reg_exchange(Exec& ex, cplx<R>* lds, F&& f) {
   });
 }
 
-// The whole sweep for line block blk of field `field`.
-template <class R, int N, int ORIENT, class Exec>
+// The whole sweep for line block blk of field `field`. XS: the program may
+// contain the magnetic Dirichlet split (rows only); without it the code path
+// and its second register set are compiled out.
+template <class R, int N, int ORIENT, bool XS, class Exec>
 ITP_HD void reg_sweep_program(Exec& ex, const SweepArgs<R>& a, cplx<R>* lds, int field, int blk) {
   using G = RegGeom<N, ORIENT>;
   constexpr int P = G::P, Q = G::Q, n = N;
@@ -484,6 +494,7 @@ ITP_HD void reg_sweep_program(Exec& ex, const SweepArgs<R>& a, cplx<R>* lds, int
       reg_fft<G, false, 0>(ex, lds, a.tw);
       pd.kind = dst ? PK_PERM_DST : PK_PERM_DCT;
     } else if (code == LOP_XSPLIT) {
+      if constexpr (XS) {
       // rows: with b = P, v_x = X[y][x] b_x and w_x = i X2[y][x-1] b_{x-1} (w_0 = 0),
       // the result RODFT01(v) + REDFT01(w) (expkinetic.cpp:176-184,
       // kinetic.cpp:128-134). Both pre-steps need only b_{n-1-k} and b_{k-1}:
@@ -505,6 +516,7 @@ ITP_HD void reg_sweep_program(Exec& ex, const SweepArgs<R>& a, cplx<R>* lds, int
           const long long row = (long long)(line0 + l) * sx;
 #pragma unroll
           for (int q = 0; q < Q; q++) {
+            if (q % 4 == 0) sched_fence();
             const int k = t + P * q;
             const int pa = n - 1 - k;
             const int pb = k == 0 ? pa : k - 1;
@@ -533,6 +545,7 @@ ITP_HD void reg_sweep_program(Exec& ex, const SweepArgs<R>& a, cplx<R>* lds, int
         }
       });
       pd.kind = PK_PERM_DCT;
+      }
     }
     pd.el = -1;
   }
@@ -596,13 +609,25 @@ ITP_HD void reg_sweep_program(Exec& ex, const SweepArgs<R>& a, cplx<R>* lds, int
   }
 }
 
-// Calls f(integral_constant<N>, integral_constant<ORIENT>) for a supported
-// line length; false otherwise.
+// Does the op list use the magnetic Dirichlet split?
+ITP_HD bool ops_have_xsplit(const LineOp* ops, int nops) {
+  for (int i = 0; i < nops; i++)
+    if (ops[i].code == LOP_XSPLIT) return true;
+  return false;
+}
+
+// Calls f(integral_constant<N>, integral_constant<ORIENT>, integral_constant<XS>)
+// for a supported line length (XS only with rows); false otherwise.
 template <class F>
-inline bool reg_dispatch(int n, int orient, F&& f) {
+inline bool reg_dispatch(int n, int orient, bool xs, F&& f) {
   auto go = [&](auto nc) {
-    if (orient == 0) f(nc, std::integral_constant<int, 0>{});
-    else f(nc, std::integral_constant<int, 1>{});
+    if (orient == 0) {
+      if (xs) f(nc, std::integral_constant<int, 0>{}, std::integral_constant<bool, true>{});
+      else f(nc, std::integral_constant<int, 0>{}, std::integral_constant<bool, false>{});
+    } else {
+      if (xs) return false;
+      f(nc, std::integral_constant<int, 1>{}, std::integral_constant<bool, false>{});
+    }
     return true;
   };
   switch (n) {

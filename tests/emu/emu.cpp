@@ -38,8 +38,9 @@ struct CpuBackend {
       const int lines = a.orient == 0 ? a.sy : a.sx;
       if (a.L != reg_lines_per_wg(n) || nblk * a.L != lines || (a.orient == 0 ? a.sx : a.sy) != n)
         throw std::runtime_error("register sweep shape");
-      const bool ok = reg_dispatch(n, a.orient, [&](auto nc, auto oc) {
+      const bool ok = reg_dispatch(n, a.orient, ops_have_xsplit(a.ops, a.nops), [&](auto nc, auto oc, auto xc) {
         constexpr int N = decltype(nc)::value, O = decltype(oc)::value;
+        constexpr bool X = decltype(xc)::value;
         const size_t bytes = reg_lds_bytes<R>(N);
 #pragma omp parallel
         {
@@ -48,7 +49,7 @@ struct CpuBackend {
           for (int f = 0; f < nfields; f++)
             for (int b = 0; b < nblk; b++) {
               CpuRegExec<RegState<R, RegShape<N>::Q>> ex;
-              reg_sweep_program<R, N, O>(ex, a, lds.data(), f, b);
+              reg_sweep_program<R, N, O, X>(ex, a, lds.data(), f, b);
             }
         }
       });
