@@ -33,8 +33,9 @@ struct BlockJacobiArgs {
   cplx<double>* Q;       // [nb/2][JS][JS] rotations of the round
   cplx<double>* D;       // [nb/2][JS][JS] diagonalised sub-blocks of the round
   int* ident;            // [nb/2] 1 if Q_k is the identity
-  double* slots;         // [2 (max_sweeps + 1)]: per whole-matrix check, the largest element ratio,
-                         // then (second half) the largest coupling-to-gap ratio (atomic max)
+  double* slots;         // [2 (max_sweeps + 1) + 1]: per whole-matrix check, the largest element ratio,
+                         // then (second half) the largest coupling-to-gap ratio (atomic max);
+                         // the last 8 bytes hold the grid barrier's counter and error flag
   double* info;          // [0] sweeps, [1] 1 converged / 2 handed off / 0 neither
   int np;
   int max_sweeps;

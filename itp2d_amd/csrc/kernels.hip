@@ -274,8 +274,33 @@ template hipError_t launch_gram<float>(const cplx<float>*, const cplx<float>*, l
 // sweep's own slot, which every workgroup reads after the sweep's last
 // barrier, so all workgroups take the same decision.
 // ---------------------------------------------------------------------------
+// Grid-wide barrier of a cooperative launch: an arrival counter (zeroed
+// before the launch) with agent-scope release/acquire fences. Barrier number
+// k of every workgroup waits for k * gridDim.x arrivals. A spin bound turns a
+// barrier that never completes into an error flag instead of a hang.
+__device__ __forceinline__ void grid_barrier(unsigned* cnt, unsigned& epoch, unsigned* err) {
+  __syncthreads();
+  const unsigned target = ++epoch * gridDim.x;
+  if (threadIdx.x == 0) {
+    __threadfence();
+    atomicAdd(cnt, 1u);
+    unsigned spins = 0;
+    while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > (1u << 24)) {
+        atomicOr(err, 1u);
+        break;
+      }
+    }
+    __threadfence();
+  }
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(256) void block_jacobi_kernel(BlockJacobiArgs a) {
-  cg::grid_group grid = cg::this_grid();
+  unsigned* bar = reinterpret_cast<unsigned*>(a.slots + 2 * (a.max_sweeps + 1));
+  unsigned* bar_err = bar + 1;
+  unsigned epoch = 0;
   extern __shared__ __align__(16) char bjl[];
   const int nt = (int)blockDim.x;
   BJLds s = bj_carve(bjl, nt);
@@ -297,7 +322,7 @@ __global__ __launch_bounds__(256) void block_jacobi_kernel(BlockJacobiArgs a) {
       atomicMax(slot, dbits(s.red[nt]));
       atomicMax(gslot, dbits(s.red[nt + 1]));
     }
-    grid.sync();
+    grid_barrier(bar, epoch, bar_err);
     const double worst = bitsd(__hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     const double rho = bitsd(__hip_atomic_load(gslot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     if (worst < 1.0) { conv = true; break; }
@@ -307,16 +332,16 @@ __global__ __launch_bounds__(256) void block_jacobi_kernel(BlockJacobiArgs a) {
     bj_reset_positions(ex, s, nb);
     for (int r = 0; r < nb - 1; r++) {
       for (int k = blockIdx.x; k < P; k += gridDim.x) bj_inner(ex, a, k, s.pos2blk[2 * k], s.pos2blk[2 * k + 1], s);
-      grid.sync();
+      grid_barrier(bar, epoch, bar_err);
       for (int t = blockIdx.x; t < items; t += gridDim.x) bj_apply(ex, a, t, s);
-      grid.sync();
+      grid_barrier(bar, epoch, bar_err);
       bj_next_positions(ex, s, nb);
     }
     sweeps++;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     a.info[0] = sweeps;
-    a.info[1] = conv ? 1.0 : (handed ? 2.0 : 0.0);
+    a.info[1] = (*bar_err != 0) ? 0.0 : (conv ? 1.0 : (handed ? 2.0 : 0.0));
     a.info[2] = 0;
   }
 }
@@ -332,8 +357,9 @@ hipError_t launch_block_jacobi(cplx<double>* A, cplx<double>* V, cplx<double>* Q
   BlockJacobiArgs a;
   a.A = A; a.V = V; a.Q = Q; a.D = D; a.ident = ident; a.slots = slots; a.info = info;
   a.np = np; a.max_sweeps = max_sweeps; a.max_inner = max_inner; a.tol = tol; a.handoff = handoff;
+  // the two maxima per check, then the barrier counter and its error flag
   hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(256), 0, st, reinterpret_cast<unsigned long long*>(slots),
-                     (long long)(2 * (max_sweeps + 1)));
+                     (long long)(2 * (max_sweeps + 1) + 1));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const size_t lds = bj_lds_bytes(256);
