@@ -167,13 +167,23 @@ def main():
         f"{args.precision}; CUs={ctx.num_cus}")
 
     if args.iterations:
+        t0 = time.perf_counter()
         s = I.ITPSystem(ctx, cfg, V)
+        ctx.synchronize()
+        log(f"setup {time.perf_counter() - t0:.2f}s")
         ctx.enable_timing(True)
         ctx.reset_phase_times()
-        for _ in range(args.iterations):
+        walls = []
+        for k in range(args.iterations):
+            t0 = time.perf_counter()
             s.step()
-        ctx.synchronize()
-        print(json.dumps({"iterations": args.iterations, "phase_ms": ctx.phase_times()}), flush=True)
+            ctx.synchronize()
+            walls.append(time.perf_counter() - t0)
+            log(f"iteration {k}: {walls[-1] * 1e3:.1f} ms")
+        print(json.dumps({"workload": args.workload, "path": args.precision, "iterations": args.iterations,
+                          "iteration_ms": [round(w * 1e3, 2) for w in walls],
+                          "phase_ms": ctx.phase_times(), "eigensolver": ctx.eig_stats(),
+                          "refinement": ctx.oa_stats()}), flush=True)
         return
 
     mixed = None

@@ -142,6 +142,14 @@ int itp_energies_complex(itp_hamiltonian* h, itp_stateset* s, double* mean, doub
  * deviation of operators.cpp:40-50 for an eigenpair estimate). Slot n of the
  * set then holds the Ritz vector of E[n]. E, sigma nullable. */
 int itp_rayleigh_ritz(itp_hamiltonian* h, itp_stateset* s, double* E, double* sigma);
+/* Fills the set with seeded uniform noise, the initial states of
+ * StateSet::init_to_gaussian_noise (stateset.cpp:124-131) drawn as
+ * itp_init_states draws them, generated on the device: element values are
+ * identical to
+ * itp_init_states'; the per-state normalisation sums in another order, so the
+ * scale may differ in the last bits. Advances *rng_state past the 2 N M draws
+ * used, as itp_init_states does. */
+int itp_stateset_init_noise(itp_stateset* s, uint64_t* rng_state, double dx);
 /* Copies the fields of src into dst (same grid and size), converting the
  * precision if the two differ. */
 int itp_stateset_convert(itp_stateset* dst, const itp_stateset* src);
@@ -185,7 +193,8 @@ int itp_ctx_eig_stats(itp_ctx* ctx, long long* calls, long long* sweeps);
 /* Upper bound on the element sweeps spent on each block sub-problem per round
  * of the block Jacobi eigensolver (default 1). */
 int itp_ctx_set_eig_inner(itp_ctx* ctx, int inner);
-/* Workgroups of the cooperative eigensolver launch (default 64). */
+/* Workgroups of the cooperative eigensolver launch (default 0: one per 16
+ * super-tile updates of a round, at least 64, at most what is co-resident). */
 int itp_ctx_set_eig_grid(itp_ctx* ctx, int workgroups);
 /* Eigensolver of the overlap matrix (replaces zheev, eigensolver.hpp:80-84):
  * 0 (default) block Jacobi sweeps until every unconverged off-diagonal element

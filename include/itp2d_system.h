@@ -65,7 +65,8 @@ typedef struct itps_params {
 typedef struct itps_system itps_system;
 
 /* V may be NULL (null potential); init_states ([N][sy][sx] complex double) may
- * be NULL, in which case seeded noise is used (itp_init_states). */
+ * be NULL, in which case seeded noise is used (itp_stateset_init_noise, the
+ * values of itp_init_states generated on the device). */
 int itps_create(itp_ctx* ctx, const itps_params* p, const double* V, const double* init_states, itps_system** out);
 void itps_destroy(itps_system* s);
 /* ITPSystem::step (itpsystem.cpp:407-470) */

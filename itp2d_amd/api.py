@@ -114,6 +114,7 @@ def lib():
             "itp_energies": [_vp, _vp, _dp, _dp],
             "itp_rayleigh_ritz": [_vp, _vp, _dp, _dp],
             "itp_stateset_convert": [_vp, _vp],
+            "itp_stateset_init_noise": [_vp, C.POINTER(C.c_uint64), C.c_double],
             "itp_stateset_overlap_eigenvalues": [_vp, _dp],
             "itp_energies_complex": [_vp, _vp, _dp, _dp],
             "itp_apply_expkinetic_host": [_vp, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _sz],
@@ -332,6 +333,12 @@ class StateSet:
     def assign(self, other):
         """Copy the fields of another set of the same grid and size (converting precision)."""
         self.grid.ctx.check(lib().itp_stateset_convert(self.h, other.h), "convert")
+
+    def init_noise(self, seed, dx):
+        """Fill with the seeded noise of init_states, generated on the device; returns the advanced seed."""
+        st = C.c_uint64(seed)
+        self.grid.ctx.check(lib().itp_stateset_init_noise(self.h, C.byref(st), C.c_double(dx)), "init_noise")
+        return st.value
 
     def overlap_eigenvalues(self):
         lam = np.zeros(self.N)

@@ -36,7 +36,7 @@ struct itp_ctx {
   long long cnt[ITP_NUM_PHASES] = {};
   long long eig_calls = 0, eig_sweeps = 0;
   int eig_inner = 1;   // element sweeps per block sub-problem and round (upper bound)
-  int eig_grid = 64;   // workgroups of the eigensolver (its grid barrier cost grows with them)
+  int eig_grid = 0;    // workgroups of the eigensolver (0: sized by the matrix; the grid barrier cost grows with them)
   int eig_mode = 0;    // 0: Jacobi handing off to the refinement iteration, 1: Jacobi only
   double eig_handoff = 0.5;   // hand-off threshold on |a_ij| / |a_ii - a_jj|
   long long oa_done = 0, oa_fallback = 0;
@@ -78,7 +78,7 @@ struct itp_stateset {
   cplx<double>* bjD = nullptr;
   double* bjslots = nullptr;
   int* bjident = nullptr;
-  cplx<double>* oa_work = nullptr;  // refinement iteration: Y, S, G, E (np x np each)
+  cplx<double>* oa_work = nullptr;  // refinement iteration: Y, S, G, E, second iterate (np x np each)
   double* oa_lam = nullptr;
   double* oa_slots = nullptr;
   int* oa_state = nullptr;
@@ -375,10 +375,9 @@ int eigen_stage(itp_stateset* s, size_t* bad_n, double* bad_lambda, bool scale, 
   if (bad) return set_err(c, ITP_E_EIGENSOLVER, "overlap matrix has non-finite entries");
   if (info[1] == 2.0 && state[0] == 1) c->oa_done++;
   if (info[1] == 2.0 && state[0] != 1) {
-    // the refinement did not converge: plain Jacobi from the saved matrix
+    // the refinement did not converge: the Jacobi sweeps continue from where
+    // they handed off (A[0] and V[0] are untouched by the refinement)
     c->oa_fallback++;
-    HIPCK(c, hipMemcpyAsync(s->A[0], s->A[1], np2, hipMemcpyDeviceToDevice, c->stream));
-    HIPCK(c, hipMemcpyAsync(s->V[0], s->Vinit, np2, hipMemcpyDeviceToDevice, c->stream));
     HIPCK(c, launch_block_jacobi(s->A[0], s->V[0], s->bjQ, s->bjD, s->bjident, s->bjslots, s->np, kMaxSweeps,
                                  c->eig_inner, 1e-14, 0.0, s->info, c->eig_grid, c->stream));
     HIPCK(c, launch_eig_finish(s->A[0], s->V[0], s->np, N, s->lam, s->perm, s->C, s->nc, scale ? 1 : 0, c->stream));
@@ -652,7 +651,7 @@ int itp_ctx_oa_stats(itp_ctx* c, long long* refined, long long* fallbacks) {
 }
 
 int itp_ctx_set_eig_grid(itp_ctx* c, int workgroups) {
-  if (!c || workgroups < 1 || workgroups > 4096) return ITP_E_INVALID;
+  if (!c || workgroups < 0 || workgroups > 4096) return ITP_E_INVALID;
   c->eig_grid = workgroups;
   return ITP_OK;
 }
@@ -829,7 +828,7 @@ int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out
       if ((rc = dalloc(c, (void**)&s->bjD, qbytes))) return rc;
       if ((rc = dalloc(c, (void**)&s->bjslots, 3 * (kMaxSweeps + 1) * sizeof(double)))) return rc;
       if ((rc = dalloc(c, (void**)&s->bjident, (size_t)(s->np / JS) * sizeof(int)))) return rc;
-      if ((rc = dalloc(c, (void**)&s->oa_work, 4 * np2))) return rc;
+      if ((rc = dalloc(c, (void**)&s->oa_work, 5 * np2))) return rc;
       if ((rc = dalloc(c, (void**)&s->oa_lam, (size_t)s->np * sizeof(double)))) return rc;
       if ((rc = dalloc(c, (void**)&s->oa_slots, kOAMaxIter * sizeof(double)))) return rc;
       if ((rc = dalloc(c, (void**)&s->oa_state, 2 * sizeof(int)))) return rc;
@@ -991,6 +990,26 @@ int itp_stateset_convert(itp_stateset* dst, const itp_stateset* src) {
                               c->stream));
     else
       HIPCK(c, launch_convert(src->buf[src->cur], dst->buf[dst->cur], n, dst->prec == ITP_FP32 ? 1 : 0, c->stream));
+    return ITP_OK;
+  });
+}
+
+int itp_stateset_init_noise(itp_stateset* s, uint64_t* rng_state, double dx) {
+  if (!s || !rng_state || !(dx > 0)) return ITP_E_INVALID;
+  itp_ctx* c = s->grid->ctx;
+  return guarded(c, [&]() -> int {
+    const long long M = s->grid->g.M;
+    const size_t need = (size_t)s->N * noise_blocks(M) * sizeof(double);
+    const size_t have = 2 * s->N * (size_t)std::max(s->grid->g.sx, s->grid->g.sy) * 16;
+    if (need > have) return set_err(c, ITP_E_UNSUPPORTED, "noise partials exceed the energy-partials buffer");
+    const uint64_t s0 = *rng_state;
+    if (s->prec == ITP_FP64)
+      HIPCK(c, launch_noise<double>(s0, M, (int)s->N, dx, (double*)s->epart, (cplx<double>*)s->buf[s->cur],
+                                    c->stream));
+    else
+      HIPCK(c, launch_noise<float>(s0, M, (int)s->N, dx, (double*)s->epart, (cplx<float>*)s->buf[s->cur],
+                                   c->stream));
+    *rng_state = s0 + 2ull * (unsigned long long)s->N * (unsigned long long)M * 0x9E3779B97F4A7C15ULL;
     return ITP_OK;
   });
 }

@@ -11,6 +11,12 @@ hipError_t launch_energy_reduce(const cplx<double>* p1, const cplx<double>* p2, 
                                 double* E, double* sigma, cplx<double>* mean, cplx<double>* msq, hipStream_t st);
 // n complex elements: cplx<double> -> cplx<float> (to_f32) or back
 hipError_t launch_convert(const void* in, void* out, long long n, int to_f32, hipStream_t st);
+// N seeded noise states of M elements, each of dx-norm 1, from the splitmix64
+// stream at s0 (draws 2 (n M + i) and 2 (n M + i) + 1); part holds
+// N * noise_blocks(M) doubles
+int noise_blocks(long long M);
+template <class R>
+hipError_t launch_noise(uint64_t s0, long long M, int N, double dx, double* part, cplx<R>* out, hipStream_t st);
 // S = dx^2 Psi^H Phi from the upper 64 x 64 tile pairs, mirrored (Hermitian
 // when Phi = Psi or Phi = H Psi with H Hermitian); with full, from all tile
 // pairs, and S is the Hermitian part (G + G^H) / 2.
@@ -29,9 +35,13 @@ hipError_t launch_rotate_f32(const cplx<float>* in, cplx<float>* out, const cplx
 template <class R>
 hipError_t launch_residual(const cplx<R>* W, const cplx<R>* Psi, const double* E, long long M, int N, double dx,
                            double* sigma, hipStream_t st);
+// grid_wgs <= 0 sizes the cooperative grid by the matrix
 hipError_t launch_block_jacobi(cplx<double>* A, cplx<double>* V, cplx<double>* Q, cplx<double>* D, int* ident,
                                double* slots, int np, int max_sweeps, int max_inner, double tol, double handoff,
                                double* info, int grid_wgs, hipStream_t st);
+// Refinement of the eigenvector estimate X0 of A0: X0 is overwritten only on
+// convergence (iterates go to X1 and to work's fifth np x np slot); work holds
+// 5 np^2 complex values
 hipError_t launch_oa(const cplx<double>* A0, cplx<double>* X0, cplx<double>* X1, cplx<double>* work, double* lam,
                      double* slots, int* state, const double* info, int np, int N, int max_iter, cplx<double>* Aout,
                      hipStream_t st);

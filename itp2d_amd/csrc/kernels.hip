@@ -150,6 +150,71 @@ hipError_t launch_convert(const void* in, void* out, long long n, int to_f32, hi
 }
 
 // ---------------------------------------------------------------------------
+// Seeded uniform-noise initial states on the device.
+//
+// The host generator draws x_k = splitmix64 output number k + 1 of the stream
+// starting at s0, i.e. mix(s0 + (k + 1) * gamma); each draw is independent of
+// the others, so state n, element i takes draws 2 (n M + i) and 2 (n M + i) + 1
+// directly. Values are 2 u - 1 with u = (x >> 11) 2^-53 (exact in FP64), each
+// state scaled to dx-norm 1 in FP64 and then rounded to R. Pass 1 writes
+// per-block sums of squares (fixed order), pass 2 sums them per state in
+// block order and writes the scaled values.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double noise_value(uint64_t s0, unsigned long long k) {
+  uint64_t z = s0 + (k + 1) * 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  z ^= z >> 31;
+  return 2.0 * (double(z >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
+}
+
+__global__ __launch_bounds__(256) void noise_norm_kernel(uint64_t s0, long long M, double* part) {
+  const int n = blockIdx.y, nb = gridDim.x;
+  const unsigned long long base = 2ull * (unsigned long long)n * (unsigned long long)M;
+  double acc = 0.0;
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < M; i += 256ll * nb) {
+    const double re = noise_value(s0, base + 2ull * i), im = noise_value(s0, base + 2ull * i + 1);
+    acc += re * re + im * im;
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[(long long)n * nb + blockIdx.x] = red[0];
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void noise_write_kernel(uint64_t s0, long long M, double dx, const double* part,
+                                                          cplx<R>* out) {
+  const int n = blockIdx.y, nb = gridDim.x;
+  double acc = 0.0;
+  for (int b = 0; b < nb; b++) acc += part[(long long)n * nb + b];
+  const double f = 1.0 / (sqrt(acc) * dx);
+  const unsigned long long base = 2ull * (unsigned long long)n * (unsigned long long)M;
+  cplx<R>* o = out + (long long)n * M;
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < M; i += 256ll * nb) {
+    const double re = noise_value(s0, base + 2ull * i), im = noise_value(s0, base + 2ull * i + 1);
+    o[i] = mk<R>(R(re * f), R(im * f));
+  }
+}
+
+int noise_blocks(long long M) { return (int)std::max<long long>(1, std::min<long long>(64, M / 4096)); }
+
+template <class R>
+hipError_t launch_noise(uint64_t s0, long long M, int N, double dx, double* part, cplx<R>* out, hipStream_t st) {
+  if (N <= 0 || M <= 0) return hipSuccess;
+  const dim3 grid(noise_blocks(M), N);
+  hipLaunchKernelGGL(noise_norm_kernel, grid, dim3(256), 0, st, s0, M, part);
+  hipLaunchKernelGGL(noise_write_kernel<R>, grid, dim3(256), 0, st, s0, M, dx, (const double*)part, out);
+  return hipGetLastError();
+}
+template hipError_t launch_noise<double>(uint64_t, long long, int, double, double*, cplx<double>*, hipStream_t);
+template hipError_t launch_noise<float>(uint64_t, long long, int, double, double*, cplx<float>*, hipStream_t);
+
+// ---------------------------------------------------------------------------
 // Overlap matrix on FP64 MFMA.
 //
 // v_mfma_f64_16x16x4f64: D(16x16) += A(16x4) B(4x16). Lane l holds
@@ -370,7 +435,11 @@ hipError_t launch_block_jacobi(cplx<double>* A, cplx<double>* V, cplx<double>* Q
   int dev = 0, ncu = 0;
   if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     return hipErrorInvalidValue;
-  int grid = std::min(grid_wgs, per_cu * ncu);
+  // grid_wgs <= 0: one workgroup per 16 super-tile updates of a round, at
+  // least 64 (below that the grid barrier is cheap and the sub-problems of a
+  // round are the critical path), at most what is co-resident
+  const int want = grid_wgs > 0 ? grid_wgs : std::max(64, bj_items(np) / 16);
+  int grid = std::min(want, per_cu * ncu);
   grid = std::max(1, std::min(grid, bj_items(np)));
   void* args[] = {&a};
   e = hipLaunchCooperativeKernel((const void*)block_jacobi_kernel, dim3(grid), dim3(256), args, (unsigned)lds, st);
@@ -489,7 +558,7 @@ __global__ void oa_decide_kernel(const unsigned long long* slot, int k, int* sta
   if (state[0] != 0) return;
   const double m = bitsd(*slot);
   if (!(m < 0.5)) { state[0] = 2; return; }
-  if (m < 1e-8) { state[0] = 1; state[1] = (k + 1) & 1; }
+  if (m < 1e-8) { state[0] = 1; state[1] = (k & 1) ? 2 : 1; }
 }
 
 // Starts the refinement if the Jacobi stage handed off (info[1] == 2).
@@ -498,14 +567,16 @@ __global__ void oa_init_kernel(const double* info, int* state) {
   state[1] = 0;
 }
 
-// Result into the places the eigenvalue sort reads: X -> V (if it ended in
-// the second buffer), lambda -> diagonal of A.
-__global__ void oa_finalize_kernel(const int* state, const cplx<double>* X1, cplx<double>* V, cplx<double>* A,
-                                   const double* lam, int np, int N) {
+// Result into the places the eigenvalue sort reads: X -> V (from the buffer
+// the last iterate went to), lambda -> diagonal of A. Without convergence V
+// and A keep the Jacobi stage's eigenvector estimate and rotated matrix.
+__global__ void oa_finalize_kernel(const int* state, const cplx<double>* X1, const cplx<double>* X2, cplx<double>* V,
+                                   cplx<double>* A, const double* lam, int np, int N) {
   if (state[0] != 1) return;
   const long long n2 = (long long)np * np;
+  const cplx<double>* X = state[1] == 1 ? X1 : X2;
   for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n2; e += (long long)gridDim.x * blockDim.x) {
-    if (state[1] == 1) V[e] = X1[e];
+    V[e] = X[e];
     const int i = (int)(e / np), j = (int)(e - (long long)i * np);
     if (i == j && i < N) A[e] = mk<double>(lam[i], 0.0);
   }
@@ -520,13 +591,16 @@ hipError_t launch_oa(const cplx<double>* A0, cplx<double>* X0, cplx<double>* X1,
   cplx<double>* S = work + n2;
   cplx<double>* G = work + 2 * n2;
   cplx<double>* E = work + 3 * n2;
+  cplx<double>* X2 = work + 4 * n2;
   hipLaunchKernelGGL(oa_init_kernel, dim3(1), dim3(1), 0, st, info, state);
   hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<unsigned long long*>(slots),
                      (long long)max_iter);
   const dim3 gg(np / 32, np / 32);
   for (int k = 0; k < max_iter; k++) {
-    cplx<double>* X = (k & 1) ? X1 : X0;
-    cplx<double>* Xn = (k & 1) ? X0 : X1;
+    // X0 is only read (iterates alternate between X1 and X2), so a failed
+    // refinement leaves the starting estimate in place
+    cplx<double>* X = k == 0 ? X0 : ((k & 1) ? X1 : X2);
+    cplx<double>* Xn = (k & 1) ? X2 : X1;
     unsigned long long* slot = reinterpret_cast<unsigned long long*>(slots) + k;
     hipLaunchKernelGGL(zgemm_kernel, gg, dim3(256), 0, st, A0, 0, X, Y, nullptr, np, state);
     hipLaunchKernelGGL(zgemm_kernel, gg, dim3(256), 0, st, X, 1, Y, S, nullptr, np, state);
@@ -536,7 +610,7 @@ hipError_t launch_oa(const cplx<double>* A0, cplx<double>* X0, cplx<double>* X1,
     hipLaunchKernelGGL(zgemm_kernel, gg, dim3(256), 0, st, X, 0, E, Xn, X, np, state);
     hipLaunchKernelGGL(oa_decide_kernel, dim3(1), dim3(1), 0, st, slot, k, state);
   }
-  hipLaunchKernelGGL(oa_finalize_kernel, dim3(256), dim3(256), 0, st, state, X1, X0, Aout, lam, np, N);
+  hipLaunchKernelGGL(oa_finalize_kernel, dim3(256), dim3(256), 0, st, state, X1, X2, X0, Aout, lam, np, N);
   return hipGetLastError();
 }
 

@@ -149,6 +149,9 @@ class StateSet {
     tr_.context().check(itp_stateset_overlap_eigenvalues(s_, lam.data()), "overlap_eigenvalues");
   }
   // Fields of other (same grid and size), converted to this set's precision.
+  void init_noise(uint64_t& rng, double dx) {
+    tr_.context().check(itp_stateset_init_noise(s_, &rng, dx), "init_noise");
+  }
   void assign(const StateSet& other) { tr_.context().check(itp_stateset_convert(s_, other.get()), "convert"); }
   itp_stateset* get() const { return s_; }
   Transformer& transformer() const { return tr_; }

@@ -35,15 +35,12 @@ class ITPSystem {
     T_.reset(new MultiProductSplit(p.halforder, *pot_, eps, p.B));
     mixed = p.precision == ITP_FP32;
     if (mixed) {
-      states64_.reset(new StateSet(*tr_, p.N, FP64));
       if (!(params.mixed_tol > 0)) params.mixed_tol = 2e-7;
       if (!(params.accuracy > 0)) params.accuracy = 1e-8;
     }
     rng = p.seed;
-    host_states.resize(p.N * p.sx * p.sy);
-    if (init) std::memcpy(host_states.data(), init, host_states.size() * sizeof(comp));
+    if (init) states_->upload(reinterpret_cast<const comp*>(init));
     else init_noise();
-    states_->upload(host_states.data());
     ts_conv.assign(p.N, 0);
     fin_conv.assign(p.N, 0);
   }
@@ -104,10 +101,7 @@ class ITPSystem {
   }
 
   void init_noise() {  // StateSet::init_to_gaussian_noise with uniform noise (stateset.cpp:124-131)
-    uint64_t st = rng;
-    itp_init_states(&st, params.sx, params.sy, params.lenx / double(params.sx), params.N,
-                    reinterpret_cast<double*>(host_states.data()));
-    rng = st;
+    states_->init_noise(rng, params.lenx / double(params.sx));
   }
 
   void set_ts(size_t n, bool v) {
@@ -145,7 +139,6 @@ class ITPSystem {
         just_recovered = true;
         change_time_step();
         init_noise();
-        states_->upload(host_states.data());
       } else {
         error_flag = true;
         finish();
@@ -203,8 +196,14 @@ class ITPSystem {
     // (they carry its O(eps^(2h)) splitting error), so parity needs that fixed
     // point; from FP32-converged states its energies are within second-order
     // terms, and the FP64 steps only confirm convergence.
+    // The FP64 set is allocated here and the FP32 one released after the
+    // copy, so the two precisions' sets are never both resident longer than
+    // the copy (a 1024^2 grid with 4000 states holds 62.5 GiB per FP32 set
+    // pair and 125 GiB per FP64 pair).
+    states64_.reset(new StateSet(*tr_, params.N, FP64));
     states64_->assign(*states_);
     std::swap(states_, states64_);
+    states64_.reset();
     mixed = false;
     energies.clear();
     stdevs.clear();
@@ -308,10 +307,9 @@ class ITPSystem {
   std::unique_ptr<Transformer> tr_;
   std::unique_ptr<Potential> pot_;
   std::unique_ptr<StateSet> states_;
-  std::unique_ptr<StateSet> states64_;  // mixed path: the other precision's set (FP64 for the final stage)
+  std::unique_ptr<StateSet> states64_;  // mixed path: the FP64 set, alive only while the FP32 states are copied
   std::unique_ptr<Hamiltonian> H_;
   std::unique_ptr<MultiProductSplit> T_;
-  std::vector<comp> host_states;
   uint64_t rng = 0;
   std::deque<double> eps_values;
   bool exhausting = false;

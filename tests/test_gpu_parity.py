@@ -86,6 +86,32 @@ def test_propagate_and_energies(ctx, bt, B, h):
     assert np.abs(sd - so).max() < 1e-9 * np.abs(so).max()
 
 
+@pytest.mark.parametrize("n,bt,B,h,V", [(512, I.PERIODIC, 0.0, 1, "ho"), (1024, I.DIRICHLET, 2.0, 3, "stadium"),
+                                         (1024, I.PERIODIC, 1.0, 1, "ho")])
+def test_large_grids(ctx, n, bt, B, h, V):
+    """The 512^2 and 1024^2 line lengths (the large-grid configurations):
+    transforms, one propagation and the energies, FP64 and complex FP32."""
+    L = 16.0 if V == "ho" else 5.0
+    dx = L / n
+    say(f"large grid {n}^2 bt={bt} B={B} h={h}")
+    g = I.Grid(ctx, n, n, dx, bt)
+    Vt = W.harmonic(n, n, dx) if V == "ho" else W.hard_stadium(n, n, dx)
+    pot = I.Potential(g, Vt)
+    f = rand_states(2, n, n, dx, n + h)
+    for name in I.TRANSFORMS:
+        assert rel(g.transform(f[:1], name), O.transform(f[0], name)[None]) < 1e-13, name
+    ref = O.propagate(f, dx, bt, B, Vt, h, 0.02)
+    Eo, so, _, _ = O.energies(ref, dx, bt, B, Vt)
+    for prec, tol in ((I.FP64, 1e-13), (I.FP32, 2e-6)):
+        S = I.StateSet(g, 2, prec)
+        S.upload(f)
+        I.MultiProductSplit(pot, h, 0.02, B).propagate(S)
+        out = S.download()
+        assert rel(out, ref) < tol, prec
+        E, sd = I.Hamiltonian(pot, B).energies(S)
+        assert np.abs(E - Eo).max() < (1e-12 if prec == I.FP64 else 1e-5) * np.abs(Eo).max(), prec
+
+
 def test_null_potential_split(ctx):
     sx = sy = 32
     dx = 0.2
@@ -313,6 +339,33 @@ def test_gram_bf16x3(ctx, N, sx):
     Gr = (A.conj() @ A.T) * dx * dx
     assert np.abs(G - Gr).max() < 1e-6 * np.abs(Gr).max()
     assert np.abs(G - G.conj().T).max() == 0
+
+
+@pytest.mark.parametrize("N,sx,sy", [(3, 16, 8), (2, 64, 64), (5, 256, 256), (2, 512, 96)])
+def test_device_noise_matches_oracle(ctx, N, sx, sy):
+    """Device-generated initial states: the same draws as the host generator,
+    normalised to rounding, in both precisions; the seed advances by 2 N M draws."""
+    say(f"device noise N={N} {sx}x{sy}")
+    dx = 0.25
+    seed = W.SEED_BASE + 11
+    ref = O.init_states(seed, N, sx, sy, dx)
+    g = I.Grid(ctx, sx, sy, dx, 0)
+    for prec in (I.FP64, I.FP32):
+        S = I.StateSet(g, N, prec)
+        nxt = S.init_noise(seed, dx)
+        assert nxt == (seed + 2 * N * sx * sy * 0x9E3779B97F4A7C15) % (1 << 64)
+        out = S.download()
+        if prec == I.FP64:
+            # the values are the same draws; only the norm's summation order differs
+            assert rel(out, ref) < 1e-14
+        else:
+            r32 = ref.astype(np.complex64).astype(np.complex128)
+            assert np.abs(out - r32).max() <= 2.0 ** -22 * np.abs(ref).max()
+    # the continued stream matches the host generator's continuation
+    S = I.StateSet(g, 1, I.FP64)
+    S.init_noise(nxt, dx)
+    both = O.init_states(seed, N + 1, sx, sy, dx)
+    assert rel(S.download()[0], both[N]) < 1e-14
 
 
 def test_orthonormalize_fp32(ctx):
