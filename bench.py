@@ -130,6 +130,65 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes": n * 4, "dispatches": 3}), flush=True)
 
 
+def batch_bench(I, W, args):
+    """The batched configuration: 64 independent instances run by the batch
+    API (worker threads with one stream each), full solves; value = converged
+    states of all instances / wall time of the batch run. The same instances
+    run by one worker give the no-overlap reference."""
+    insts = W.c4_instances(64)
+    prec = I.FP32 if args.precision == "fp32" else I.FP64
+    insts = [(dict(c, precision=prec), V) for c, V in insts]
+    log(f"batch: {len(insts)} instances, {sum(c['N'] for c, _ in insts)} states, "
+        f"workers {args.batch_workers}, {args.precision}")
+    warm = I.Batch(0, workers=args.batch_workers)
+    for c, V in insts[: args.batch_workers]:
+        warm.add(c, V)
+    warm.run()
+    log(f"warmup batch {warm.wall_time:.2f}s")
+    del warm
+
+    def one(workers):
+        b = I.Batch(0, workers=workers)
+        for c, V in insts:
+            b.add(c, V)
+        rc = b.run()
+        res = [b.result(i) for i in range(len(insts))]
+        return rc, b.wall_time, res
+
+    rc, wall, res = one(args.batch_workers)
+    conv = sum(r["converged"] for r in res)
+    iters = sum(r["steps"] for r in res)
+    bad = [i for i, r in enumerate(res) if r["status"] or r["error_flag"]]
+    log(f"batch run rc={rc}: {wall:.2f}s, {conv} converged, {iters} iterations, failed instances {bad}")
+    seq = None
+    if not args.no_batch_serial:
+        _, wall1, res1 = one(1)
+        seq = {"workers": 1, "wall_s": round(wall1, 3),
+               "value": round(sum(r["converged"] for r in res1) / wall1, 3)}
+        log(f"one worker: {wall1:.2f}s")
+    line = {
+        "metric": METRIC,
+        "value": round(conv / wall, 3),
+        "unit": "converged states/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": round(wall * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "c32-fields/f64-dense",
+        "data": "synthetic (64 seeded instances: grids 128^2/192^2/256^2, 50-400 states, B in [0,2), "
+                "oscillator plus impurity potentials, seeded noise initial states)",
+        "config": {"workload": "c4_batch64", "instances": len(insts), "states": sum(c["N"] for c, _ in insts),
+                   "path": args.precision, "workers": args.batch_workers, "converged": conv,
+                   "itp_iterations": iters, "itp_iterations_per_s": round(iters / wall, 3),
+                   "failed_instances": bad},
+        "serial": seq,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +205,10 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="profiling mode: run 3 plain device copies of this many MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against a known byte count) and exit")
+    ap.add_argument("--batch-workers", type=int, default=4,
+                    help="batched configuration (--workload c4_batch64): worker threads, one stream each")
+    ap.add_argument("--no-batch-serial", action="store_true",
+                    help="batched configuration: skip the one-worker comparison run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -158,6 +221,9 @@ def main():
     import itp2d_amd as I
     from itp2d_amd import workloads as W
 
+    if args.workload == "c4_batch64":
+        batch_bench(I, W, args)
+        return
     cfg, V = W.config(args.workload)
     cfg = dict(cfg, precision=I.FP32 if args.precision == "fp32" else I.FP64)
     N, M = cfg["N"], cfg["sx"] * cfg["sy"]

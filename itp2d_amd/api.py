@@ -140,6 +140,15 @@ def lib():
             "itps_converged": [_vp, C.POINTER(C.c_int)],
             "itps_rayleigh_ritz": [_vp],
             "itps_mixed_steps": [_vp],
+            "itp_batch_create": [C.c_int, C.c_int, C.POINTER(_vp)],
+            "itp_batch_destroy": [_vp],
+            "itp_batch_add": [_vp, C.POINTER(_Params), _dp, C.POINTER(_sz)],
+            "itp_batch_size": [_vp],
+            "itp_batch_run": [_vp],
+            "itp_batch_wall_time": [_vp],
+            "itp_batch_result": [_vp, _sz, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(_sz),
+                                 C.POINTER(C.c_int), _dp],
+            "itp_batch_sorted": [_vp, _sz, _dp, _dp, C.POINTER(C.c_int)],
         }
         for name, args in sig.items():
             getattr(L, name).argtypes = args
@@ -151,6 +160,8 @@ def lib():
         L.itps_total_time.restype = C.c_double
         L.itps_num_converged.restype = _sz
         L.itps_num_timestep_converged.restype = _sz
+        L.itp_batch_size.restype = _sz
+        L.itp_batch_wall_time.restype = C.c_double
         _lib = L
     return _lib
 
@@ -537,4 +548,62 @@ class ITPSystem:
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             _lib.itps_destroy(self.h)
+            self.h = None
+
+
+class Batch:
+    """Independent ITP problems run together (include/itp2d_batch.h): worker
+    host threads, each with its own context and stream, take the instances
+    largest first; every instance runs the loop of ITPSystem."""
+
+    def __init__(self, device=0, workers=4):
+        self.h = _vp()
+        rc = lib().itp_batch_create(int(device), int(workers), C.byref(self.h))
+        if rc:
+            raise ITPError(rc, "itp_batch_create")
+        self._cfgs = []
+
+    def add(self, cfg, V=None):
+        p, eps = _params(cfg)
+        Vc = None if V is None else np.ascontiguousarray(V, dtype=np.float64)
+        idx = _sz(0)
+        rc = lib().itp_batch_add(self.h, C.byref(p), _p(Vc), C.byref(idx))
+        if rc:
+            raise ITPError(rc, "itp_batch_add")
+        self._cfgs.append(cfg)
+        return idx.value
+
+    def __len__(self):
+        return lib().itp_batch_size(self.h)
+
+    def run(self):
+        """Runs every instance; returns the first failing status (0 if none)."""
+        return lib().itp_batch_run(self.h)
+
+    @property
+    def wall_time(self):
+        return lib().itp_batch_wall_time(self.h)
+
+    def result(self, i):
+        st, steps, err = C.c_int(0), C.c_int(0), C.c_int(0)
+        conv = _sz(0)
+        t = np.zeros(1)
+        rc = lib().itp_batch_result(self.h, int(i), C.byref(st), C.byref(steps), C.byref(conv), C.byref(err), _p(t))
+        if rc:
+            raise ITPError(rc, "itp_batch_result")
+        return dict(status=st.value, steps=steps.value, converged=conv.value, error_flag=bool(err.value),
+                    time=float(t[0]))
+
+    def sorted(self, i):
+        N = int(self._cfgs[i]["N"])
+        E, sd = np.zeros(N), np.zeros(N)
+        f = np.zeros(N, dtype=np.int32)
+        rc = lib().itp_batch_sorted(self.h, int(i), _p(E), _p(sd), f.ctypes.data_as(C.POINTER(C.c_int)))
+        if rc:
+            raise ITPError(rc, "itp_batch_sorted")
+        return E, sd, f.astype(bool)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.itp_batch_destroy(self.h)
             self.h = None

@@ -1,7 +1,8 @@
 """Builds the gfx950 library itp2d_amd/lib/libitp2d_amd.so with hipcc.
 
 Sources: csrc/kernels.hip, csrc/mixed.hip (kernels), csrc/capi.hip (C ABI of include/itp2d.h),
-host/itpsystem.cpp (ITP driver, C ABI of include/itp2d_system.h).
+host/itpsystem.cpp (ITP driver, C ABI of include/itp2d_system.h), host/batch.cpp (batches of
+instances on worker threads, C ABI of include/itp2d_batch.h).
 """
 import os
 import subprocess
@@ -17,6 +18,7 @@ SOURCES = [
     os.path.join(HERE, "csrc", "mixed.hip"),
     os.path.join(HERE, "csrc", "capi.hip"),
     os.path.join(HERE, "host", "itpsystem.cpp"),
+    os.path.join(HERE, "host", "batch.cpp"),
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -26,6 +28,7 @@ FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
+    "-pthread",
     # packed FP32 instructions stay disabled in all device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall",
@@ -44,7 +47,7 @@ def build(verbose=False):
         subprocess.run(cmd, check=True)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

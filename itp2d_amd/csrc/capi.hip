@@ -3,6 +3,7 @@
 // Objects own device memory; all kernels go to the context's one stream.
 // Table construction (host, FP64, the reference's formulas) and the sweep
 // sequences come from engine.hpp; kernels from kernels.hip.
+#include <mutex>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -344,6 +345,16 @@ int energies_impl(itp_hamiltonian* h, itp_stateset* s, double* E, double* sigma,
 
 // Overlap + eigendecomposition + coefficients; returns after the host has
 // checked the eigenvalues (stateset.cpp:145-177).
+// The block-Jacobi launch is cooperative: its workgroups spin on each other
+// at grid barriers. Contexts on other host threads (batches) could start a
+// second one while the first is only partly resident, and each would wait for
+// the other's workgroups to leave; one cooperative launch at a time per
+// process rules that out. Held from the launch until the stream has drained it.
+std::mutex& coop_mutex() {
+  static std::mutex m;
+  return m;
+}
+
 int eigen_stage(itp_stateset* s, size_t* bad_n, double* bad_lambda, bool scale, bool check_positive = true) {
   itp_ctx* c = s->grid->ctx;
   const int N = (int)s->N;
@@ -355,6 +366,7 @@ int eigen_stage(itp_stateset* s, size_t* bad_n, double* bad_lambda, bool scale, 
   // untouched copy A[1] starting from the Jacobi eigenvectors.
   HIPCK(c, hipMemcpyAsync(s->V[0], s->Vinit, np2, hipMemcpyDeviceToDevice, c->stream));
   if (hybrid) HIPCK(c, hipMemcpyAsync(s->A[1], s->A[0], np2, hipMemcpyDeviceToDevice, c->stream));
+  std::unique_lock<std::mutex> coop(coop_mutex());
   HIPCK(c, launch_block_jacobi(s->A[0], s->V[0], s->bjQ, s->bjD, s->bjident, s->bjslots, s->np, kMaxSweeps,
                                c->eig_inner, 1e-14, hybrid ? c->eig_handoff : 0.0, s->info, c->eig_grid, c->stream));
   if (hybrid)
@@ -386,6 +398,7 @@ int eigen_stage(itp_stateset* s, size_t* bad_n, double* bad_lambda, bool scale, 
     HIPCK(c, hipStreamSynchronize(c->stream));
     c->eig_sweeps += (long long)info[0];
   }
+  coop.unlock();
   pe.stop();
   if (info[1] != 1.0 && !(info[1] == 2.0 && state[0] == 1))
     return set_err(c, ITP_E_EIGENSOLVER, "Jacobi eigensolver did not converge in " + std::to_string((int)info[0]) + " sweeps");

@@ -82,12 +82,11 @@ hipError_t launch_reg_sweep(const SweepArgs<R>& a, int nblk, int nfields, hipStr
   const bool ok = reg_dispatch(n, a.orient, xs, [&](auto nc, auto oc, auto xc) {
     constexpr int N = decltype(nc)::value, O = decltype(oc)::value;
     constexpr bool X = decltype(xc)::value;
-    static bool attr_set = false;
-    if (!attr_set) {
-      (void)hipFuncSetAttribute((const void*)reg_sweep_kernel<R, N, O, X>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-      attr_set = true;
-    }
+    // once per instantiation (thread-safe static initialisation)
+    static const bool attr_set = (hipFuncSetAttribute((const void*)reg_sweep_kernel<R, N, O, X>,
+                                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024),
+                                  true);
+    (void)attr_set;
     hipLaunchKernelGGL((reg_sweep_kernel<R, N, O, X>), dim3(nblk, nfields), dim3(kRegThreads), lds, st, a);
     err = hipGetLastError();
   });
@@ -101,12 +100,10 @@ hipError_t launch_sweep(const SweepArgs<R>& a, int nblk, int nfields, hipStream_
   if (a.reg) return launch_reg_sweep<R>(a, nblk, nfields, st);
   const size_t lds = sweep_lds_bytes<R>(a.L, a.plan.n, 256);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
-  static bool attr_set = false;
-  if (!attr_set) {
-    // allow the full 160 KiB of LDS per workgroup
-    (void)hipFuncSetAttribute((const void*)sweep_kernel<R>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    attr_set = true;
-  }
+  // allow the full 160 KiB of LDS per workgroup (once, thread-safe static initialisation)
+  static const bool attr_set =
+      (hipFuncSetAttribute((const void*)sweep_kernel<R>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024), true);
+  (void)attr_set;
   if (nblk <= 0 || nfields <= 0) return hipSuccess;
   if (nfields > 65535) return hipErrorInvalidValue;
   hipLaunchKernelGGL(sweep_kernel<R>, dim3(nblk, nfields), dim3(256), lds, st, a);

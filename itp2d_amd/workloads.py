@@ -149,3 +149,27 @@ def config(name):
                    seed=SEED_BASE, timestep_test=("relchange", 1e-9), final_test=("relchange", 1e-9))
         return cfg, hard_stadium(sx, sy, L / sx)
     raise KeyError(name)
+
+
+def c4_instances(count=64, seed=SEED_BASE):
+    """The batched configuration: `count` independent instances, instance i
+    drawn from the stream seeded base + i: sx = sy uniform in {128, 192, 256},
+    N uniform in 50..400 (4/5 of them wanted), B uniform in [0, 2); V is the
+    oscillator plus 40 Gaussian impurities (seed base + 1000 + i). Pinned
+    where unspecified: L = 20, periodic, 4th-order multi-product split,
+    eps = 0.05, energy-change tests at 1e-9 relative.
+    Returns a list of (cfg, V)."""
+    out = []
+    for i in range(count):
+        rng = SplitMix64(seed + i)
+        n = (128, 192, 256)[rng.randint(0, 2)]
+        N = rng.randint(50, 400)
+        B = rng.uniform(0.0, 2.0)
+        L = 20.0
+        dx = L / n
+        imp, _ = gaussian_impurities(n, n, dx, 40, seed + 1000 + i, (-10.0, 10.0))
+        cfg = dict(sx=n, sy=n, lenx=L, boundary=0, N=N, n_wanted=(4 * N) // 5, halforder=2, B=B,
+                   eps_values=[0.05], eps_divisor=5.0, max_steps=2000, min_time_step=1e-9,
+                   seed=seed + i, timestep_test=("relchange", 1e-9), final_test=("relchange", 1e-9))
+        out.append((cfg, harmonic(n, n, dx) + imp))
+    return out

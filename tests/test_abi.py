@@ -15,7 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_functions():
     names = set()
-    for h in ("itp2d.h", "itp2d_system.h"):
+    for h in ("itp2d.h", "itp2d_system.h", "itp2d_batch.h"):
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         for m in re.finditer(r"\b(itps?_[a-z0-9_]+)\s*\(", text):
@@ -56,3 +56,40 @@ def test_workloads_deterministic():
     _, imps = W.gaussian_impurities(256, 256, 20 / 256, 40, W.SEED_BASE + 1, (-10, 10))
     for (x, y, A, w) in imps:
         assert -10 <= x <= 10 and -10 <= y <= 10 and -2 <= A <= 2 and 0.3 <= w <= 0.8
+
+
+def test_batch_host_side_validation():
+    """Batch API argument checks and an empty run need no GPU."""
+    lib = itp2d_amd.lib()
+    h = ctypes.c_void_p()
+    assert lib.itp_batch_create(0, 0, ctypes.byref(h)) != 0
+    assert lib.itp_batch_create(0, 65, ctypes.byref(h)) != 0
+    assert lib.itp_batch_create(0, 2, ctypes.byref(h)) == 0
+    try:
+        assert lib.itp_batch_size(h) == 0
+        assert lib.itp_batch_run(h) == 0
+        assert lib.itp_batch_wall_time(h) == 0.0
+        assert lib.itp_batch_result(h, 0, None, None, None, None, None) != 0
+        cfg, V = W.config("c1_ho64")
+        b = itp2d_amd.api._params(dict(cfg, N=0))[0]
+        assert lib.itp_batch_add(h, ctypes.byref(b), None, None) != 0
+        p, _eps = itp2d_amd.api._params(cfg)
+        idx = ctypes.c_size_t(7)
+        assert lib.itp_batch_add(h, ctypes.byref(p), V.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                 ctypes.byref(idx)) == 0
+        assert idx.value == 0 and lib.itp_batch_size(h) == 1
+        # not run yet: no sorted energies
+        E = np.zeros(cfg["N"])
+        assert lib.itp_batch_sorted(h, 0, E.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), None, None) != 0
+    finally:
+        lib.itp_batch_destroy(h)
+
+
+def test_c4_instances_pinned():
+    xs = W.c4_instances()
+    assert len(xs) == 64
+    for cfg, V in xs:
+        assert cfg["sx"] in (128, 192, 256) and 50 <= cfg["N"] <= 400 and 0 <= cfg["B"] < 2
+        assert V.shape == (cfg["sy"], cfg["sx"])
+    again = W.c4_instances(4)
+    assert [c for c, _ in again] == [c for c, _ in xs[:4]]
