@@ -207,6 +207,17 @@ int itp_ctx_set_eig_mode(itp_ctx* ctx, int mode);
 int itp_ctx_set_eig_handoff(itp_ctx* ctx, double ratio);
 /* Calls finished by the refinement iteration and calls that fell back to Jacobi. */
 int itp_ctx_oa_stats(itp_ctx* ctx, long long* refined, long long* fallbacks);
+/* Cluster start of mode 0 (tried before any Jacobi sweep): the pairs with
+ * |a_ij| > tau |a_ii - a_jj| are joined into clusters of at most 48 indices,
+ * each diagonalised exactly, and the refinement iteration starts from those
+ * rotations; with more than 8192 such pairs, a larger cluster or no
+ * convergence the block-Jacobi route runs as before. tau in [0, 0.5); 0 turns
+ * the cluster start off (default 0.1). No counterpart in the reference, which
+ * calls zheev (eigensolver.hpp:80-84): same eigenpairs, another route. */
+int itp_ctx_set_eig_cluster(itp_ctx* ctx, double tau);
+/* out[4]: calls finished by the cluster start, clusters solved, largest
+ * cluster, calls that fell back to the block-Jacobi route. */
+int itp_ctx_cluster_stats(itp_ctx* ctx, long long* out);
 
 #ifdef __cplusplus
 }

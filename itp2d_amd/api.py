@@ -87,6 +87,8 @@ def lib():
             "itp_ctx_set_eig_mode": [_vp, C.c_int],
             "itp_ctx_set_eig_handoff": [_vp, C.c_double],
             "itp_ctx_oa_stats": [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
+            "itp_ctx_set_eig_cluster": [_vp, C.c_double],
+            "itp_ctx_cluster_stats": [_vp, C.POINTER(C.c_longlong)],
             "itp_grid_create": [_vp, _sz, _sz, C.c_double, C.c_int, C.POINTER(_vp)],
             "itp_grid_destroy": [_vp],
             "itp_grid_kvalues": [_vp, _dp, _dp, _dp, _dp],
@@ -241,6 +243,16 @@ class Context:
         a, b = C.c_longlong(0), C.c_longlong(0)
         lib().itp_ctx_oa_stats(self.h, C.byref(a), C.byref(b))
         return a.value, b.value
+
+    def set_eig_cluster(self, tau):
+        """Cluster start threshold on |a_ij| / |a_ii - a_jj| (0: off)."""
+        self.check(lib().itp_ctx_set_eig_cluster(self.h, float(tau)), "set_eig_cluster")
+
+    def cluster_stats(self):
+        """(calls finished by the cluster start, clusters solved, largest cluster, fallbacks to Jacobi)"""
+        out = (C.c_longlong * 4)()
+        self.check(lib().itp_ctx_cluster_stats(self.h, out), "cluster_stats")
+        return tuple(out)
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

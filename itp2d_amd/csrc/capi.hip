@@ -16,6 +16,7 @@
 #include "../../include/itp2d.h"
 #include "engine.hpp"
 #include "block_jacobi.hpp"
+#include "cluster_eig.hpp"
 #include "gpu.hpp"
 
 using namespace itp;
@@ -40,7 +41,9 @@ struct itp_ctx {
   int eig_grid = 0;    // workgroups of the eigensolver (0: sized by the matrix; the grid barrier cost grows with them)
   int eig_mode = 0;    // 0: Jacobi handing off to the refinement iteration, 1: Jacobi only
   double eig_handoff = 0.5;   // hand-off threshold on |a_ij| / |a_ii - a_jj|
+  double eig_cluster = 0.1;   // cluster start: pairs above this |a_ij| / |a_ii - a_jj| are solved exactly (0: off)
   long long oa_done = 0, oa_fallback = 0;
+  long long cl_calls = 0, cl_clusters = 0, cl_largest = 0, cl_fallback = 0;
 };
 
 struct itp_grid {
@@ -83,6 +86,11 @@ struct itp_stateset {
   double* oa_lam = nullptr;
   double* oa_slots = nullptr;
   int* oa_state = nullptr;
+  // cluster start: pair counter and largest ratio, strong pairs, cluster members and offsets
+  unsigned long long* cl_hdr = nullptr;
+  int* cl_pairs = nullptr;
+  int* cl_members = nullptr;
+  int* cl_offsets = nullptr;
   cplx<double>* C = nullptr;
   double* lam = nullptr;
   int* perm = nullptr;
@@ -121,6 +129,7 @@ namespace {
 
 constexpr int kMaxSweeps = 40;
 constexpr int kOAMaxIter = 8;
+constexpr int kClusterPairCap = 8192;  // strong pairs listed for the cluster start; more: block Jacobi
 
 int set_err(itp_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -355,15 +364,142 @@ std::mutex& coop_mutex() {
   return m;
 }
 
+// Connected components of the strong-pair graph on N indices, sizes >= 2,
+// as members (by component, ascending indices) and offsets. Returns the
+// largest component size (0 if there is no pair).
+int strong_clusters(const std::vector<int>& pairs, int N, std::vector<int>& members, std::vector<int>& offsets) {
+  std::vector<int> parent(N);
+  for (int i = 0; i < N; i++) parent[i] = i;
+  auto find = [&](int x) {
+    while (parent[x] != x) {
+      parent[x] = parent[parent[x]];
+      x = parent[x];
+    }
+    return x;
+  };
+  for (size_t p = 0; p + 1 < pairs.size(); p += 2) {
+    const int a = find(pairs[p]), b = find(pairs[p + 1]);
+    if (a != b) parent[a < b ? b : a] = a < b ? a : b;
+  }
+  std::vector<int> size(N, 0);
+  for (int i = 0; i < N; i++) size[find(i)]++;
+  std::vector<int> start(N, -1);
+  members.clear();
+  offsets.assign(1, 0);
+  int largest = 0;
+  std::vector<std::vector<int>> comp;
+  std::vector<int> slot(N, -1);
+  for (int i = 0; i < N; i++) {
+    const int r = find(i);
+    if (size[r] < 2) continue;
+    if (slot[r] < 0) {
+      slot[r] = (int)comp.size();
+      comp.emplace_back();
+    }
+    comp[slot[r]].push_back(i);
+  }
+  for (auto& cm : comp) {
+    members.insert(members.end(), cm.begin(), cm.end());
+    offsets.push_back((int)members.size());
+    largest = std::max<int>(largest, (int)cm.size());
+  }
+  return largest;
+}
+
+// Cluster start for the refinement (cluster_eig.hpp): strong pairs of A[0]
+// on the device, their clusters on the host, exact cluster eigenvectors into
+// V[0] (identity elsewhere), then the refinement from there. Returns 1 if the
+// refinement converged (lam, perm and C are then final and A[0] keeps S
+// apart from its diagonal), 0 if the block-Jacobi route must run (too many
+// strong pairs, a cluster too large, or no convergence; A[0] is untouched).
+int cluster_route(itp_stateset* s, bool scale, double* info, int* state, int* bad, std::vector<double>& lam, int& rc) {
+  itp_ctx* c = s->grid->ctx;
+  const int N = (int)s->N;
+  rc = ITP_OK;
+  auto fail = [&](hipError_t e, const char* what) {
+    rc = set_err(c, ITP_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    return 0;
+  };
+  hipError_t e = launch_cluster_pairs(s->A[0], s->np, N, c->eig_cluster, kClusterPairCap, s->cl_hdr, s->cl_pairs,
+                                      c->stream);
+  if (e != hipSuccess) return fail(e, "cluster pairs");
+  unsigned long long hdr[2] = {0, 0};
+  if ((e = hipMemcpyAsync(hdr, s->cl_hdr, sizeof(hdr), hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
+      (e = hipStreamSynchronize(c->stream)) != hipSuccess)
+    return fail(e, "cluster pairs");
+  if (hdr[0] > (unsigned long long)kClusterPairCap) return 0;
+  std::vector<int> pairs(2 * hdr[0]);
+  if (hdr[0] && ((e = hipMemcpyAsync(pairs.data(), s->cl_pairs, pairs.size() * sizeof(int), hipMemcpyDeviceToHost,
+                                     c->stream)) != hipSuccess ||
+                 (e = hipStreamSynchronize(c->stream)) != hipSuccess))
+    return fail(e, "cluster pairs");
+  std::vector<int> members, offsets;
+  const int largest = strong_clusters(pairs, N, members, offsets);
+  if (largest > kMaxCluster) return 0;
+  const int ncl = (int)offsets.size() - 1;
+  const size_t np2 = (size_t)s->np * s->np * 16;
+  if ((e = hipMemcpyAsync(s->V[0], s->Vinit, np2, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess)
+    return fail(e, "cluster start");
+  if (ncl > 0) {
+    // the host vectors outlive the copies: the stream is drained before returning
+    if ((e = hipMemcpyAsync(s->cl_members, members.data(), members.size() * sizeof(int), hipMemcpyHostToDevice,
+                            c->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(s->cl_offsets, offsets.data(), offsets.size() * sizeof(int), hipMemcpyHostToDevice,
+                            c->stream)) != hipSuccess)
+      return fail(e, "cluster upload");
+    if ((e = launch_cluster_solve(s->A[0], s->np, s->cl_members, s->cl_offsets, ncl, s->V[0], c->stream)) !=
+        hipSuccess)
+      return fail(e, "cluster solve");
+  }
+  // the refinement starts when info[1] == 2 (a hand-off)
+  const double start[4] = {0.0, 2.0, 0.0, 0.0};
+  if ((e = hipMemcpyAsync(s->info, start, sizeof(start), hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+    return fail(e, "cluster start");
+  if ((e = launch_oa(s->A[0], s->V[0], s->V[1], s->oa_work, s->oa_lam, s->oa_slots, s->oa_state, s->info, s->np, N,
+                     kOAMaxIter, s->A[0], c->stream)) != hipSuccess)
+    return fail(e, "refinement");
+  if ((e = launch_eig_finish(s->A[0], s->V[0], s->np, N, s->lam, s->perm, s->C, s->nc, scale ? 1 : 0, c->stream)) !=
+      hipSuccess)
+    return fail(e, "eigenvalue sort");
+  if ((e = hipMemcpyAsync(info, s->info, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(state, s->oa_state, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(bad, s->bad, sizeof(int), hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(lam.data(), s->lam, N * sizeof(double), hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
+      (e = hipStreamSynchronize(c->stream)) != hipSuccess)
+    return fail(e, "refinement result");
+  if (state[0] != 1) {
+    c->cl_fallback++;
+    return 0;
+  }
+  c->cl_calls++;
+  c->cl_clusters += ncl;
+  c->cl_largest = std::max<long long>(c->cl_largest, largest);
+  return 1;
+}
+
 int eigen_stage(itp_stateset* s, size_t* bad_n, double* bad_lambda, bool scale, bool check_positive = true) {
   itp_ctx* c = s->grid->ctx;
   const int N = (int)s->N;
   Phase pe(c, ITP_PHASE_EIGENSOLVE);
   const size_t np2 = (size_t)s->np * s->np * 16;
   const bool hybrid = c->eig_mode == 0;
-  // Jacobi sweeps on A (in place, rotations into V); in hybrid mode they stop
-  // as soon as the refinement iteration can take over, which works on the
-  // untouched copy A[1] starting from the Jacobi eigenvectors.
+  // Hybrid mode tries the cluster start first (no Jacobi sweeps when the
+  // matrix is close to diagonal). Otherwise Jacobi sweeps on A (in place,
+  // rotations into V); in hybrid mode they stop as soon as the refinement
+  // iteration can take over, which works on the untouched copy A[1] starting
+  // from the Jacobi eigenvectors.
+  double info[4];
+  int state[2] = {3, 0};
+  int bad = 0;
+  std::vector<double> lam(N);
+  int crc = ITP_OK;
+  const bool clustered = hybrid && c->eig_cluster > 0 && cluster_route(s, scale, info, state, &bad, lam, crc);
+  if (crc) return crc;
+  if (clustered) {
+    c->eig_calls++;
+    c->oa_done++;
+    if (bad) return set_err(c, ITP_E_EIGENSOLVER, "overlap matrix has non-finite entries");
+  } else {
   HIPCK(c, hipMemcpyAsync(s->V[0], s->Vinit, np2, hipMemcpyDeviceToDevice, c->stream));
   if (hybrid) HIPCK(c, hipMemcpyAsync(s->A[1], s->A[0], np2, hipMemcpyDeviceToDevice, c->stream));
   std::unique_lock<std::mutex> coop(coop_mutex());
@@ -373,10 +509,6 @@ int eigen_stage(itp_stateset* s, size_t* bad_n, double* bad_lambda, bool scale, 
     HIPCK(c, launch_oa(s->A[1], s->V[0], s->V[1], s->oa_work, s->oa_lam, s->oa_slots, s->oa_state, s->info, s->np, N,
                        kOAMaxIter, s->A[0], c->stream));
   HIPCK(c, launch_eig_finish(s->A[0], s->V[0], s->np, N, s->lam, s->perm, s->C, s->nc, scale ? 1 : 0, c->stream));
-  double info[4];
-  int state[2] = {3, 0};
-  int bad = 0;
-  std::vector<double> lam(N);
   HIPCK(c, hipMemcpyAsync(info, s->info, sizeof(info), hipMemcpyDeviceToHost, c->stream));
   if (hybrid) HIPCK(c, hipMemcpyAsync(state, s->oa_state, sizeof(state), hipMemcpyDeviceToHost, c->stream));
   HIPCK(c, hipMemcpyAsync(&bad, s->bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -399,6 +531,7 @@ int eigen_stage(itp_stateset* s, size_t* bad_n, double* bad_lambda, bool scale, 
     c->eig_sweeps += (long long)info[0];
   }
   coop.unlock();
+  }
   pe.stop();
   if (info[1] != 1.0 && !(info[1] == 2.0 && state[0] == 1))
     return set_err(c, ITP_E_EIGENSOLVER, "Jacobi eigensolver did not converge in " + std::to_string((int)info[0]) + " sweeps");
@@ -663,6 +796,21 @@ int itp_ctx_oa_stats(itp_ctx* c, long long* refined, long long* fallbacks) {
   return ITP_OK;
 }
 
+int itp_ctx_set_eig_cluster(itp_ctx* c, double tau) {
+  if (!c || !(tau >= 0) || !(tau < 0.5)) return ITP_E_INVALID;
+  c->eig_cluster = tau;
+  return ITP_OK;
+}
+
+int itp_ctx_cluster_stats(itp_ctx* c, long long* out) {
+  if (!c || !out) return ITP_E_INVALID;
+  out[0] = c->cl_calls;
+  out[1] = c->cl_clusters;
+  out[2] = c->cl_largest;
+  out[3] = c->cl_fallback;
+  return ITP_OK;
+}
+
 int itp_ctx_set_eig_grid(itp_ctx* c, int workgroups) {
   if (!c || workgroups < 0 || workgroups > 4096) return ITP_E_INVALID;
   c->eig_grid = workgroups;
@@ -845,6 +993,10 @@ int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out
       if ((rc = dalloc(c, (void**)&s->oa_lam, (size_t)s->np * sizeof(double)))) return rc;
       if ((rc = dalloc(c, (void**)&s->oa_slots, kOAMaxIter * sizeof(double)))) return rc;
       if ((rc = dalloc(c, (void**)&s->oa_state, 2 * sizeof(int)))) return rc;
+      if ((rc = dalloc(c, (void**)&s->cl_hdr, 2 * sizeof(unsigned long long)))) return rc;
+      if ((rc = dalloc(c, (void**)&s->cl_pairs, 2 * (size_t)kClusterPairCap * sizeof(int)))) return rc;
+      if ((rc = dalloc(c, (void**)&s->cl_members, (size_t)s->np * sizeof(int)))) return rc;
+      if ((rc = dalloc(c, (void**)&s->cl_offsets, ((size_t)s->np + 1) * sizeof(int)))) return rc;
     }
     {
       std::vector<cplx<double>> I((size_t)s->np * s->np, mk<double>(0.0, 0.0));
@@ -871,7 +1023,8 @@ void itp_stateset_destroy(itp_stateset* s) {
   if (!s) return;
   (void)hipDeviceSynchronize();  // no parent access: parents may already be gone
   void* ptrs[] = {s->buf[0], s->buf[1], s->part, s->A[0], s->A[1], s->V[0], s->V[1], s->Vinit, s->bjQ, s->bjD,
-                  s->bjslots, s->bjident, s->oa_work, s->oa_lam, s->oa_slots, s->oa_state, s->C, s->C32,
+                  s->bjslots, s->bjident, s->oa_work, s->oa_lam, s->oa_slots, s->oa_state, s->cl_hdr, s->cl_pairs,
+                  s->cl_members, s->cl_offsets, s->C, s->C32,
                   s->lam, s->perm, s->info, s->bad, s->epart, s->dE, s->dmean};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);

@@ -45,6 +45,16 @@ hipError_t launch_block_jacobi(cplx<double>* A, cplx<double>* V, cplx<double>* Q
 hipError_t launch_oa(const cplx<double>* A0, cplx<double>* X0, cplx<double>* X1, cplx<double>* work, double* lam,
                      double* slots, int* state, const double* info, int np, int N, int max_iter, cplx<double>* Aout,
                      hipStream_t st);
+// Strong pairs (i < j < N, |a_ij| > tau |a_ii - a_jj|) of the np x np matrix
+// A: hdr[0] counts them (the first cap are listed in pairs as (i, j)),
+// hdr[1] holds the bits of the largest ratio
+hipError_t launch_cluster_pairs(const cplx<double>* A, int np, int N, double tau, int cap, unsigned long long* hdr,
+                                int* pairs, hipStream_t st);
+// Cluster c (members[offsets[c] .. offsets[c+1]), 2..kMaxCluster indices):
+// eigenvectors of A's principal sub-matrix written into X at those rows and
+// columns (X holds the identity elsewhere)
+hipError_t launch_cluster_solve(const cplx<double>* A, int np, const int* members, const int* offsets, int nclusters,
+                                cplx<double>* X, hipStream_t st);
 hipError_t launch_eig_finish(const cplx<double>* A, const cplx<double>* V, int np, int N, double* lam_sorted,
                              int* perm, cplx<double>* C, int nc, int scale, hipStream_t st);
 template <class R>

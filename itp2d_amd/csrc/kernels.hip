@@ -16,6 +16,7 @@
 #include <hip/hip_cooperative_groups.h>
 
 #include "block_jacobi.hpp"
+#include "cluster_eig.hpp"
 #include "dense_ops.hpp"
 #include "gpu.hpp"
 #include "line_ops.hpp"
@@ -83,9 +84,8 @@ hipError_t launch_reg_sweep(const SweepArgs<R>& a, int nblk, int nfields, hipStr
     constexpr int N = decltype(nc)::value, O = decltype(oc)::value;
     constexpr bool X = decltype(xc)::value;
     // once per instantiation (thread-safe static initialisation)
-    static const bool attr_set = (hipFuncSetAttribute((const void*)reg_sweep_kernel<R, N, O, X>,
-                                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024),
-                                  true);
+    static const hipError_t attr_set = hipFuncSetAttribute((const void*)reg_sweep_kernel<R, N, O, X>,
+                                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     (void)attr_set;
     hipLaunchKernelGGL((reg_sweep_kernel<R, N, O, X>), dim3(nblk, nfields), dim3(kRegThreads), lds, st, a);
     err = hipGetLastError();
@@ -101,8 +101,8 @@ hipError_t launch_sweep(const SweepArgs<R>& a, int nblk, int nfields, hipStream_
   const size_t lds = sweep_lds_bytes<R>(a.L, a.plan.n, 256);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
   // allow the full 160 KiB of LDS per workgroup (once, thread-safe static initialisation)
-  static const bool attr_set =
-      (hipFuncSetAttribute((const void*)sweep_kernel<R>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024), true);
+  static const hipError_t attr_set =
+      hipFuncSetAttribute((const void*)sweep_kernel<R>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
   (void)attr_set;
   if (nblk <= 0 || nfields <= 0) return hipSuccess;
   if (nfields > 65535) return hipErrorInvalidValue;
@@ -608,6 +608,61 @@ hipError_t launch_oa(const cplx<double>* A0, cplx<double>* X0, cplx<double>* X1,
     hipLaunchKernelGGL(oa_decide_kernel, dim3(1), dim3(1), 0, st, slot, k, state);
   }
   hipLaunchKernelGGL(oa_finalize_kernel, dim3(256), dim3(256), 0, st, state, X1, X2, X0, Aout, lam, np, N);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Cluster start for the refinement (cluster_eig.hpp): strong pairs of the
+// upper triangle, then one workgroup per cluster.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void cl_pairs_kernel(const cplx<double>* A, int np, int N, double tau, int cap,
+                                                       unsigned long long* hdr, int* pairs) {
+  const int i = blockIdx.x;  // grid = N rows: the whole block takes the same path
+  const double mx = cl_row_pairs(A, np, N, i, (int)threadIdx.x, (int)blockDim.x, tau, [&](int a, int b) {
+    const unsigned long long slot = atomicAdd(&hdr[0], 1ull);
+    if (slot < (unsigned long long)cap) {
+      pairs[2 * slot] = a;
+      pairs[2 * slot + 1] = b;
+    }
+  });
+  __shared__ double red[256];
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = red[threadIdx.x + w] > red[threadIdx.x] ? red[threadIdx.x + w] : red[threadIdx.x];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(&hdr[1], dbits(red[0]));
+}
+
+hipError_t launch_cluster_pairs(const cplx<double>* A, int np, int N, double tau, int cap, unsigned long long* hdr,
+                                int* pairs, hipStream_t st) {
+  hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, st, hdr, 2ll);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || N < 2) return e;
+  hipLaunchKernelGGL(cl_pairs_kernel, dim3(N), dim3(256), 0, st, A, np, N, tau, cap, hdr, pairs);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void cl_solve_kernel(const cplx<double>* A, int np, const int* members,
+                                                       const int* offsets, cplx<double>* X, double tol,
+                                                       int max_sweeps) {
+  extern __shared__ __align__(16) char cll[];
+  const int c = blockIdx.x;
+  const int o0 = offsets[c], k = offsets[c + 1] - o0;
+  if (k < 1 || k > kMaxCluster) return;  // the host builds 2..kMaxCluster; the whole block agrees
+  GpuExec ex;
+  cl_solve(ex, A, np, members + o0, k, X, tol, max_sweeps, cll);
+}
+
+hipError_t launch_cluster_solve(const cplx<double>* A, int np, const int* members, const int* offsets, int nclusters,
+                                cplx<double>* X, hipStream_t st) {
+  if (nclusters <= 0) return hipSuccess;
+  const size_t lds = cl_lds_bytes(256);
+  static const hipError_t attr =
+      hipFuncSetAttribute((const void*)cl_solve_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (attr != hipSuccess) return attr;
+  hipLaunchKernelGGL(cl_solve_kernel, dim3(nclusters), dim3(256), lds, st, A, np, members, offsets, X, 1e-15, 30);
   return hipGetLastError();
 }
 

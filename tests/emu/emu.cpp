@@ -9,6 +9,7 @@
 #include <omp.h>
 #include "../../itp2d_amd/csrc/engine.hpp"
 #include "../../itp2d_amd/csrc/block_jacobi.hpp"
+#include "../../itp2d_amd/csrc/cluster_eig.hpp"
 #include <algorithm>
 
 using namespace itp;
@@ -310,6 +311,35 @@ int emu_block_jacobi_dbg(int n, double* Ain, double* w, int max_inner, int max_s
     for (int c = 0; c < n; c++) out[(size_t)r * n + c] = V[(size_t)c * np + idx[r]];
   }
   return sweeps;
+}
+
+// Cluster solve (cluster_eig.hpp) of the principal sub-matrix of the n x n
+// Hermitian A (row-major, complex) on idx[0..k): writes the cluster's
+// eigenvectors into X (row-major, n x n, left as given elsewhere).
+int emu_cluster_solve(int n, const double* A, const int* idx, int k, double* X) {
+  if (k < 1 || k > kMaxCluster) return -1;
+  std::vector<char> lds(cl_lds_bytes(256));
+  CpuExec ex;
+  cl_solve(ex, reinterpret_cast<const cplx<double>*>(A), n, idx, k, reinterpret_cast<cplx<double>*>(X), 1e-15, 30,
+           lds.data());
+  return 0;
+}
+
+// Strong pairs of rows 0..N) as the device kernel lists them (one emulated
+// workgroup per row); returns the count and fills pairs (up to cap).
+int emu_cluster_pairs(int np, int N, const double* A, double tau, int cap, int* pairs, double* maxrho) {
+  int cnt = 0;
+  double mx = 0;
+  for (int i = 0; i < N; i++)
+    for (int t = 0; t < 256; t++) {
+      const double m = cl_row_pairs(reinterpret_cast<const cplx<double>*>(A), np, N, i, t, 256, tau, [&](int a, int b) {
+        if (cnt < cap) { pairs[2 * cnt] = a; pairs[2 * cnt + 1] = b; }
+        cnt++;
+      });
+      mx = std::max(mx, m);
+    }
+  *maxrho = mx;
+  return cnt;
 }
 
 }  // extern "C"

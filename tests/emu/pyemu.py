@@ -26,6 +26,8 @@ def lib():
         _lib.emu_plan.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         _lib.emu_block_jacobi.argtypes = [C.c_int, _dp, _dp, C.c_int]
         _lib.emu_set_reg.argtypes = [C.c_int]
+        _lib.emu_cluster_solve.argtypes = [C.c_int, _dp, C.POINTER(C.c_int), C.c_int, _dp]
+        _lib.emu_cluster_pairs.argtypes = [C.c_int, C.c_int, _dp, C.c_double, C.c_int, C.POINTER(C.c_int), _dp]
         _lib.emu_exppotential.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, _dp, C.c_double, C.c_double,
                                           C.c_double, _dp, C.c_int]
     return _lib
@@ -98,3 +100,26 @@ def block_jacobi(A, max_inner=8):
     w = np.zeros(n)
     sweeps = lib().emu_block_jacobi(n, _p(a.view(np.float64)), _p(w), max_inner)
     return w, np.ascontiguousarray(a.reshape(n, n).T), sweeps
+
+
+def cluster_solve(A, idx, X=None):
+    """Emulated cluster solve: X (identity by default) with the eigenvectors of A[idx][:, idx] at idx."""
+    A = np.ascontiguousarray(A, dtype=np.complex128)
+    n = A.shape[0]
+    X = np.eye(n, dtype=np.complex128) if X is None else np.ascontiguousarray(X, dtype=np.complex128).copy()
+    ix = np.ascontiguousarray(idx, dtype=np.int32)
+    rc = lib().emu_cluster_solve(n, _p(A.view(np.float64)), ix.ctypes.data_as(C.POINTER(C.c_int)), len(ix),
+                                 _p(X.view(np.float64)))
+    assert rc == 0
+    return X
+
+
+def cluster_pairs(A, N, tau, cap=8192):
+    """Emulated strong-pair listing: (count, pairs (as listed, up to cap), largest ratio)."""
+    A = np.ascontiguousarray(A, dtype=np.complex128)
+    np_ = A.shape[0]
+    pairs = np.zeros(2 * cap, dtype=np.int32)
+    mx = np.zeros(1)
+    cnt = lib().emu_cluster_pairs(np_, N, _p(A.view(np.float64)), tau, cap, pairs.ctypes.data_as(C.POINTER(C.c_int)),
+                                  _p(mx))
+    return cnt, pairs[: 2 * min(cnt, cap)].reshape(-1, 2), float(mx[0])

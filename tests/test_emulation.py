@@ -104,6 +104,56 @@ def test_block_jacobi_emulated(n):
     assert np.abs(V.conj().T @ V - np.eye(n)).max() < 1e-14 * n
 
 
+@pytest.mark.parametrize("k", [2, 7, 12, 48])
+def test_cluster_solve_emulated(k):
+    """Cluster start (cluster_eig.hpp): the exact eigenvectors of a principal
+    sub-matrix on scattered indices, embedded in the identity, diagonalise that
+    block (eigenvalues of the block as numpy's to rounding), are unitary, and
+    leave every other row and column alone."""
+    rng = np.random.default_rng(k)
+    n = 64
+    d = np.exp(-0.05 * np.sort(rng.uniform(1, 40, n)))
+    Eps = 1e-9 * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    A = np.diag(d) + Eps + Eps.conj().T
+    idx = np.sort(rng.choice(n, size=k, replace=False))
+    # a strongly coupled, nearly degenerate cluster
+    B = rng.standard_normal((k, k)) + 1j * rng.standard_normal((k, k))
+    A[np.ix_(idx, idx)] += 1e-3 * (B + B.conj().T) + np.diag(np.full(k, 0.5 - d[idx]))
+    X = E.cluster_solve(A, idx)
+    U = X[np.ix_(idx, idx)]
+    assert np.abs(U.conj().T @ U - np.eye(k)).max() < 1e-14 * k
+    Acc = A[np.ix_(idx, idx)]
+    D = U.conj().T @ Acc @ U
+    off = D - np.diag(np.diag(D))
+    assert np.abs(off).max() < 1e-15 * k * np.abs(Acc).max()
+    assert np.abs(np.sort(np.diag(D).real) - np.linalg.eigvalsh(Acc)).max() < 1e-15 * k * np.abs(Acc).max()
+    rest = np.setdiff1d(np.arange(n), idx)
+    assert np.array_equal(X[np.ix_(rest, np.arange(n))], np.eye(n)[rest])
+    assert np.array_equal(X[np.ix_(idx, rest)], np.zeros((k, n - k)))
+
+
+def test_cluster_pairs_emulated():
+    """Strong-pair listing: exactly the pairs i < j < N with |a_ij| > tau |a_ii - a_jj|
+    (brute force over the matrix), padded rows and columns ignored."""
+    rng = np.random.default_rng(5)
+    np_, N, tau = 96, 90, 0.1
+    d = np.sort(rng.uniform(0, 1, np_))
+    Eps = 2e-3 * (rng.standard_normal((np_, np_)) + 1j * rng.standard_normal((np_, np_)))
+    A = np.diag(d) + Eps + Eps.conj().T
+    cnt, pairs, mx = E.cluster_pairs(A, N, tau)
+    want = set()
+    rho_max = 0.0
+    for i in range(N):
+        for j in range(i + 1, N):
+            rho = abs(A[i, j]) / abs(A[i, i].real - A[j, j].real)
+            rho_max = max(rho_max, rho)
+            if rho > tau:
+                want.add((i, j))
+    assert cnt == len(want) and set(map(tuple, pairs.tolist())) == want
+    assert abs(mx - rho_max) <= 1e-15 * rho_max
+    assert cnt > 10
+
+
 def test_block_jacobi_near_diagonal_converges_fast():
     rng = np.random.default_rng(2)
     n = 150

@@ -191,6 +191,41 @@ def test_eigh_refinement(ctx, n, cluster):
     assert np.abs(V.conj().T @ V - np.eye(n)).max() < 1e-13
 
 
+@pytest.mark.parametrize("n", [64, 300, 500])
+def test_eigh_cluster_start(ctx, n):
+    """Nearly diagonal matrices with strongly coupled near-degenerate groups
+    (couplings far above their gaps, so the refinement alone cannot start):
+    the cluster start solves the groups exactly and the refinement finishes
+    without a Jacobi sweep; eigenpairs against LAPACK and against the
+    block-Jacobi route on the same matrix."""
+    say(f"eigh cluster start n={n}")
+    rng = np.random.default_rng(2000 + n)
+    d = np.exp(-0.05 * np.sort(rng.uniform(1, 60, n)))
+    X = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    A = np.diag(d) + 1e-6 * (X + X.conj().T) * np.sqrt(np.outer(d, d))
+    groups = [np.arange(5, 8), np.arange(20, 22), np.array([n // 2, n // 2 + 3, n - 9])]
+    for g in groups:
+        A[np.ix_(g, g)] += np.diag(np.full(len(g), d[g[0]]) - d[g])  # degenerate group
+        C = rng.standard_normal((len(g), len(g))) + 1j * rng.standard_normal((len(g), len(g)))
+        A[np.ix_(g, g)] += 1e-4 * d[g[0]] * (C + C.conj().T)
+    sweeps0, (cl0, ncl0, _, fb0) = ctx.eig_stats()[1], ctx.cluster_stats()
+    w, V = I.eigh(ctx, A)
+    sweeps1, (cl1, ncl1, largest, fb1) = ctx.eig_stats()[1], ctx.cluster_stats()
+    assert cl1 == cl0 + 1 and fb1 == fb0
+    assert ncl1 - ncl0 >= len(groups) and largest >= 3
+    assert sweeps1 == sweeps0
+    wl = np.linalg.eigvalsh(A)
+    assert np.abs(w - wl).max() < 1e-14 * np.abs(wl).max()
+    assert np.abs(A @ V - V * w).max() < 1e-13 * np.abs(A).max()
+    assert np.abs(V.conj().T @ V - np.eye(n)).max() < 1e-13
+    try:
+        ctx.set_eig_cluster(0.0)
+        wj, Vj = I.eigh(ctx, A)
+    finally:
+        ctx.set_eig_cluster(0.1)
+    assert np.abs(w - wj).max() < 1e-14 * np.abs(wl).max()
+
+
 def test_zero_state_raises_nonpositive(ctx):
     sx = sy = 16
     dx = 0.5
