@@ -91,6 +91,7 @@ struct itp_stateset {
   int* cl_pairs = nullptr;
   int* cl_members = nullptr;
   int* cl_offsets = nullptr;
+  cplx<double>* Vsave = nullptr;  // eigenvector estimate before a cluster start, restored if its refinement fails
   cplx<double>* C = nullptr;
   double* lam = nullptr;
   int* perm = nullptr;
@@ -406,27 +407,49 @@ int strong_clusters(const std::vector<int>& pairs, int N, std::vector<int>& memb
   return largest;
 }
 
-// Cluster start for the refinement (cluster_eig.hpp): strong pairs of A[0]
-// on the device, their clusters on the host, exact cluster eigenvectors into
-// V[0] (identity elsewhere), then the refinement from there. Returns 1 if the
-// refinement converged (lam, perm and C are then final and A[0] keeps S
-// apart from its diagonal), 0 if the block-Jacobi route must run (too many
-// strong pairs, a cluster too large, or no convergence; A[0] is untouched).
-int cluster_route(itp_stateset* s, bool scale, double* info, int* state, int* bad, std::vector<double>& lam, int& rc) {
+// Result of one eigensolver attempt, read back after the stream drained.
+struct EigRead {
+  double info[4] = {0, 0, 0, 0};
+  int state[2] = {3, 0};
+  int bad = 0;
+};
+
+int read_eig(itp_stateset* s, EigRead& r, std::vector<double>& lam, bool oa) {
+  itp_ctx* c = s->grid->ctx;
+  HIPCK(c, hipMemcpyAsync(r.info, s->info, sizeof(r.info), hipMemcpyDeviceToHost, c->stream));
+  if (oa) HIPCK(c, hipMemcpyAsync(r.state, s->oa_state, sizeof(r.state), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipMemcpyAsync(&r.bad, s->bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipMemcpyAsync(lam.data(), s->lam, s->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  return ITP_OK;
+}
+
+// Cluster start for the refinement (cluster_eig.hpp). Acur = V^H S V is the
+// current (partly rotated) matrix with eigenvector estimate V = V[0], S = A0
+// the original matrix. Strong pairs of Acur on the device, their clusters on
+// the host; each cluster's exact rotation is applied to V's cluster columns,
+// and the refinement iteration runs on S from there (writing the eigenvalues
+// into Acur's diagonal on convergence). Returns 1 on convergence (lam, perm
+// and C final), 0 otherwise: too many strong pairs or a cluster too large
+// (nothing changed), or no convergence (V restored from Vsave).
+int cluster_route(itp_stateset* s, cplx<double>* Acur, const cplx<double>* A0, bool scale, EigRead& rd,
+                  std::vector<double>& lam, int& rc) {
   itp_ctx* c = s->grid->ctx;
   const int N = (int)s->N;
+  const size_t np2 = (size_t)s->np * s->np * 16;
   rc = ITP_OK;
   auto fail = [&](hipError_t e, const char* what) {
     rc = set_err(c, ITP_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
     return 0;
   };
-  hipError_t e = launch_cluster_pairs(s->A[0], s->np, N, c->eig_cluster, kClusterPairCap, s->cl_hdr, s->cl_pairs,
+  hipError_t e = launch_cluster_pairs(Acur, s->np, N, c->eig_cluster, kClusterPairCap, s->cl_hdr, s->cl_pairs,
                                       c->stream);
   if (e != hipSuccess) return fail(e, "cluster pairs");
   unsigned long long hdr[2] = {0, 0};
   if ((e = hipMemcpyAsync(hdr, s->cl_hdr, sizeof(hdr), hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
       (e = hipStreamSynchronize(c->stream)) != hipSuccess)
     return fail(e, "cluster pairs");
+  rd.state[0] = 3;  // refinement not run (yet)
   if (hdr[0] > (unsigned long long)kClusterPairCap) return 0;
   std::vector<int> pairs(2 * hdr[0]);
   if (hdr[0] && ((e = hipMemcpyAsync(pairs.data(), s->cl_pairs, pairs.size() * sizeof(int), hipMemcpyDeviceToHost,
@@ -437,38 +460,33 @@ int cluster_route(itp_stateset* s, bool scale, double* info, int* state, int* ba
   const int largest = strong_clusters(pairs, N, members, offsets);
   if (largest > kMaxCluster) return 0;
   const int ncl = (int)offsets.size() - 1;
-  const size_t np2 = (size_t)s->np * s->np * 16;
-  if ((e = hipMemcpyAsync(s->V[0], s->Vinit, np2, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess)
-    return fail(e, "cluster start");
   if (ncl > 0) {
+    if ((e = hipMemcpyAsync(s->Vsave, s->V[0], np2, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess)
+      return fail(e, "cluster start");
     // the host vectors outlive the copies: the stream is drained before returning
     if ((e = hipMemcpyAsync(s->cl_members, members.data(), members.size() * sizeof(int), hipMemcpyHostToDevice,
                             c->stream)) != hipSuccess ||
         (e = hipMemcpyAsync(s->cl_offsets, offsets.data(), offsets.size() * sizeof(int), hipMemcpyHostToDevice,
                             c->stream)) != hipSuccess)
       return fail(e, "cluster upload");
-    if ((e = launch_cluster_solve(s->A[0], s->np, s->cl_members, s->cl_offsets, ncl, s->V[0], c->stream)) !=
-        hipSuccess)
+    if ((e = launch_cluster_solve(Acur, s->np, s->cl_members, s->cl_offsets, ncl, s->V[0], c->stream)) != hipSuccess)
       return fail(e, "cluster solve");
   }
   // the refinement starts when info[1] == 2 (a hand-off)
   const double start[4] = {0.0, 2.0, 0.0, 0.0};
   if ((e = hipMemcpyAsync(s->info, start, sizeof(start), hipMemcpyHostToDevice, c->stream)) != hipSuccess)
     return fail(e, "cluster start");
-  if ((e = launch_oa(s->A[0], s->V[0], s->V[1], s->oa_work, s->oa_lam, s->oa_slots, s->oa_state, s->info, s->np, N,
-                     kOAMaxIter, s->A[0], c->stream)) != hipSuccess)
+  if ((e = launch_oa(A0, s->V[0], s->V[1], s->oa_work, s->oa_lam, s->oa_slots, s->oa_state, s->info, s->np, N,
+                     kOAMaxIter, Acur, c->stream)) != hipSuccess)
     return fail(e, "refinement");
-  if ((e = launch_eig_finish(s->A[0], s->V[0], s->np, N, s->lam, s->perm, s->C, s->nc, scale ? 1 : 0, c->stream)) !=
+  if ((e = launch_eig_finish(Acur, s->V[0], s->np, N, s->lam, s->perm, s->C, s->nc, scale ? 1 : 0, c->stream)) !=
       hipSuccess)
     return fail(e, "eigenvalue sort");
-  if ((e = hipMemcpyAsync(info, s->info, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
-      (e = hipMemcpyAsync(state, s->oa_state, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
-      (e = hipMemcpyAsync(bad, s->bad, sizeof(int), hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
-      (e = hipMemcpyAsync(lam.data(), s->lam, N * sizeof(double), hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
-      (e = hipStreamSynchronize(c->stream)) != hipSuccess)
-    return fail(e, "refinement result");
-  if (state[0] != 1) {
+  if ((rc = read_eig(s, rd, lam, true))) return 0;
+  if (rd.state[0] != 1) {
     c->cl_fallback++;
+    if (ncl > 0 && (e = hipMemcpyAsync(s->V[0], s->Vsave, np2, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess)
+      return fail(e, "cluster restore");
     return 0;
   }
   c->cl_calls++;
@@ -483,58 +501,77 @@ int eigen_stage(itp_stateset* s, size_t* bad_n, double* bad_lambda, bool scale, 
   Phase pe(c, ITP_PHASE_EIGENSOLVE);
   const size_t np2 = (size_t)s->np * s->np * 16;
   const bool hybrid = c->eig_mode == 0;
-  // Hybrid mode tries the cluster start first (no Jacobi sweeps when the
-  // matrix is close to diagonal). Otherwise Jacobi sweeps on A (in place,
-  // rotations into V); in hybrid mode they stop as soon as the refinement
-  // iteration can take over, which works on the untouched copy A[1] starting
-  // from the Jacobi eigenvectors.
-  double info[4];
-  int state[2] = {3, 0};
-  int bad = 0;
+  const bool clusters = hybrid && c->eig_cluster > 0;
+  // Mode 1: Jacobi sweeps to convergence (on A in place, rotations into V).
+  // Mode 0 (hybrid): the refinement iteration on the untouched copy A[1]
+  // finishes as soon as it can start: from the cluster start (tried before
+  // the first sweep and after each sweep, until one attempt's refinement
+  // fails), or when the Jacobi sweeps hand off (every unconverged coupling
+  // below eig_handoff times its gap). If the refinement fails after a
+  // hand-off, the sweeps continue to convergence.
+  EigRead rd;
   std::vector<double> lam(N);
-  int crc = ITP_OK;
-  const bool clustered = hybrid && c->eig_cluster > 0 && cluster_route(s, scale, info, state, &bad, lam, crc);
-  if (crc) return crc;
-  if (clustered) {
-    c->eig_calls++;
-    c->oa_done++;
-    if (bad) return set_err(c, ITP_E_EIGENSOLVER, "overlap matrix has non-finite entries");
-  } else {
+  // a non-finite overlap entry (flagged by the Gram kernels) ends here
+  HIPCK(c, hipMemcpyAsync(&rd.bad, s->bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCK(c, hipMemcpyAsync(s->V[0], s->Vinit, np2, hipMemcpyDeviceToDevice, c->stream));
   if (hybrid) HIPCK(c, hipMemcpyAsync(s->A[1], s->A[0], np2, hipMemcpyDeviceToDevice, c->stream));
-  std::unique_lock<std::mutex> coop(coop_mutex());
-  HIPCK(c, launch_block_jacobi(s->A[0], s->V[0], s->bjQ, s->bjD, s->bjident, s->bjslots, s->np, kMaxSweeps,
-                               c->eig_inner, 1e-14, hybrid ? c->eig_handoff : 0.0, s->info, c->eig_grid, c->stream));
-  if (hybrid)
-    HIPCK(c, launch_oa(s->A[1], s->V[0], s->V[1], s->oa_work, s->oa_lam, s->oa_slots, s->oa_state, s->info, s->np, N,
-                       kOAMaxIter, s->A[0], c->stream));
-  HIPCK(c, launch_eig_finish(s->A[0], s->V[0], s->np, N, s->lam, s->perm, s->C, s->nc, scale ? 1 : 0, c->stream));
-  HIPCK(c, hipMemcpyAsync(info, s->info, sizeof(info), hipMemcpyDeviceToHost, c->stream));
-  if (hybrid) HIPCK(c, hipMemcpyAsync(state, s->oa_state, sizeof(state), hipMemcpyDeviceToHost, c->stream));
-  HIPCK(c, hipMemcpyAsync(&bad, s->bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  HIPCK(c, hipMemcpyAsync(lam.data(), s->lam, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCK(c, hipStreamSynchronize(c->stream));
+  if (rd.bad) {
+    c->eig_calls++;
+    return set_err(c, ITP_E_EIGENSOLVER, "overlap matrix has non-finite entries");
+  }
+  bool done = false, try_clusters = clusters, handoff = hybrid;
+  int sweeps = 0, rc = ITP_OK;
+  if (try_clusters) {
+    done = cluster_route(s, s->A[0], s->A[1], scale, rd, lam, rc) != 0;
+    if (rc) return rc;
+    if (done) c->oa_done++;
+    else if (rd.state[0] != 3) try_clusters = false;  // its refinement failed: no more attempts
+  }
+  while (!done && sweeps < kMaxSweeps) {
+    // one sweep per launch while cluster starts are tried, else up to the budget
+    {
+      std::unique_lock<std::mutex> coop(coop_mutex());
+      HIPCK(c, launch_block_jacobi(s->A[0], s->V[0], s->bjQ, s->bjD, s->bjident, s->bjslots, s->np,
+                                   try_clusters ? 1 : kMaxSweeps - sweeps, c->eig_inner, 1e-14,
+                                   handoff ? c->eig_handoff : 0.0, s->info, c->eig_grid, c->stream));
+      HIPCK(c, hipMemcpyAsync(rd.info, s->info, sizeof(rd.info), hipMemcpyDeviceToHost, c->stream));
+      HIPCK(c, hipStreamSynchronize(c->stream));
+    }
+    sweeps += (int)rd.info[0];
+    c->eig_sweeps += (long long)rd.info[0];
+    if (rd.info[1] == 1.0) {  // converged
+      HIPCK(c, launch_eig_finish(s->A[0], s->V[0], s->np, N, s->lam, s->perm, s->C, s->nc, scale ? 1 : 0, c->stream));
+      if ((rc = read_eig(s, rd, lam, false))) return rc;
+      done = true;
+      break;
+    }
+    if (rd.info[1] == 2.0) {  // handed off: refinement from the Jacobi eigenvectors
+      HIPCK(c, launch_oa(s->A[1], s->V[0], s->V[1], s->oa_work, s->oa_lam, s->oa_slots, s->oa_state, s->info, s->np,
+                         N, kOAMaxIter, s->A[0], c->stream));
+      HIPCK(c, launch_eig_finish(s->A[0], s->V[0], s->np, N, s->lam, s->perm, s->C, s->nc, scale ? 1 : 0, c->stream));
+      if ((rc = read_eig(s, rd, lam, true))) return rc;
+      if (rd.state[0] == 1) {
+        c->oa_done++;
+        done = true;
+        break;
+      }
+      c->oa_fallback++;  // the refinement failed: sweeps to convergence
+      handoff = false;
+      try_clusters = false;
+      continue;
+    }
+    if (try_clusters) {
+      done = cluster_route(s, s->A[0], s->A[1], scale, rd, lam, rc) != 0;
+      if (rc) return rc;
+      if (done) c->oa_done++;
+      else if (rd.state[0] != 3) try_clusters = false;
+    }
+  }
   c->eig_calls++;
-  c->eig_sweeps += (long long)info[0];
-  if (bad) return set_err(c, ITP_E_EIGENSOLVER, "overlap matrix has non-finite entries");
-  if (info[1] == 2.0 && state[0] == 1) c->oa_done++;
-  if (info[1] == 2.0 && state[0] != 1) {
-    // the refinement did not converge: the Jacobi sweeps continue from where
-    // they handed off (A[0] and V[0] are untouched by the refinement)
-    c->oa_fallback++;
-    HIPCK(c, launch_block_jacobi(s->A[0], s->V[0], s->bjQ, s->bjD, s->bjident, s->bjslots, s->np, kMaxSweeps,
-                                 c->eig_inner, 1e-14, 0.0, s->info, c->eig_grid, c->stream));
-    HIPCK(c, launch_eig_finish(s->A[0], s->V[0], s->np, N, s->lam, s->perm, s->C, s->nc, scale ? 1 : 0, c->stream));
-    HIPCK(c, hipMemcpyAsync(info, s->info, sizeof(info), hipMemcpyDeviceToHost, c->stream));
-    HIPCK(c, hipMemcpyAsync(lam.data(), s->lam, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCK(c, hipStreamSynchronize(c->stream));
-    c->eig_sweeps += (long long)info[0];
-  }
-  coop.unlock();
-  }
   pe.stop();
-  if (info[1] != 1.0 && !(info[1] == 2.0 && state[0] == 1))
-    return set_err(c, ITP_E_EIGENSOLVER, "Jacobi eigensolver did not converge in " + std::to_string((int)info[0]) + " sweeps");
+  if (!done)
+    return set_err(c, ITP_E_EIGENSOLVER, "Jacobi eigensolver did not converge in " + std::to_string(sweeps) + " sweeps");
   s->lam_host = lam;
   for (int n = 0; n < N; n++) {
     const double ev = lam[n];
@@ -997,6 +1034,7 @@ int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out
       if ((rc = dalloc(c, (void**)&s->cl_pairs, 2 * (size_t)kClusterPairCap * sizeof(int)))) return rc;
       if ((rc = dalloc(c, (void**)&s->cl_members, (size_t)s->np * sizeof(int)))) return rc;
       if ((rc = dalloc(c, (void**)&s->cl_offsets, ((size_t)s->np + 1) * sizeof(int)))) return rc;
+      if ((rc = dalloc(c, (void**)&s->Vsave, np2))) return rc;
     }
     {
       std::vector<cplx<double>> I((size_t)s->np * s->np, mk<double>(0.0, 0.0));
@@ -1024,7 +1062,7 @@ void itp_stateset_destroy(itp_stateset* s) {
   (void)hipDeviceSynchronize();  // no parent access: parents may already be gone
   void* ptrs[] = {s->buf[0], s->buf[1], s->part, s->A[0], s->A[1], s->V[0], s->V[1], s->Vinit, s->bjQ, s->bjD,
                   s->bjslots, s->bjident, s->oa_work, s->oa_lam, s->oa_slots, s->oa_state, s->cl_hdr, s->cl_pairs,
-                  s->cl_members, s->cl_offsets, s->C, s->C32,
+                  s->cl_members, s->cl_offsets, s->Vsave, s->C, s->C32,
                   s->lam, s->perm, s->info, s->bad, s->epart, s->dE, s->dmean};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);

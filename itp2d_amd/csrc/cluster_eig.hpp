@@ -49,16 +49,18 @@ ITP_HD double cl_row_pairs(const cplx<double>* A, int np, int N, int i, int j0, 
   return mx;
 }
 
-// Exact eigen-decomposition of the principal sub-matrix of A on the k
-// indices idx[0..k) (k <= kMaxCluster), written as the k x k block of X at
-// those rows and columns (X[idx[a]][idx[b]] = U[a][b], eigenvector b in
-// column b). Cyclic two-sided Jacobi on the size padded to even (a zero pad
-// row and column need no rotation), round-robin pairings of dense_ops.hpp;
-// stops once every off-diagonal element is below tol sqrt(|a_ii a_jj|) or
-// k eps max|a_ii|, or after max_sweeps sweeps.
+// Exact eigen-decomposition U of the principal sub-matrix of A on the k
+// indices idx[0..k) (k <= kMaxCluster), applied to the np x np matrix X on
+// those columns: X[:, idx] <- X[:, idx] U (with X the identity this writes U
+// as the k x k block at rows and columns idx, eigenvector b in column b).
+// Cyclic two-sided Jacobi on the size padded to even (a zero pad row and
+// column need no rotation), round-robin pairings of dense_ops.hpp; stops
+// once every off-diagonal element is below tol sqrt(|a_ii a_jj|) or
+// k eps max|a_ii|, or after max_sweeps sweeps. Clusters have disjoint
+// columns, so clusters update X concurrently without overlap.
 template <class Exec>
 ITP_HD void cl_solve(Exec& ex, const cplx<double>* A, int np, const int* idx, int k, cplx<double>* X, double tol,
-                     int max_sweeps, void* lds) {
+                     int max_sweeps, void* lds, int nrows) {
   const int k2 = k + (k & 1), m = k2 / 2, kk = k2 * k2;
   cplx<double>* M[2];
   cplx<double>* U[2];
@@ -153,13 +155,28 @@ ITP_HD void cl_solve(Exec& ex, const cplx<double>* A, int np, const int* idx, in
       cur ^= 1;
     }
   }
+  // X[:, idx] <- X[:, idx] U over the first nrows rows, in chunks of
+  // kClRows rows staged in the now free M buffers
   const cplx<double>* Uf = U[cur];
-  ex.run([&](int tid, int nt) {
-    for (int e = tid; e < k * k; e += nt) {
-      const int a = e / k, b = e - a * k;
-      X[(long long)idx[a] * np + idx[b]] = Uf[a * k2 + b];
-    }
-  });
+  cplx<double>* Xs = M[0];
+  constexpr int kClRows = 64;
+  for (int r0 = 0; r0 < nrows; r0 += kClRows) {
+    const int rn = nrows - r0 < kClRows ? nrows - r0 : kClRows;
+    ex.run([&](int tid, int nt) {
+      for (int e = tid; e < rn * k; e += nt) {
+        const int r = e / k, a = e - r * k;
+        Xs[e] = X[(long long)(r0 + r) * np + idx[a]];
+      }
+    });
+    ex.run([&](int tid, int nt) {
+      for (int e = tid; e < rn * k; e += nt) {
+        const int r = e / k, b = e - r * k;
+        cplx<double> acc = mk<double>(0.0, 0.0);
+        for (int a = 0; a < k; a++) acc = acc + Xs[r * k + a] * Uf[a * k2 + b];
+        X[(long long)(r0 + r) * np + idx[b]] = acc;
+      }
+    });
+  }
 }
 
 }  // namespace itp

@@ -51,8 +51,8 @@ hipError_t launch_oa(const cplx<double>* A0, cplx<double>* X0, cplx<double>* X1,
 hipError_t launch_cluster_pairs(const cplx<double>* A, int np, int N, double tau, int cap, unsigned long long* hdr,
                                 int* pairs, hipStream_t st);
 // Cluster c (members[offsets[c] .. offsets[c+1]), 2..kMaxCluster indices):
-// eigenvectors of A's principal sub-matrix written into X at those rows and
-// columns (X holds the identity elsewhere)
+// X[:, members] <- X[:, members] U_c with U_c the eigenvectors of A's
+// principal sub-matrix on the members (clusters are disjoint)
 hipError_t launch_cluster_solve(const cplx<double>* A, int np, const int* members, const int* offsets, int nclusters,
                                 cplx<double>* X, hipStream_t st);
 hipError_t launch_eig_finish(const cplx<double>* A, const cplx<double>* V, int np, int N, double* lam_sorted,

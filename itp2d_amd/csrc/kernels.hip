@@ -652,7 +652,7 @@ __global__ __launch_bounds__(256) void cl_solve_kernel(const cplx<double>* A, in
   const int o0 = offsets[c], k = offsets[c + 1] - o0;
   if (k < 1 || k > kMaxCluster) return;  // the host builds 2..kMaxCluster; the whole block agrees
   GpuExec ex;
-  cl_solve(ex, A, np, members + o0, k, X, tol, max_sweeps, cll);
+  cl_solve(ex, A, np, members + o0, k, X, tol, max_sweeps, cll, np);
 }
 
 hipError_t launch_cluster_solve(const cplx<double>* A, int np, const int* members, const int* offsets, int nclusters,

@@ -314,14 +314,14 @@ int emu_block_jacobi_dbg(int n, double* Ain, double* w, int max_inner, int max_s
 }
 
 // Cluster solve (cluster_eig.hpp) of the principal sub-matrix of the n x n
-// Hermitian A (row-major, complex) on idx[0..k): writes the cluster's
-// eigenvectors into X (row-major, n x n, left as given elsewhere).
+// Hermitian A (row-major, complex) on idx[0..k): X[:, idx] <- X[:, idx] U
+// with U the cluster's eigenvectors (X row-major, n x n).
 int emu_cluster_solve(int n, const double* A, const int* idx, int k, double* X) {
   if (k < 1 || k > kMaxCluster) return -1;
   std::vector<char> lds(cl_lds_bytes(256));
   CpuExec ex;
   cl_solve(ex, reinterpret_cast<const cplx<double>*>(A), n, idx, k, reinterpret_cast<cplx<double>*>(X), 1e-15, 30,
-           lds.data());
+           lds.data(), n);
   return 0;
 }
 

@@ -132,6 +132,23 @@ def test_cluster_solve_emulated(k):
     assert np.array_equal(X[np.ix_(idx, rest)], np.zeros((k, n - k)))
 
 
+def test_cluster_solve_updates_columns_emulated():
+    """With a general X the cluster rotation multiplies X's cluster columns:
+    X[:, idx] U, every other column untouched."""
+    rng = np.random.default_rng(9)
+    n, k = 80, 9
+    d = np.sort(rng.uniform(0.1, 1.0, n))
+    Eps = 1e-3 * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    A = np.diag(d) + Eps + Eps.conj().T
+    idx = np.sort(rng.choice(n, size=k, replace=False))
+    X0 = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))[0]
+    U = E.cluster_solve(A, idx)[np.ix_(idx, idx)]
+    X = E.cluster_solve(A, idx, X0)
+    assert np.abs(X[:, idx] - X0[:, idx] @ U).max() < 1e-14
+    rest = np.setdiff1d(np.arange(n), idx)
+    assert np.array_equal(X[:, rest], X0[:, rest])
+
+
 def test_cluster_pairs_emulated():
     """Strong-pair listing: exactly the pairs i < j < N with |a_ij| > tau |a_ii - a_jj|
     (brute force over the matrix), padded rows and columns ignored."""
