@@ -431,9 +431,10 @@ int read_eig(itp_stateset* s, EigRead& r, std::vector<double>& lam, bool oa) {
 // and the refinement iteration runs on S from there (writing the eigenvalues
 // into Acur's diagonal on convergence). Returns 1 on convergence (lam, perm
 // and C final), 0 otherwise: too many strong pairs or a cluster too large
-// (nothing changed), or no convergence (V restored from Vsave).
-int cluster_route(itp_stateset* s, cplx<double>* Acur, const cplx<double>* A0, bool scale, EigRead& rd,
-                  std::vector<double>& lam, int& rc) {
+// (nothing changed), or no convergence (V restored from Vsave). ident_start:
+// V is the identity on entry (no sweep has run).
+int cluster_route(itp_stateset* s, cplx<double>* Acur, const cplx<double>* A0, bool ident_start, bool scale,
+                  EigRead& rd, std::vector<double>& lam, int& rc) {
   itp_ctx* c = s->grid->ctx;
   const int N = (int)s->N;
   const size_t np2 = (size_t)s->np * s->np * 16;
@@ -476,8 +477,10 @@ int cluster_route(itp_stateset* s, cplx<double>* Acur, const cplx<double>* A0, b
   const double start[4] = {0.0, 2.0, 0.0, 0.0};
   if ((e = hipMemcpyAsync(s->info, start, sizeof(start), hipMemcpyHostToDevice, c->stream)) != hipSuccess)
     return fail(e, "cluster start");
+  // without clusters, a start that is the identity (before any sweep) needs
+  // no products in the refinement's first iteration
   if ((e = launch_oa(A0, s->V[0], s->V[1], s->oa_work, s->oa_lam, s->oa_slots, s->oa_state, s->info, s->np, N,
-                     kOAMaxIter, Acur, c->stream)) != hipSuccess)
+                     kOAMaxIter, Acur, c->stream, ident_start && ncl == 0 ? s->Vinit : nullptr)) != hipSuccess)
     return fail(e, "refinement");
   if ((e = launch_eig_finish(Acur, s->V[0], s->np, N, s->lam, s->perm, s->C, s->nc, scale ? 1 : 0, c->stream)) !=
       hipSuccess)
@@ -523,7 +526,7 @@ int eigen_stage(itp_stateset* s, size_t* bad_n, double* bad_lambda, bool scale, 
   bool done = false, try_clusters = clusters, handoff = hybrid;
   int sweeps = 0, rc = ITP_OK;
   if (try_clusters) {
-    done = cluster_route(s, s->A[0], s->A[1], scale, rd, lam, rc) != 0;
+    done = cluster_route(s, s->A[0], s->A[1], true, scale, rd, lam, rc) != 0;
     if (rc) return rc;
     if (done) c->oa_done++;
     else if (rd.state[0] != 3) try_clusters = false;  // its refinement failed: no more attempts
@@ -562,7 +565,7 @@ int eigen_stage(itp_stateset* s, size_t* bad_n, double* bad_lambda, bool scale, 
       continue;
     }
     if (try_clusters) {
-      done = cluster_route(s, s->A[0], s->A[1], scale, rd, lam, rc) != 0;
+      done = cluster_route(s, s->A[0], s->A[1], false, scale, rd, lam, rc) != 0;
       if (rc) return rc;
       if (done) c->oa_done++;
       else if (rd.state[0] != 3) try_clusters = false;

@@ -465,8 +465,11 @@ hipError_t launch_block_jacobi(cplx<double>* A, cplx<double>* V, cplx<double>* Q
 // one 16 x 16 quarter per wavefront.
 __global__ __launch_bounds__(256) void zgemm_kernel(const cplx<double>* A, int ah, const cplx<double>* B,
                                                     cplx<double>* C, const cplx<double>* Cadd, int n,
-                                                    const int* state) {
+                                                    const int* state, int herm) {
   if (state && state[0] != 0) return;
+  // herm: C is Hermitian (A^H B with B = A, or with B = M A for Hermitian M):
+  // the upper tiles only, each written with its mirror
+  if (herm && blockIdx.y > blockIdx.x) return;
   __shared__ cplx<double> As[32][17];
   __shared__ cplx<double> Bs[16][33];
   const int i0 = blockIdx.y * 32, j0 = blockIdx.x * 32;
@@ -505,6 +508,8 @@ __global__ __launch_bounds__(256) void zgemm_kernel(const cplx<double>* A, int a
     cplx<double> r = mk<double>(cr[v], ci[v]);
     if (Cadd) r = Cadd[o] + r;
     C[o] = r;
+    if (herm && blockIdx.y != blockIdx.x)
+      C[(long long)(j0 + wj * 16 + li) * n + i0 + wi * 16 + lk + 4 * v] = conj(r);
   }
 }
 
@@ -581,7 +586,7 @@ __global__ void oa_finalize_kernel(const int* state, const cplx<double>* X1, con
 
 hipError_t launch_oa(const cplx<double>* A0, cplx<double>* X0, cplx<double>* X1, cplx<double>* work, double* lam,
                      double* slots, int* state, const double* info, int np, int N, int max_iter, cplx<double>* Aout,
-                     hipStream_t st) {
+                     hipStream_t st, const cplx<double>* x0_identity) {
   if (np % 32) return hipErrorInvalidValue;
   const long long n2 = (long long)np * np;
   cplx<double>* Y = work;
@@ -599,12 +604,18 @@ hipError_t launch_oa(const cplx<double>* A0, cplx<double>* X0, cplx<double>* X1,
     cplx<double>* X = k == 0 ? X0 : ((k & 1) ? X1 : X2);
     cplx<double>* Xn = (k & 1) ? X2 : X1;
     unsigned long long* slot = reinterpret_cast<unsigned long long*>(slots) + k;
-    hipLaunchKernelGGL(zgemm_kernel, gg, dim3(256), 0, st, A0, 0, X, Y, nullptr, np, state);
-    hipLaunchKernelGGL(zgemm_kernel, gg, dim3(256), 0, st, X, 1, Y, S, nullptr, np, state);
-    hipLaunchKernelGGL(zgemm_kernel, gg, dim3(256), 0, st, X, 1, X, G, nullptr, np, state);
-    hipLaunchKernelGGL(oa_update_kernel, dim3((np + 255) / 256, np), dim3(256), 0, st, S, G, E, lam, np, N, slot,
-                       state);
-    hipLaunchKernelGGL(zgemm_kernel, gg, dim3(256), 0, st, X, 0, E, Xn, X, np, state);
+    if (k == 0 && x0_identity) {
+      // X = I: S = A0 and G = I, no products needed
+      hipLaunchKernelGGL(oa_update_kernel, dim3((np + 255) / 256, np), dim3(256), 0, st, A0, x0_identity, E, lam,
+                         np, N, slot, state);
+    } else {
+      hipLaunchKernelGGL(zgemm_kernel, gg, dim3(256), 0, st, A0, 0, X, Y, nullptr, np, state, 0);
+      hipLaunchKernelGGL(zgemm_kernel, gg, dim3(256), 0, st, X, 1, Y, S, nullptr, np, state, 1);
+      hipLaunchKernelGGL(zgemm_kernel, gg, dim3(256), 0, st, X, 1, X, G, nullptr, np, state, 1);
+      hipLaunchKernelGGL(oa_update_kernel, dim3((np + 255) / 256, np), dim3(256), 0, st, S, G, E, lam, np, N, slot,
+                         state);
+    }
+    hipLaunchKernelGGL(zgemm_kernel, gg, dim3(256), 0, st, X, 0, E, Xn, X, np, state, 0);
     hipLaunchKernelGGL(oa_decide_kernel, dim3(1), dim3(1), 0, st, slot, k, state);
   }
   hipLaunchKernelGGL(oa_finalize_kernel, dim3(256), dim3(256), 0, st, state, X1, X2, X0, Aout, lam, np, N);
@@ -666,17 +677,31 @@ hipError_t launch_cluster_solve(const cplx<double>* A, int np, const int* member
   return hipGetLastError();
 }
 
-// Ascending order of the eigenvalues (ties by index, like a stable sort).
-__global__ void eig_sort_kernel(const cplx<double>* A, int np, int N, double* lam_sorted, int* perm) {
+// Ascending order of the eigenvalues (ties by index, like a stable sort):
+// each thread ranks one eigenvalue against the diagonal, which every
+// workgroup stages through LDS in chunks (read as broadcasts).
+__global__ __launch_bounds__(256) void eig_sort_kernel(const cplx<double>* A, int np, int N, double* lam_sorted,
+                                                       int* perm) {
+  constexpr int kChunk = 2048;
+  __shared__ double d[kChunk];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const double li = A[(long long)i * np + i].x;
+  const double li = i < N ? A[(long long)i * np + i].x : 0.0;
   int rank = 0;
-  for (int j = 0; j < N; j++) {
-    const double lj = A[(long long)j * np + j].x;
-    rank += (lj < li) || (lj == li && j < i);
+  for (int j0 = 0; j0 < N; j0 += kChunk) {
+    const int jn = N - j0 < kChunk ? N - j0 : kChunk;
+    __syncthreads();
+    for (int j = threadIdx.x; j < jn; j += blockDim.x) d[j] = A[(long long)(j0 + j) * np + (j0 + j)].x;
+    __syncthreads();
+    if (i < N)
+      for (int j = 0; j < jn; j++) {
+        const double lj = d[j];
+        rank += (lj < li) || (lj == li && j0 + j < i);
+      }
   }
-  if (rank < N) { lam_sorted[rank] = li; perm[rank] = i; }
+  if (i < N && rank < N) {
+    lam_sorted[rank] = li;
+    perm[rank] = i;
+  }
 }
 
 // C[c][r] = v_{perm r}[c] / sqrt(lambda_r)  (stateset.cpp:175 then the row-major read at :205-210)
@@ -700,7 +725,7 @@ hipError_t launch_eig_finish(const cplx<double>* A, const cplx<double>* V, int n
   hipLaunchKernelGGL(fill_kernel, dim3((N + 255) / 256), dim3(256), 0, st, lam_sorted, (long long)N, (double)NAN);
   hipError_t e = hipMemsetAsync(perm, 0xff, (size_t)N * sizeof(int), st);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(eig_sort_kernel, dim3((N + 127) / 128), dim3(128), 0, st, A, np, N, lam_sorted, perm);
+  hipLaunchKernelGGL(eig_sort_kernel, dim3((N + 255) / 256), dim3(256), 0, st, A, np, N, lam_sorted, perm);
   hipLaunchKernelGGL(eig_coeff_kernel, dim3((N + 127) / 128, N), dim3(128), 0, st, V, np, N, lam_sorted, perm, C, nc, scale);
   return hipGetLastError();
 }
