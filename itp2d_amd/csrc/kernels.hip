@@ -732,33 +732,65 @@ hipError_t launch_eig_finish(const cplx<double>* A, const cplx<double>* V, int n
 
 // ---------------------------------------------------------------------------
 // Rotation Psi_out[r][t] = sum_c C[c][r] Psi_in[c][t] on FP64 MFMA.
-// Tile: 64 output states x 64 grid points per workgroup, K = input states.
+// Tile: 64 output states x 64 grid points per workgroup, K = input states,
+// staged 16 at a time through LDS with the next step's operands prefetched
+// into registers. Workgroups are dealt to the 8 XCDs round-robin in launch
+// order; the block order below gives each XCD whole grid-point slabs and
+// runs all output-state tiles of a slab back to back there, so the slab
+// (N x 64 values) is read from HBM once and reused from that XCD's L2.
 // ---------------------------------------------------------------------------
+constexpr int kXcds = 8;
+
+// (output-state tile, grid-point tile) of launch-order block b; false for
+// the padding blocks past the last slab.
+__device__ __forceinline__ bool rotate_tile(int b, int ntr, long long ntt, int& rt, long long& tt) {
+  const int xcd = b % kXcds, slot = b / kXcds;
+  rt = slot % ntr;
+  tt = (long long)(slot / ntr) * kXcds + xcd;
+  return tt < ntt;
+}
+
 template <class R>
 __global__ __launch_bounds__(256) void rotate_kernel(const cplx<R>* in, cplx<R>* out, const cplx<double>* C, int nc,
                                                      long long M, int N) {
   __shared__ cplx<double> Cs[16][65];
   __shared__ cplx<double> Ps[16][65];
-  const long long t0 = (long long)blockIdx.x * 64;
-  const int r0 = blockIdx.y * 64;
+  int rt;
+  long long tt;
+  if (!rotate_tile((int)blockIdx.x, nc / 64, (M + 63) / 64, rt, tt)) return;
+  const long long t0 = tt * 64;
+  const int r0 = rt * 64;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wr = wave >> 1, wt = wave & 1;
   const int li = lane & 15, lk = lane >> 4;
   d4 accr[2][2], acci[2][2];
   for (int a = 0; a < 2; a++)
     for (int b = 0; b < 2; b++)
       for (int v = 0; v < 4; v++) { accr[a][b][v] = 0; acci[a][b][v] = 0; }
-  for (int c0 = 0; c0 < N; c0 += 16) {
+  // thread's staging slots: element e = tid + 256 q of the 16 x 64 tiles
+  cplx<double> pc[4], pp[4];
+  auto fetch = [&](int c0) {
+#pragma unroll
     for (int q = 0; q < 4; q++) {
       const int e = tid + 256 * q;
       const int kc = e >> 6, x = e & 63;
       const int c = c0 + kc;
       // C is zero-padded to nc x nc; rows c >= N contribute nothing
-      Cs[kc][x] = (c < N) ? C[(long long)c * nc + r0 + x] : mk<double>(0.0, 0.0);
+      pc[q] = (c < N) ? C[(long long)c * nc + r0 + x] : mk<double>(0.0, 0.0);
       cplx<double> v = mk<double>(0.0, 0.0);
       if (c < N && t0 + x < M) { const cplx<R> p = in[(long long)c * M + t0 + x]; v = mk<double>(double(p.x), double(p.y)); }
-      Ps[kc][x] = v;
+      pp[q] = v;
+    }
+  };
+  fetch(0);
+  for (int c0 = 0; c0 < N; c0 += 16) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int e = tid + 256 * q;
+      Cs[e >> 6][e & 63] = pc[q];
+      Ps[e >> 6][e & 63] = pp[q];
     }
     __syncthreads();
+    if (c0 + 16 < N) fetch(c0 + 16);
 #pragma unroll
     for (int kk = 0; kk < 4; kk++) {
       cplx<double> a[2], b[2];
@@ -786,9 +818,11 @@ __global__ __launch_bounds__(256) void rotate_kernel(const cplx<R>* in, cplx<R>*
 template <class R>
 hipError_t launch_rotate(const cplx<R>* in, cplx<R>* out, const cplx<double>* C, int nc, long long M, int N,
                          hipStream_t st) {
-  const long long tiles = (M + 63) / 64;
-  if (tiles > 0x7fffffffLL) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(rotate_kernel<R>, dim3((unsigned)tiles, nc / 64), dim3(256), 0, st, in, out, C, nc, M, N);
+  const long long ntt = (M + 63) / 64;
+  const int ntr = nc / 64;
+  const long long blocks = (ntt + kXcds - 1) / kXcds * kXcds * ntr;
+  if (nc % 64 || blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(rotate_kernel<R>, dim3((unsigned)blocks), dim3(256), 0, st, in, out, C, nc, M, N);
   return hipGetLastError();
 }
 template hipError_t launch_rotate<double>(const cplx<double>*, cplx<double>*, const cplx<double>*, int, long long, int,

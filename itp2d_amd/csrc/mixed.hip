@@ -196,12 +196,27 @@ hipError_t launch_gram_bf16x3(const cplx<float>* A, const cplx<float>* B, long l
 // FP32 copy of the coefficients, on v_mfma_f32_16x16x4_f32: 64 output states
 // x 64 grid points per workgroup, one 32 x 32 quarter per wavefront.
 // ---------------------------------------------------------------------------
+// Launch-order block b -> (output-state tile, grid-point tile): each of the 8
+// XCDs (dealt blocks round-robin) gets whole grid-point slabs and runs all
+// output-state tiles of a slab back to back, so the slab is read from HBM
+// once and reused from that XCD's L2. False for padding blocks.
+constexpr int kRotXcds = 8;
+__device__ __forceinline__ bool rotate_f32_tile(int b, int ntr, long long ntt, int& rt, long long& tt) {
+  const int xcd = b % kRotXcds, slot = b / kRotXcds;
+  rt = slot % ntr;
+  tt = (long long)(slot / ntr) * kRotXcds + xcd;
+  return tt < ntt;
+}
+
 __global__ __launch_bounds__(256) void rotate_f32_kernel(const cplx<float>* in, cplx<float>* out, const cplx<float>* C,
                                                          int nc, long long M, int N) {
   __shared__ cplx<float> Cs[16][65];
   __shared__ cplx<float> Ps[16][65];
-  const long long t0 = (long long)blockIdx.x * 64;
-  const int r0 = blockIdx.y * 64;
+  int rt;
+  long long tt;
+  if (!rotate_f32_tile((int)blockIdx.x, nc / 64, (M + 63) / 64, rt, tt)) return;
+  const long long t0 = tt * 64;
+  const int r0 = rt * 64;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wr = wave >> 1, wt = wave & 1;
   const int li = lane & 15, lk = lane >> 4;
   f4 accr[2][2], acci[2][2];
@@ -255,9 +270,10 @@ __global__ __launch_bounds__(256) void rotate_f32_kernel(const cplx<float>* in, 
 
 hipError_t launch_rotate_f32(const cplx<float>* in, cplx<float>* out, const cplx<float>* C, int nc, long long M, int N,
                              hipStream_t st) {
-  const long long tiles = (M + 63) / 64;
-  if (tiles > 0x7fffffffLL || nc % 64) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(rotate_f32_kernel, dim3((unsigned)tiles, nc / 64), dim3(256), 0, st, in, out, C, nc, M, N);
+  const long long ntt = (M + 63) / 64;
+  const long long blocks = (ntt + kRotXcds - 1) / kRotXcds * kRotXcds * (nc / 64);
+  if (blocks > 0x7fffffffLL || nc % 64) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(rotate_f32_kernel, dim3((unsigned)blocks), dim3(256), 0, st, in, out, C, nc, M, N);
   return hipGetLastError();
 }
 
