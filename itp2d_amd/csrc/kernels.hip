@@ -223,12 +223,18 @@ template hipError_t launch_noise<float>(uint64_t, long long, int, double, double
 template <class R>
 __global__ __launch_bounds__(256) void gram_kernel(const cplx<R>* psi, const cplx<R>* phi, long long M, int N, int nb,
                                                    long long kchunk, cplx<double>* part, int nc, int full) {
+  // block order by xcd_tile: every XCD takes whole grid chunks and all tile
+  // pairs of a chunk, so the chunk's state slabs are reused from its L2
+  const int npairs = full ? nb * nb : nb * (nb + 1) / 2;
+  int pair;
+  long long chunk;
+  if (!xcd_tile(blockIdx.x, npairs, (M + kchunk - 1) / kchunk, pair, chunk)) return;
   int bi = 0, bj = 0;
   if (full) {
-    bi = blockIdx.x / nb;
-    bj = blockIdx.x - bi * nb;
+    bi = pair / nb;
+    bj = pair - bi * nb;
   } else {
-    int rem = blockIdx.x;
+    int rem = pair;
     while (rem >= nb - bi) { rem -= nb - bi; bi++; }
     bj = bi + rem;
   }
@@ -240,7 +246,7 @@ __global__ __launch_bounds__(256) void gram_kernel(const cplx<R>* psi, const cpl
   for (int a = 0; a < 2; a++)
     for (int b = 0; b < 2; b++)
       for (int v = 0; v < 4; v++) { accr[a][b][v] = 0; acci[a][b][v] = 0; }
-  const long long k0 = (long long)blockIdx.y * kchunk;
+  const long long k0 = chunk * kchunk;
   const long long k1 = (k0 + kchunk < M) ? k0 + kchunk : M;
   for (long long t0 = k0; t0 < k1; t0 += 16) {
     for (int r = 0; r < 4; r++) {
@@ -272,7 +278,7 @@ __global__ __launch_bounds__(256) void gram_kernel(const cplx<R>* psi, const cpl
     }
     __syncthreads();
   }
-  cplx<double>* P = part + (long long)blockIdx.y * nc * nc;
+  cplx<double>* P = part + chunk * nc * nc;
   for (int mi = 0; mi < 2; mi++)
     for (int nj = 0; nj < 2; nj++)
       for (int v = 0; v < 4; v++) {
@@ -310,7 +316,10 @@ hipError_t launch_gram(const cplx<R>* psi, const cplx<R>* phi, long long M, int 
                        int full) {
   const int nb = nc / 64;
   const int npairs = full ? nb * nb : nb * (nb + 1) / 2;
-  hipLaunchKernelGGL(gram_kernel<R>, dim3(npairs, nchunk), dim3(256), 0, st, psi, phi, M, N, nb, kchunk, part, nc,
+  if (nchunk != (M + kchunk - 1) / kchunk) return hipErrorInvalidValue;
+  const long long blocks = xcd_grid(npairs, nchunk);
+  if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(gram_kernel<R>, dim3((unsigned)blocks), dim3(256), 0, st, psi, phi, M, N, nb, kchunk, part, nc,
                      full);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -739,17 +748,6 @@ hipError_t launch_eig_finish(const cplx<double>* A, const cplx<double>* V, int n
 // runs all output-state tiles of a slab back to back there, so the slab
 // (N x 64 values) is read from HBM once and reused from that XCD's L2.
 // ---------------------------------------------------------------------------
-constexpr int kXcds = 8;
-
-// (output-state tile, grid-point tile) of launch-order block b; false for
-// the padding blocks past the last slab.
-__device__ __forceinline__ bool rotate_tile(int b, int ntr, long long ntt, int& rt, long long& tt) {
-  const int xcd = b % kXcds, slot = b / kXcds;
-  rt = slot % ntr;
-  tt = (long long)(slot / ntr) * kXcds + xcd;
-  return tt < ntt;
-}
-
 template <class R>
 __global__ __launch_bounds__(256) void rotate_kernel(const cplx<R>* in, cplx<R>* out, const cplx<double>* C, int nc,
                                                      long long M, int N) {
@@ -757,7 +755,7 @@ __global__ __launch_bounds__(256) void rotate_kernel(const cplx<R>* in, cplx<R>*
   __shared__ cplx<double> Ps[16][65];
   int rt;
   long long tt;
-  if (!rotate_tile((int)blockIdx.x, nc / 64, (M + 63) / 64, rt, tt)) return;
+  if (!xcd_tile(blockIdx.x, nc / 64, (M + 63) / 64, rt, tt)) return;
   const long long t0 = tt * 64;
   const int r0 = rt * 64;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wr = wave >> 1, wt = wave & 1;
@@ -818,9 +816,7 @@ __global__ __launch_bounds__(256) void rotate_kernel(const cplx<R>* in, cplx<R>*
 template <class R>
 hipError_t launch_rotate(const cplx<R>* in, cplx<R>* out, const cplx<double>* C, int nc, long long M, int N,
                          hipStream_t st) {
-  const long long ntt = (M + 63) / 64;
-  const int ntr = nc / 64;
-  const long long blocks = (ntt + kXcds - 1) / kXcds * kXcds * ntr;
+  const long long blocks = xcd_grid(nc / 64, (M + 63) / 64);
   if (nc % 64 || blocks > 0x7fffffffLL) return hipErrorInvalidValue;
   hipLaunchKernelGGL(rotate_kernel<R>, dim3((unsigned)blocks), dim3(256), 0, st, in, out, C, nc, M, N);
   return hipGetLastError();

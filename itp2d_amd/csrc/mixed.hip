@@ -58,17 +58,21 @@ __global__ __launch_bounds__(256) void gram_bf16x3_kernel(const float* A, const 
                                                           long long kchunk, cplx<double>* part, int nc, int full) {
   __shared__ __align__(16) __bf16 As[3][kG3Rows][kG3KT + kG3Pad];
   __shared__ __align__(16) __bf16 Bs[3][kG3Rows][kG3KT + kG3Pad];
-  // tile pair from blockIdx.x: all of them if full, else I <= J
+  // tile pair (all of them if full, else I <= J) and grid chunk in the block
+  // order of xcd_tile: every XCD takes whole chunks and all pairs of a chunk
+  const int npairs = full ? nb * nb : nb * (nb + 1) / 2;
+  int pair;
+  long long chunk;
+  if (!xcd_tile(blockIdx.x, npairs, (M + kchunk - 1) / kchunk, pair, chunk)) return;
   int I = 0, J = 0;
   if (full) {
-    I = blockIdx.x / nb;
-    J = blockIdx.x - I * nb;
+    I = pair / nb;
+    J = pair - I * nb;
   } else {
-    int rem = blockIdx.x;
+    int rem = pair;
     while (rem >= nb - I) { rem -= nb - I; I++; }
     J = I + rem;
   }
-  const int chunk = blockIdx.y;
   const long long K2 = 2 * M;  // reals per field
   const long long k_begin = (long long)chunk * 2 * kchunk;
   const long long k_end = k_begin + 2 * kchunk < K2 ? k_begin + 2 * kchunk : K2;
@@ -184,7 +188,10 @@ hipError_t launch_gram_bf16x3(const cplx<float>* A, const cplx<float>* B, long l
   if (nc % kG3Rows || (2 * kchunk) % kG3Sub || M % 2) return hipErrorInvalidValue;
   const int nb = nc / kG3Rows;
   const int npairs = full ? nb * nb : nb * (nb + 1) / 2;
-  hipLaunchKernelGGL(gram_bf16x3_kernel, dim3(npairs, nchunk), dim3(256), 0, st, reinterpret_cast<const float*>(A),
+  if (nchunk != (M + kchunk - 1) / kchunk) return hipErrorInvalidValue;
+  const long long blocks = xcd_grid(npairs, nchunk);
+  if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(gram_bf16x3_kernel, dim3((unsigned)blocks), dim3(256), 0, st, reinterpret_cast<const float*>(A),
                      reinterpret_cast<const float*>(B), M, N, nb, kchunk, part, nc, full);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -196,25 +203,15 @@ hipError_t launch_gram_bf16x3(const cplx<float>* A, const cplx<float>* B, long l
 // FP32 copy of the coefficients, on v_mfma_f32_16x16x4_f32: 64 output states
 // x 64 grid points per workgroup, one 32 x 32 quarter per wavefront.
 // ---------------------------------------------------------------------------
-// Launch-order block b -> (output-state tile, grid-point tile): each of the 8
-// XCDs (dealt blocks round-robin) gets whole grid-point slabs and runs all
-// output-state tiles of a slab back to back, so the slab is read from HBM
-// once and reused from that XCD's L2. False for padding blocks.
-constexpr int kRotXcds = 8;
-__device__ __forceinline__ bool rotate_f32_tile(int b, int ntr, long long ntt, int& rt, long long& tt) {
-  const int xcd = b % kRotXcds, slot = b / kRotXcds;
-  rt = slot % ntr;
-  tt = (long long)(slot / ntr) * kRotXcds + xcd;
-  return tt < ntt;
-}
-
 __global__ __launch_bounds__(256) void rotate_f32_kernel(const cplx<float>* in, cplx<float>* out, const cplx<float>* C,
                                                          int nc, long long M, int N) {
   __shared__ cplx<float> Cs[16][65];
   __shared__ cplx<float> Ps[16][65];
   int rt;
   long long tt;
-  if (!rotate_f32_tile((int)blockIdx.x, nc / 64, (M + 63) / 64, rt, tt)) return;
+  // block order by xcd_tile: each XCD takes whole grid-point slabs (read from
+  // HBM once) and all output-state tiles of a slab
+  if (!xcd_tile(blockIdx.x, nc / 64, (M + 63) / 64, rt, tt)) return;
   const long long t0 = tt * 64;
   const int r0 = rt * 64;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wr = wave >> 1, wt = wave & 1;
@@ -270,8 +267,7 @@ __global__ __launch_bounds__(256) void rotate_f32_kernel(const cplx<float>* in, 
 
 hipError_t launch_rotate_f32(const cplx<float>* in, cplx<float>* out, const cplx<float>* C, int nc, long long M, int N,
                              hipStream_t st) {
-  const long long ntt = (M + 63) / 64;
-  const long long blocks = (ntt + kRotXcds - 1) / kRotXcds * kRotXcds * (nc / 64);
+  const long long blocks = xcd_grid(nc / 64, (M + 63) / 64);
   if (blocks > 0x7fffffffLL || nc % 64) return hipErrorInvalidValue;
   hipLaunchKernelGGL(rotate_f32_kernel, dim3((unsigned)blocks), dim3(256), 0, st, in, out, C, nc, M, N);
   return hipGetLastError();
