@@ -89,6 +89,16 @@ double itps_total_time(const itps_system* s);
 int itps_recoveries(const itps_system* s);
 /* Steps of the mixed path's FP32 phase (included in itps_total_steps). */
 int itps_mixed_steps(const itps_system* s);
+/* Abort flag polled at the start of every step (the reference's
+ * abort_flagptr, itpsystem.cpp:407-416, raised by its SIGINT handler,
+ * itp2d.cpp:35-60): when *flag is non-zero the run finishes with the error
+ * flag set. NULL removes it. The pointer must stay valid while steps run. */
+int itps_set_abort_flag(itps_system* s, const volatile int* flag);
+/* Progress printout on stdout/stderr as the reference's verbosity levels
+ * (0 quiet, the default; 1: start, epsilon changes and final message;
+ * 2: per-step lines, phase ratios and final energies; 3: energies every
+ * step), itpsystem.cpp:158-200,405-470,541-589. */
+int itps_set_verbosity(itps_system* s, int level);
 /* Final-convergence flag of each sorted position (N ints). */
 int itps_converged(const itps_system* s, int* flags);
 /* Rayleigh-Ritz pass on the current states (itp_rayleigh_ritz): replaces the

@@ -142,6 +142,8 @@ def lib():
             "itps_converged": [_vp, C.POINTER(C.c_int)],
             "itps_rayleigh_ritz": [_vp],
             "itps_mixed_steps": [_vp],
+            "itps_set_abort_flag": [_vp, C.POINTER(C.c_int)],
+            "itps_set_verbosity": [_vp, C.c_int],
             "itp_batch_create": [C.c_int, C.c_int, C.POINTER(_vp)],
             "itp_batch_destroy": [_vp],
             "itp_batch_add": [_vp, C.POINTER(_Params), _dp, C.POINTER(_sz)],
@@ -533,6 +535,16 @@ class ITPSystem:
     def mixed_steps(self):
         """Steps of the mixed path's FP32 phase."""
         return lib().itps_mixed_steps(self.h)
+
+    def set_abort_flag(self, flag):
+        """Poll a ctypes.c_int at the start of every step; non-zero ends the run with the error flag
+        (the reference's SIGINT abort flag). None removes it."""
+        self._abort = flag
+        self.ctx.check(lib().itps_set_abort_flag(self.h, None if flag is None else C.byref(flag)), "set_abort_flag")
+
+    def set_verbosity(self, level):
+        """Progress printout levels 0-3 as the reference's -v."""
+        self.ctx.check(lib().itps_set_verbosity(self.h, int(level)), "set_verbosity")
 
     def rayleigh_ritz(self):
         """Rayleigh-Ritz pass on the current states (bias-free Ritz values and residual norms)."""

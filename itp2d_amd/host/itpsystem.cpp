@@ -4,7 +4,9 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <ctime>
 #include <deque>
 #include <memory>
 #include <tuple>
@@ -26,6 +28,7 @@ class ITPSystem {
     if (params.n_wanted + params.ignore_lowest > params.N) throw GeneralError("more wanted states than states");
     tr_.reset(new Transformer(*ctx_holder(), p.sx, p.sy, p.lenx / double(p.sx), BoundaryType(p.boundary)));
     pot_.reset(new Potential(*tr_, V));
+    pot_null = V == nullptr;
     states_.reset(new StateSet(*tr_, p.N, Precision(p.precision)));
     H_.reset(new Hamiltonian(*pot_, p.B));
     exhausting = p.exhaust_eps != 0;
@@ -47,23 +50,48 @@ class ITPSystem {
 
   // ITPSystem::step (itpsystem.cpp:407-470)
   void step() {
-    if (total_step_counter == 0) t0 = std::chrono::steady_clock::now();
-    if (total_step_counter >= params.max_steps) { error_flag = true; finish(); return; }
+    if (abort_flag && *abort_flag) {  // raised by the caller's signal handler
+      error_flag = true;
+      finish();
+      return;
+    }
+    if (total_step_counter == 0) {
+      print_initial_message();
+      if (verbosity >= 1) std::fprintf(out, "Starting first propagation step.\n");
+      if (verbosity >= 2) (void)itp_ctx_enable_timing(ctx_, 1);
+      t0 = std::chrono::steady_clock::now();
+    }
+    if (total_step_counter >= params.max_steps) {
+      if (verbosity >= 0)
+        std::fprintf(err, "Error: Maximum number of total steps reached (%d).\nBailing out.\n", params.max_steps);
+      error_flag = true;
+      finish();
+      return;
+    }
     step_counter++;
     total_step_counter++;
+    if (verbosity >= 2)
+      std::fprintf(out, "Step %d (step %d with eps = %e) starting at %s\n", total_step_counter, step_counter, eps,
+                   timestring().c_str());
     T_->operate(*states_);          // propagate (itpsystem.cpp:243-256)
     orthonormalize();               // itpsystem.cpp:258-289
     if (finished) return;
     if (mixed) { mixed_energies_and_tests(); return; }
     calculate_energies();           // itpsystem.cpp:473-511
+    if (verbosity >= 3) print_energies();
     check_timestep_convergence();
     if (all_ts) {
+      if (verbosity >= 2)
+        std::fprintf(out,
+                     "\t\tAll needed states converged in respect to time step with time step = %e after %d steps.\n",
+                     eps, step_counter);
       check_final_convergence();
       if (all_final) { finish(); return; }
       change_time_step();
     } else if (exhausting) {
       change_time_step();
     }
+    if (verbosity > 0) std::fflush(out);
   }
 
   void finish() {
@@ -71,6 +99,64 @@ class ITPSystem {
     (void)itp_ctx_synchronize(ctx_);
     total_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     finished = true;
+    print_final_message();
+    if (verbosity > 0) std::fflush(out);
+  }
+
+  // ITPSystem::print_initial_message (itpsystem.cpp:158-200): the parts this
+  // driver knows (no potential or noise descriptions)
+  void print_initial_message() const {
+    if (verbosity < 1) return;
+    std::fprintf(out, "\tconverging first %zu eigenstates starting from %zu. %zu states propagated in total\n",
+                 params.n_wanted, params.ignore_lowest, params.N);
+    std::fprintf(out, "\tmagnetic field strength: %.3f\n\tgrid: %zux%zu of length %.3f, %s boundary conditions\n",
+                 params.B, params.sx, params.sy, params.lenx, params.boundary ? "Dirichlet" : "periodic");
+    if (T_->halforder == 1 && pot_null) std::fprintf(out, "\tzero potential -> no operator splitting needed\n");
+    else std::fprintf(out, "\toperator splitting order: %d\n", 2 * T_->halforder);
+    if (params.boundary && params.B != 0)
+      std::fprintf(err, "Warning: you are using Dirichlet boundary conditions with a magnetic field. This can cause "
+                        "slower convergence. Please see the README for more details.\n");
+  }
+
+  // ITPSystem::print_energies (itpsystem.cpp:573-589)
+  void print_energies() const {
+    std::fprintf(out, "\tEnergies:\n");
+    for (size_t n = 0; n < esn.size(); n++) {
+      std::fprintf(out, "\t%zu\t%.5f", n, std::get<0>(esn[n]));
+      if (!(params.B != 0 && params.boundary)) std::fprintf(out, " \u00b1 %.5e", std::get<1>(esn[n]));
+      if (!fin_conv[std::get<2>(esn[n])]) std::fprintf(out, " (not converged)");
+      std::fprintf(out, "\n");
+    }
+  }
+
+  // ITPSystem::print_final_message (itpsystem.cpp:541-571); the ratios are
+  // the device phase times (orthonormalization = overlap + eigensolver +
+  // combination) over the total time
+  void print_final_message() const {
+    if (verbosity < 1) return;
+    std::fprintf(out, "Finished at %s.\nTotal %d steps of ITP performed.\nTotal simulation time: %.3f s.\n",
+                 timestring().c_str(), total_step_counter, total_time);
+    if (verbosity < 2) return;
+    double ms[ITP_NUM_PHASES] = {};
+    long long cnt[ITP_NUM_PHASES] = {};
+    (void)itp_ctx_phase_times(ctx_, ms, cnt);
+    const double tot = total_time > 0 ? total_time * 1e3 : 1.0;
+    const double prop = ms[ITP_PHASE_PROPAGATE] / tot, dot = ms[ITP_PHASE_OVERLAP] / tot;
+    const double eig = ms[ITP_PHASE_EIGENSOLVE] / tot, lin = ms[ITP_PHASE_LINCOMB] / tot;
+    const double conv = ms[ITP_PHASE_ENERGIES] / tot, ortho = dot + eig + lin;
+    std::fprintf(out,
+                 "Ratios:\n\tPropagation:         %.3f\n\tOrthonormalization:  %.3f\n\t      dot products:  %.3f\n"
+                 "\t      eigenvalues:   %.3f\n\t      combination:   %.3f\n\tConvergence testing: %.3f\n"
+                 "\tI/O:                 %.3f\n\tOther:               %.3f\n\n",
+                 prop, ortho, dot, eig, lin, conv, 0.0, 1.0 - prop - ortho - conv);
+    if (!error_flag) print_energies();
+  }
+
+  static std::string timestring() {
+    const std::time_t t = std::time(nullptr);
+    char buf[64];
+    std::strftime(buf, sizeof(buf), "%Y-%m-%d %H:%M:%S", std::localtime(&t));
+    return buf;
   }
 
   void sorted(double* E, double* sd, size_t* idx) const {
@@ -92,6 +178,13 @@ class ITPSystem {
   int total_step_counter = 0, step_counter = 0, recoveries = 0;
   size_t n_ts = 0, n_fin = 0;
   double total_time = 0;
+  // abort flag polled at the start of every step (the reference's
+  // abort_flagptr, raised by its SIGINT handler, itp2d.cpp:35-60)
+  const volatile int* abort_flag = nullptr;
+  int verbosity = 0;  // 0 quiet; 1-3 as the reference's -v levels
+  bool pot_null = false;
+  FILE* out = stdout;
+  FILE* err = stderr;
 
  private:
   // the context belongs to the caller; the facade borrows it
@@ -123,8 +216,14 @@ class ITPSystem {
     } else {
       eps /= params.eps_divisor;
     }
-    if (eps < params.min_time_step) { error_flag = true; finish(); return; }
+    if (eps < params.min_time_step) {
+      std::fprintf(err, "Error: time step %e below the minimum %e.\nBailing out.\n", eps, params.min_time_step);
+      error_flag = true;
+      finish();
+      return;
+    }
     T_->set_time_step(eps);
+    if (verbosity >= 1) std::fprintf(out, "\tEpsilon changed to %e.\n", eps);
     step_counter = 0;
     all_ts = false;
     for (size_t n = 0; n < params.N; n++) set_ts(n, false);
@@ -396,6 +495,18 @@ int itps_rayleigh_ritz(itps_system* s) {
   } catch (const std::exception&) {
     return ITP_E_HIP;
   }
+  return ITP_OK;
+}
+
+int itps_set_abort_flag(itps_system* s, const volatile int* flag) {
+  if (!s) return ITP_E_INVALID;
+  s->sys->abort_flag = flag;
+  return ITP_OK;
+}
+
+int itps_set_verbosity(itps_system* s, int level) {
+  if (!s || level < 0 || level > 3) return ITP_E_INVALID;
+  s->sys->verbosity = level;
   return ITP_OK;
 }
 

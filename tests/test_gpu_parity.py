@@ -313,6 +313,33 @@ def test_reference_big_tests(ctx, case):
         assert np.abs(E - known).max() < 2e-3
 
 
+def test_driver_abort_flag_and_printout(ctx, capfd):
+    """The driver's abort flag (polled at each step start, as the reference's
+    SIGINT flag) and its verbosity printout."""
+    import ctypes
+    say("driver abort flag and printout")
+    cfg, V = W.config("c1_ho64")
+    flag = ctypes.c_int(0)
+    s = I.ITPSystem(ctx, cfg, V)
+    s.set_abort_flag(flag)
+    for _ in range(3):
+        s.step()
+    assert s.total_steps == 3 and not s.finished
+    flag.value = 1
+    s.step()
+    assert s.finished and s.error_flag and s.total_steps == 3
+    del s
+    capfd.readouterr()
+    s = I.ITPSystem(ctx, cfg, V)
+    s.set_verbosity(3)
+    assert s.run() == 0
+    out = capfd.readouterr().out
+    assert "Starting first propagation step." in out
+    assert f"Total {s.total_steps} steps of ITP performed." in out
+    assert "Ratios:" in out and "\tEnergies:" in out
+    assert out.count("Step ") >= s.total_steps
+
+
 def test_c2_full_size_steps(ctx):
     """The 256x256, N=500 Dirichlet B=1 workload: a few steps keep the set
     orthonormal and the energies finite and ordered (size-independent checks)."""
