@@ -88,6 +88,40 @@ int itp_transform_host(itp_grid* g, double* fields, size_t nfields, int kind);
  * V == NULL is the null potential (Potential::is_null). ---- */
 int itp_potential_create(itp_grid* g, const double* V, itp_potential** out);
 void itp_potential_destroy(itp_potential* p);
+/* A named potential type of the reference (parse_potential_description and
+ * the PotentialType classes, potentialtypes.cpp:56-500, potentialtypes.hpp:
+ * 72-382): "zero", "harmonic" (A, x0, y0), "elliptic" (Ax, Ay),
+ * "prettyhardsquare" (e), "softpentagon", "henonheiles"/"henon" (a, b),
+ * "gaussian"/"gaussianblob" (A, w, x0, y0), "quartic"/"quarticoscillator" (b),
+ * "squareoscillator" (alpha), "power"/"poweroscillator" (a, A),
+ * "ring"/"ringpotential" (R, w, e, A_asym, w_asym), "cosh"/"coshpotential"
+ * (A, L), "softstadium" (R, L, V, a, b), "powerstadium" (R, L, a), with the
+ * reference's accepted parameter counts, defaults and validity checks
+ * (ITP_E_INVALID otherwise). Evaluated on the device on the cell-centred grid
+ * (Potential::init_values, potential.cpp:54-63). "zero" is the null
+ * potential until impurities are added. */
+int itp_potential_create_type(itp_grid* g, const char* name, const double* params, size_t nparams,
+                              itp_potential** out);
+/* Host-side parameter resolution of itp_potential_create_type: the kind
+ * index (order of the list above) and 5 resolved parameters (defaults filled
+ * in; stadium lengths halved as the reference stores them). */
+int itp_potential_type_params(const char* name, const double* params, size_t nparams, int* kind, double* resolved);
+/* Impurities added to V (SpatialImpurities::add_noise with the ImpurityType
+ * formulas, noise.cpp:25-110), given as realised positions and parameters
+ * (the reference draws them from its own RNG):
+ * ITP_IMP_GAUSSIAN  data [count][4] = x, y, A, w: += A exp(-r^2 / (2 w^2));
+ * ITP_IMP_COULOMB   data [count][2] = x, y; kind_params = {e, alpha}:
+ *                   += alpha r^-e;
+ * ITP_IMP_HEMISPHERE data [count][2]; kind_params = {A, radius}:
+ *                   += A sqrt(1 - r^2 / radius^2) inside the radius;
+ * ITP_IMP_DELTA     data [count][3] = x, y, A: += A at the nearest grid point.
+ * Add impurities before creating splits or Hamiltonians on the potential
+ * (their multiplier tables are made from V at creation). */
+enum itp_impurity_kind { ITP_IMP_GAUSSIAN = 0, ITP_IMP_COULOMB, ITP_IMP_HEMISPHERE, ITP_IMP_DELTA };
+int itp_potential_add_impurities(itp_potential* p, int kind, const double* data, size_t count,
+                                 const double* kind_params);
+/* The potential's values ([sy][sx]; zeros for the null potential). */
+int itp_potential_values(const itp_potential* p, double* V);
 
 /* ---- state set: StateSet storage (stateset.cpp:26-48); ITP_FP64 or ITP_FP32
  * fields. Host buffers are [N][sy][sx] interleaved complex double. ---- */

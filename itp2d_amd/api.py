@@ -144,6 +144,10 @@ def lib():
             "itps_mixed_steps": [_vp],
             "itps_set_abort_flag": [_vp, C.POINTER(C.c_int)],
             "itps_set_verbosity": [_vp, C.c_int],
+            "itp_potential_create_type": [_vp, C.c_char_p, _dp, _sz, C.POINTER(_vp)],
+            "itp_potential_type_params": [C.c_char_p, _dp, _sz, C.POINTER(C.c_int), _dp],
+            "itp_potential_add_impurities": [_vp, C.c_int, _dp, _sz, _dp],
+            "itp_potential_values": [_vp, _dp],
             "itp_batch_create": [C.c_int, C.c_int, C.POINTER(_vp)],
             "itp_batch_destroy": [_vp],
             "itp_batch_add": [_vp, C.POINTER(_Params), _dp, C.POINTER(_sz)],
@@ -304,14 +308,57 @@ class Grid:
             self.h = None
 
 
+POTENTIAL_TYPES = ["zero", "harmonic", "elliptic", "prettyhardsquare", "softpentagon", "henonheiles", "gaussian",
+                   "quartic", "squareoscillator", "power", "ring", "cosh", "softstadium", "powerstadium"]
+IMP_GAUSSIAN, IMP_COULOMB, IMP_HEMISPHERE, IMP_DELTA = 0, 1, 2, 3
+
+
+def potential_type_params(name, params=()):
+    """Kind index and the 5 resolved parameters of a named potential type (host only);
+    raises ITPError for parameters the reference rejects."""
+    a = np.ascontiguousarray(params, dtype=np.float64)
+    kind = C.c_int(-1)
+    out = np.zeros(5)
+    rc = lib().itp_potential_type_params(name.encode(), _p(a) if a.size else None, a.size, C.byref(kind), _p(out))
+    if rc:
+        raise ITPError(rc, f"potential type {name}{tuple(params)}")
+    return kind.value, out
+
+
 class Potential:
     """Potential (potential.hpp:31-58); V=None is the null potential."""
 
-    def __init__(self, grid, V=None):
+    def __init__(self, grid, V=None, _handle=None):
         self.grid = grid
-        self.V = None if V is None else np.ascontiguousarray(V, dtype=np.float64)
         self.h = _vp()
+        if _handle is not None:
+            self.h = _handle
+            self.V = None
+            return
+        self.V = None if V is None else np.ascontiguousarray(V, dtype=np.float64)
         grid.ctx.check(lib().itp_potential_create(grid.h, _p(self.V), C.byref(self.h)), "itp_potential_create")
+
+    @classmethod
+    def of_type(cls, grid, name, params=()):
+        """A named potential type of the reference, evaluated on the device."""
+        a = np.ascontiguousarray(params, dtype=np.float64)
+        h = _vp()
+        grid.ctx.check(lib().itp_potential_create_type(grid.h, name.encode(), _p(a) if a.size else None, a.size,
+                                                       C.byref(h)), f"potential {name}")
+        return cls(grid, _handle=h)
+
+    def add_impurities(self, kind, data, kind_params=None):
+        """Adds realised impurities (rows x, y[, per-impurity parameters]) of a kind."""
+        d = np.ascontiguousarray(data, dtype=np.float64)
+        kp = None if kind_params is None else np.ascontiguousarray(kind_params, dtype=np.float64)
+        n = d.shape[0] if d.size else 0
+        self.grid.ctx.check(lib().itp_potential_add_impurities(self.h, int(kind), _p(d) if n else None, n, _p(kp)),
+                            "add_impurities")
+
+    def values(self):
+        out = np.zeros((self.grid.sy, self.grid.sx))
+        self.grid.ctx.check(lib().itp_potential_values(self.h, _p(out)), "potential values")
+        return out
 
     def apply_exppotential(self, fields, e, coeff=-0.5, prefactor=1.0):
         f = _c128(fields).copy()

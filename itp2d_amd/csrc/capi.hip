@@ -976,6 +976,90 @@ int itp_potential_create(itp_grid* g, const double* V, itp_potential** out) {
   });
 }
 
+int itp_potential_type_params(const char* name, const double* params, size_t nparams, int* kind, double* resolved) {
+  if (!name || (nparams && !params)) return ITP_E_INVALID;
+  PotParams q;
+  if (pot_resolve(name, params, nparams, q)) return ITP_E_INVALID;
+  if (kind) *kind = q.kind;
+  if (resolved) std::copy(q.p, q.p + kPotMaxParams, resolved);
+  return ITP_OK;
+}
+
+int itp_potential_create_type(itp_grid* g, const char* name, const double* params, size_t nparams,
+                              itp_potential** out) {
+  if (!g || !out || !name || (nparams && !params)) return ITP_E_INVALID;
+  itp_ctx* c = g->ctx;
+  PotParams q;
+  if (const char* why = pot_resolve(name, params, nparams, q)) return set_err(c, ITP_E_INVALID, why);
+  return guarded(c, [&]() -> int {
+    std::unique_ptr<itp_potential> p(new itp_potential);
+    p->grid = g;
+    // the zero potential without impurities is the null potential (potential.cpp:28-40)
+    p->null = q.kind == kPotZero;
+    p->V.assign(g->g.M, 0.0);
+    int rc = dalloc(c, (void**)&p->dV, g->g.M * sizeof(double));
+    if (rc) return rc;
+    HIPCK(c, launch_potential(q, g->g.sx, g->g.sy, g->g.dx, p->dV, c->stream));
+    HIPCK(c, hipMemcpyAsync(p->V.data(), p->dV, g->g.M * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    *out = p.release();
+    return ITP_OK;
+  });
+}
+
+int itp_potential_add_impurities(itp_potential* p, int kind, const double* data, size_t count,
+                                 const double* kind_params) {
+  if (!p || kind < 0 || kind >= kImpNumKinds || (count && !data)) return ITP_E_INVALID;
+  if ((kind == kImpCoulomb || kind == kImpHemisphere) && !kind_params) return ITP_E_INVALID;
+  itp_grid* g = p->grid;
+  itp_ctx* c = g->ctx;
+  return guarded(c, [&]() -> int {
+    const GridInfo& gi = g->g;
+    if (!p->dV) {
+      int rc = dalloc(c, (void**)&p->dV, gi.M * sizeof(double));
+      if (rc) return rc;
+      p->V.assign(gi.M, 0.0);
+      HIPCK(c, hipMemcpy(p->dV, p->V.data(), gi.M * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (count) p->null = false;
+    if (kind == kImpDelta) {
+      // one grid point each (DeltaImpurities::add_noise, noise.cpp:80-88): on the host copy
+      HIPCK(c, hipMemcpyAsync(p->V.data(), p->dV, gi.M * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIPCK(c, hipStreamSynchronize(c->stream));
+      for (size_t n = 0; n < count; n++) {
+        const long long x = grid_nearest(data[3 * n], gi.sx, gi.dx), y = grid_nearest(data[3 * n + 1], gi.sy, gi.dx);
+        if (x < 0 || y < 0) return set_err(c, ITP_E_INVALID, "delta impurity outside the grid");
+        p->V[(size_t)y * gi.sx + x] += data[3 * n + 2];
+      }
+      HIPCK(c, hipMemcpy(p->dV, p->V.data(), gi.M * sizeof(double), hipMemcpyHostToDevice));
+      return ITP_OK;
+    }
+    const int stride = kind == kImpGaussian ? 4 : 2;
+    double g0 = 0, g1 = 0;
+    if (kind == kImpCoulomb) { g0 = kind_params[0] / 2; g1 = kind_params[1]; }     // exponent e, prefactor alpha
+    if (kind == kImpHemisphere) { g0 = kind_params[0]; g1 = kind_params[1] * kind_params[1]; }  // amplitude, radius
+    double* dd = nullptr;
+    int rc = dalloc(c, (void**)&dd, std::max<size_t>(1, count) * stride * sizeof(double));
+    if (rc) return rc;
+    hipError_t e = hipMemcpy(dd, data, count * stride * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_impurities(kind, dd, stride, (long long)count, g0, g1, gi.sx, gi.sy, gi.dx, p->dV,
+                                               c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(p->V.data(), p->dV, gi.M * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(dd);
+    if (e != hipSuccess) return set_err(c, ITP_E_HIP, std::string("impurities: ") + hipGetErrorString(e));
+    return ITP_OK;
+  });
+}
+
+int itp_potential_values(const itp_potential* p, double* V) {
+  if (!p || !V) return ITP_E_INVALID;
+  if (p->V.size() != p->grid->g.M) std::fill(V, V + p->grid->g.M, 0.0);
+  else std::copy(p->V.begin(), p->V.end(), V);
+  return ITP_OK;
+}
+
 void itp_potential_destroy(itp_potential* p) {
   if (!p) return;
   (void)hipDeviceSynchronize();  // no parent access: parents may already be gone

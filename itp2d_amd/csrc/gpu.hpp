@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include "line_ops.hpp"
+#include "potentials.hpp"
 
 namespace itp {
 
@@ -9,6 +10,12 @@ template <class R>
 hipError_t launch_sweep(const SweepArgs<R>& a, int nblk, int nfields, hipStream_t st);
 hipError_t launch_energy_reduce(const cplx<double>* p1, const cplx<double>* p2, int nblk, int N, double dx,
                                 double* E, double* sigma, cplx<double>* mean, cplx<double>* msq, hipStream_t st);
+// V (sx x sy, device) = the potential kind q on the cell-centred grid
+hipError_t launch_potential(const PotParams& q, int sx, int sy, double dx, double* V, hipStream_t st);
+// V += count impurities of a kind: data [count][stride] holds x, y and the
+// per-impurity parameters; g0, g1 the kind's shared parameters
+hipError_t launch_impurities(int kind, const double* data, int stride, long long count, double g0, double g1, int sx,
+                             int sy, double dx, double* V, hipStream_t st);
 // n complex elements: cplx<double> -> cplx<float> (to_f32) or back
 hipError_t launch_convert(const void* in, void* out, long long n, int to_f32, hipStream_t st);
 // N seeded noise states of M elements, each of dx-norm 1, from the splitmix64

@@ -20,6 +20,7 @@
 #include "dense_ops.hpp"
 #include "gpu.hpp"
 #include "line_ops.hpp"
+#include "potentials.hpp"
 #include "reg_sweep.hpp"
 
 namespace itp {
@@ -143,6 +144,51 @@ hipError_t launch_convert(const void* in, void* out, long long n, int to_f32, hi
   if (n <= 0) return hipSuccess;
   if (to_f32) hipLaunchKernelGGL(f64_to_f32_kernel, dim3(blocks), dim3(256), 0, st, (const cplx<double>*)in, (cplx<float>*)out, n);
   else hipLaunchKernelGGL(f32_to_f64_kernel, dim3(blocks), dim3(256), 0, st, (const cplx<float>*)in, (cplx<double>*)out, n);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Potentials (potentials.hpp): V of a potential kind on the cell-centred
+// grid, and impurities added point by point in impurity order (the order of
+// the reference's loop over impurities, noise.cpp:80-110).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void potential_kernel(PotParams q, int sx, int sy, double dx, double* V) {
+  const long long M = (long long)sx * sy;
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < M; i += (long long)gridDim.x * 256) {
+    const long long y = i / sx, x = i - y * sx;
+    V[i] = pot_value(q, grid_pos(x, sx, dx), grid_pos(y, sy, dx));
+  }
+}
+
+__global__ __launch_bounds__(256) void impurity_kernel(int kind, const double* data, int stride, long long count,
+                                                       double g0, double g1, int sx, int sy, double dx, double* V) {
+  const long long M = (long long)sx * sy;
+  const double g[2] = {g0, g1};
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < M; i += (long long)gridDim.x * 256) {
+    const long long y = i / sx, x = i - y * sx;
+    const double px = grid_pos(x, sx, dx), py = grid_pos(y, sy, dx);
+    double v = V[i];
+    for (long long n = 0; n < count; n++) {
+      const double* d = data + n * stride;
+      v += imp_value(kind, d + 2, g, d[0], d[1], px, py);
+    }
+    V[i] = v;
+  }
+}
+
+hipError_t launch_potential(const PotParams& q, int sx, int sy, double dx, double* V, hipStream_t st) {
+  const long long M = (long long)sx * sy;
+  const int blocks = (int)std::min<long long>((M + 255) / 256, 8192);
+  hipLaunchKernelGGL(potential_kernel, dim3(blocks), dim3(256), 0, st, q, sx, sy, dx, V);
+  return hipGetLastError();
+}
+
+hipError_t launch_impurities(int kind, const double* data, int stride, long long count, double g0, double g1, int sx,
+                             int sy, double dx, double* V, hipStream_t st) {
+  const long long M = (long long)sx * sy;
+  if (count <= 0) return hipSuccess;
+  const int blocks = (int)std::min<long long>((M + 255) / 256, 8192);
+  hipLaunchKernelGGL(impurity_kernel, dim3(blocks), dim3(256), 0, st, kind, data, stride, count, g0, g1, sx, sy, dx, V);
   return hipGetLastError();
 }
 

@@ -1,0 +1,135 @@
+"""Potential types and impurities (the step before the ITP loop).
+
+CPU: the numpy restatement (oracle/potentials.py) is pinned by values the
+definitions fix by hand (potentialtypes.hpp:96-380), and the library's
+host-side parameter resolution matches the reference constructors' counts,
+defaults and checks (potentialtypes.cpp:23-500). GPU: the device evaluation
+of every type and impurity kind against the restatement.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import itp2d_amd as I
+from itp2d_amd import api
+from oracle import potentials as P
+
+PHI = 1.6180339887498948482
+
+
+@pytest.mark.parametrize("name,params,x,y,want", [
+    ("harmonic", [2.0, 1.0, -1.0], 1.0, -1.0, 0.0),
+    ("harmonic", [2.0, 1.0, -1.0], 2.0, -1.0, 1.0),
+    ("elliptic", [], 1.0, 1.0, 0.5 * (1 + PHI)),
+    ("prettyhardsquare", [2.0], math.pi / 2, 0.1, 1.0),
+    ("softpentagon", [], math.cos(math.pi / 10), math.sin(math.pi / 10), 0.75),
+    ("henonheiles", [], 1.0, 0.0, 1 + 205 / 42 - 13 / 3),
+    ("henonheiles", [], -1.0, 0.0, 1 + 205 / 42 + 13 / 3),
+    ("gaussian", [], 1.0, 0.0, math.exp(-0.5)),
+    ("gaussian", [3.0, 2.0, 1.0, 1.0], 1.0, 3.0, 3.0 * math.exp(-0.5)),
+    ("quartic", [0.5], 1.0, 0.0, 0.125 + 0.5 / 8),
+    ("squareoscillator", [2.0], 1.0, -1.0, 1.0),
+    ("power", [], 1.0, 0.0, 0.5),
+    ("power", [2.0, 3.0], 0.0, 2.0, 6.0),
+    ("ring", [], 3.0, 0.0, 0.0),
+    ("ring", [], 0.0, 4.0, 0.5),
+    ("ring", [3.0, 1.0, 2.0, 5.0, 1.0], 3.0, 0.0, 5.0),
+    ("cosh", [], 1.0, 0.0, math.cosh(1.0) - 1),
+    ("softstadium", [], 0.0, 0.0, 100 / (1 + math.exp(10))),
+    ("softstadium", [], 2.0, 0.0, 50.0),
+    ("softstadium", [], 0.5, 1.0, 50.0),
+    ("powerstadium", [], 0.3, 1.0, 0.5),
+    ("powerstadium", [], -2.0, 0.0, 0.5),
+    ("powerstadium", [2.0], 0.0, 0.5, 0.125),
+])
+def test_oracle_potential_values(name, params, x, y, want):
+    got = float(P.value(name, P.resolve(name, params), np.array(x), np.array(y)))
+    assert abs(got - want) <= 1e-14 * max(1.0, abs(want))
+
+
+KINDS = {n: i for i, n in enumerate(api.POTENTIAL_TYPES)}
+
+
+@pytest.mark.parametrize("name,alias", [("henonheiles", "henon"), ("gaussian", "gaussianblob"),
+                                        ("quartic", "quarticoscillator"), ("power", "poweroscillator"),
+                                        ("ring", "ringpotential"), ("cosh", "coshpotential")])
+def test_type_aliases(name, alias):
+    assert api.potential_type_params(alias)[0] == KINDS[name]
+
+
+def test_type_params_counts_and_defaults():
+    """Every accepted parameter count gives the oracle's resolved values (stadium
+    lengths are stored halved, as the reference does); other counts are rejected."""
+    for name, (defaults, counts) in P.TYPES.items():
+        for n in range(0, 7):
+            args = [0.5 + 0.25 * i for i in range(n)]
+            if n in counts:
+                kind, got = api.potential_type_params(name, args)
+                assert kind == KINDS[name]
+                want = P.resolve(name, args)
+                if name in ("softstadium", "powerstadium"):
+                    want = [want[0], want[1] / 2] + want[2:]
+                assert np.array_equal(got[:len(want)], np.array(want, dtype=float)), (name, n)
+            else:
+                with pytest.raises(api.ITPError):
+                    api.potential_type_params(name, args)
+
+
+@pytest.mark.parametrize("name,params", [("harmonic", [-1.0]), ("elliptic", [1.0, -2.0]), ("gaussian", [-1.0, 1.0]),
+                                         ("gaussian", [1.0, 0.0]), ("quartic", [-0.1]), ("power", [2.0, -1.0]),
+                                         ("ring", [-1.0, 1.0, 2.0]), ("ring", [1.0, 0.0, 2.0]), ("cosh", [0.0, 1.0]),
+                                         ("cosh", [1.0, -1.0]), ("softstadium", [0.0, 2.0, 1.0]),
+                                         ("softstadium", [1.0, 0.0, 1.0]), ("softstadium", [1.0, 2.0, -1.0]),
+                                         ("softstadium", [1.0, 2.0, 1.0, 0.0, 1.0]),
+                                         ("softstadium", [1.0, 2.0, 1.0, 1.0, 0.0]), ("powerstadium", [0.0, 2.0, 4.0]),
+                                         ("powerstadium", [1.0, -2.0, 4.0]), ("nosuchpotential", [])])
+def test_type_params_rejected(name, params):
+    with pytest.raises(api.ITPError):
+        api.potential_type_params(name, params)
+
+
+GPU_CASES = [("zero", []), ("harmonic", [1.7, 0.3, -0.4]), ("elliptic", []), ("prettyhardsquare", [5.0]),
+             ("softpentagon", []), ("henonheiles", []), ("gaussian", [2.0, 0.7, 0.5, -0.25]), ("quartic", [0.02]),
+             ("squareoscillator", [3.5]), ("power", [2.5, 1.5]), ("ring", [2.0, 0.5, 3.0, 1.5, 0.4]),
+             ("cosh", [2.0, 1.5]), ("softstadium", [1.0, 2.0, 50.0, 1.0, 10.0]), ("powerstadium", [1.2, 1.6, 6.0])]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,params", GPU_CASES)
+def test_device_potential_types(ctx, name, params):
+    print(f"[gpu] potential {name}", flush=True)
+    sx, sy, dx = 64, 40, 0.13
+    g = I.Grid(ctx, sx, sy, dx, I.PERIODIC)
+    pot = I.Potential.of_type(g, name, params)
+    V = pot.values()
+    Vo = P.potential(name, params, sx, sy, dx)
+    assert np.abs(V - Vo).max() <= 1e-13 * max(1.0, np.abs(Vo).max())
+
+
+@pytest.mark.gpu
+def test_device_impurities(ctx):
+    print("[gpu] impurities", flush=True)
+    sx, sy, dx = 64, 40, 0.2
+    g = I.Grid(ctx, sx, sy, dx, I.DIRICHLET)
+    rng = np.random.default_rng(3)
+    base = P.potential("harmonic", [], sx, sy, dx)
+    gauss = np.column_stack([rng.uniform(-6, 6, 25), rng.uniform(-4, 4, 25), rng.uniform(-2, 2, 25),
+                             rng.uniform(0.3, 0.8, 25)])
+    pts = np.column_stack([rng.uniform(-6, 6, 7), rng.uniform(-4, 4, 7)])
+    delta = np.column_stack([rng.uniform(-6, 6, 5), rng.uniform(-4, 4, 5), rng.uniform(-1, 1, 5)])
+    pot = I.Potential.of_type(g, "harmonic")
+    pot.add_impurities(I.api.IMP_GAUSSIAN, gauss)
+    pot.add_impurities(I.api.IMP_COULOMB, pts, [1.5, 0.2])
+    pot.add_impurities(I.api.IMP_HEMISPHERE, pts, [0.8, 1.1])
+    pot.add_impurities(I.api.IMP_DELTA, delta)
+    V = P.add_impurities(base, 0, gauss, None, dx)
+    V = P.add_impurities(V, 1, pts, [1.5, 0.2], dx)
+    V = P.add_impurities(V, 2, pts, [0.8, 1.1], dx)
+    V = P.add_impurities(V, 3, delta, None, dx)
+    assert np.abs(pot.values() - V).max() <= 1e-13 * np.abs(V).max()
+    # the zero potential with impurities is no longer null: a split sees V
+    z = I.Potential.of_type(g, "zero")
+    z.add_impurities(I.api.IMP_GAUSSIAN, gauss[:3])
+    assert I.MultiProductSplit(z, 2, 0.05, 0.0).halforder == 2
+    assert I.MultiProductSplit(I.Potential.of_type(g, "zero"), 2, 0.05, 0.0).halforder == 1
