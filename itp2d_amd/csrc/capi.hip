@@ -1088,7 +1088,9 @@ int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out
     s->np = s->nc;  // the eigensolver works on the zero-padded nc x nc matrix
     const int nb = s->nc / 64;
     const int npairs = nb * (nb + 1) / 2;
-    long long nchunk = (2048 + npairs - 1) / npairs;
+    // about 2048 workgroups, rounded down so the last wave of resident
+    // workgroups is not nearly empty
+    long long nchunk = std::max(1, 2048 / npairs);
     nchunk = std::min<long long>(nchunk, std::max<long long>(1, (long long)M / 256));
     const size_t per_chunk = (size_t)s->nc * s->nc * 16;
     nchunk = std::max<long long>(1, std::min<long long>(nchunk, (long long)((512ull << 20) / per_chunk)));

@@ -294,7 +294,11 @@ __global__ __launch_bounds__(256) void gram_kernel(const cplx<R>* psi, const cpl
       for (int v = 0; v < 4; v++) { accr[a][b][v] = 0; acci[a][b][v] = 0; }
   const long long k0 = chunk * kchunk;
   const long long k1 = (k0 + kchunk < M) ? k0 + kchunk : M;
-  for (long long t0 = k0; t0 < k1; t0 += 16) {
+  // thread's staging slots e = tid + 256 r of the 64 x 16 tiles; the next
+  // step's values are fetched into registers while this step's products run
+  cplx<double> pa[4], pb[4];
+  auto fetch = [&](long long t0) {
+#pragma unroll
     for (int r = 0; r < 4; r++) {
       const int e = tid + 256 * r;
       const int s = e >> 4, k = e & 15;
@@ -305,10 +309,20 @@ __global__ __launch_bounds__(256) void gram_kernel(const cplx<R>* psi, const cpl
         if (ia < N) { const cplx<R> x = psi[(long long)ia * M + t]; va = mk<double>(double(x.x), double(x.y)); }
         if (ib < N) { const cplx<R> x = phi[(long long)ib * M + t]; vb = mk<double>(double(x.x), double(x.y)); }
       }
-      As[s][k] = va;
-      Bs[s][k] = vb;
+      pa[r] = va;
+      pb[r] = vb;
+    }
+  };
+  if (k0 < k1) fetch(k0);
+  for (long long t0 = k0; t0 < k1; t0 += 16) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int e = tid + 256 * r;
+      As[e >> 4][e & 15] = pa[r];
+      Bs[e >> 4][e & 15] = pb[r];
     }
     __syncthreads();
+    if (t0 + 16 < k1) fetch(t0 + 16);
 #pragma unroll
     for (int kk = 0; kk < 4; kk++) {
       cplx<double> a[2], b[2];
