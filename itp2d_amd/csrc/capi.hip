@@ -1088,9 +1088,19 @@ int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out
     s->np = s->nc;  // the eigensolver works on the zero-padded nc x nc matrix
     const int nb = s->nc / 64;
     const int npairs = nb * (nb + 1) / 2;
-    // about 2048 workgroups, rounded down so the last wave of resident
-    // workgroups is not nearly empty
-    long long nchunk = std::max(1, 2048 / npairs);
+    // split-K chunks: 2048 to 4096 workgroups, the count for which the last
+    // wave of resident workgroups (two per CU) is fullest
+    long long nchunk = 1;
+    {
+      const long long Q = 2LL * std::max(1, c->num_cus);
+      const long long lo = std::max(1, 2048 / npairs), hi = std::max<long long>(lo, (4096 + npairs - 1) / npairs);
+      double best = -1;
+      for (long long n = lo; n <= hi; n++) {
+        const long long W = n * npairs;
+        const double eff = (double)W / (double)(((W + Q - 1) / Q) * Q);
+        if (eff > best + 1e-12) { best = eff; nchunk = n; }
+      }
+    }
     nchunk = std::min<long long>(nchunk, std::max<long long>(1, (long long)M / 256));
     const size_t per_chunk = (size_t)s->nc * s->nc * 16;
     nchunk = std::max<long long>(1, std::min<long long>(nchunk, (long long)((512ull << 20) / per_chunk)));

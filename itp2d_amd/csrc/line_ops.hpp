@@ -55,19 +55,26 @@ template <class R> ITP_HD cplx<R> mk(R a, R b) { cplx<R> c; c.x = a; c.y = b; re
 // Workgroups are dealt to the 8 XCDs of the device round-robin in launch
 // order. For kernels whose tiles share input slabs (the rotation's grid-point
 // slabs, the overlap's grid chunks), launch-order block b maps to
-// (inner, outer) so that every XCD gets whole slabs (outer) and runs all
-// tiles of a slab (inner) back to back, reusing the slab from its own L2.
-// The grid is ceil(nouter / 8) * 8 * ninner blocks; false for the padding.
+// (inner, outer) so that, within each full group of 8 slabs, every XCD gets
+// one whole slab (outer) and runs all tiles of it (inner) back to back,
+// reusing the slab from its own L2. The slabs past the last full group are
+// spread over all XCDs in plain order, so a few slabs never leave XCDs idle.
+// The grid is exactly ninner * nouter blocks.
 constexpr int kXcdCount = 8;
 ITP_HD bool xcd_tile(long long b, int ninner, long long nouter, int& inner, long long& outer) {
-  const long long xcd = b % kXcdCount, slot = b / kXcdCount;
-  inner = (int)(slot % ninner);
-  outer = (slot / ninner) * kXcdCount + xcd;
+  const long long full = nouter / kXcdCount * kXcdCount * ninner;
+  if (b < full) {
+    const long long xcd = b % kXcdCount, slot = b / kXcdCount;
+    inner = (int)(slot % ninner);
+    outer = (slot / ninner) * kXcdCount + xcd;
+  } else {
+    const long long r = b - full;
+    inner = (int)(r % ninner);
+    outer = nouter / kXcdCount * kXcdCount + r / ninner;
+  }
   return outer < nouter;
 }
-inline long long xcd_grid(int ninner, long long nouter) {
-  return (nouter + kXcdCount - 1) / kXcdCount * kXcdCount * ninner;
-}
+inline long long xcd_grid(int ninner, long long nouter) { return nouter * ninner; }
 template <class R> ITP_HD cplx<R> operator+(cplx<R> a, cplx<R> b) { return mk<R>(a.x + b.x, a.y + b.y); }
 template <class R> ITP_HD cplx<R> operator-(cplx<R> a, cplx<R> b) { return mk<R>(a.x - b.x, a.y - b.y); }
 template <class R> ITP_HD cplx<R> operator-(cplx<R> a) { return mk<R>(-a.x, -a.y); }
