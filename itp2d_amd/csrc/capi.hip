@@ -92,6 +92,7 @@ struct itp_stateset {
   int* cl_members = nullptr;
   int* cl_offsets = nullptr;
   cplx<double>* Vsave = nullptr;  // eigenvector estimate before a cluster start, restored if its refinement fails
+  cplx<double>* oa_part = nullptr;  // split-K partial products of the refinement (small matrices only)
   cplx<double>* C = nullptr;
   double* lam = nullptr;
   int* perm = nullptr;
@@ -480,7 +481,8 @@ int cluster_route(itp_stateset* s, cplx<double>* Acur, const cplx<double>* A0, b
   // without clusters, a start that is the identity (before any sweep) needs
   // no products in the refinement's first iteration
   if ((e = launch_oa(A0, s->V[0], s->V[1], s->oa_work, s->oa_lam, s->oa_slots, s->oa_state, s->info, s->np, N,
-                     kOAMaxIter, Acur, c->stream, ident_start && ncl == 0 ? s->Vinit : nullptr)) != hipSuccess)
+                     kOAMaxIter, Acur, c->stream, ident_start && ncl == 0 ? s->Vinit : nullptr, s->oa_part)) !=
+      hipSuccess)
     return fail(e, "refinement");
   if ((e = launch_eig_finish(Acur, s->V[0], s->np, N, s->lam, s->perm, s->C, s->nc, scale ? 1 : 0, c->stream)) !=
       hipSuccess)
@@ -551,7 +553,7 @@ int eigen_stage(itp_stateset* s, size_t* bad_n, double* bad_lambda, bool scale, 
     }
     if (rd.info[1] == 2.0) {  // handed off: refinement from the Jacobi eigenvectors
       HIPCK(c, launch_oa(s->A[1], s->V[0], s->V[1], s->oa_work, s->oa_lam, s->oa_slots, s->oa_state, s->info, s->np,
-                         N, kOAMaxIter, s->A[0], c->stream));
+                         N, kOAMaxIter, s->A[0], c->stream, nullptr, s->oa_part));
       HIPCK(c, launch_eig_finish(s->A[0], s->V[0], s->np, N, s->lam, s->perm, s->C, s->nc, scale ? 1 : 0, c->stream));
       if ((rc = read_eig(s, rd, lam, true))) return rc;
       if (rd.state[0] == 1) {
@@ -1134,6 +1136,7 @@ int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out
       if ((rc = dalloc(c, (void**)&s->cl_members, (size_t)s->np * sizeof(int)))) return rc;
       if ((rc = dalloc(c, (void**)&s->cl_offsets, ((size_t)s->np + 1) * sizeof(int)))) return rc;
       if ((rc = dalloc(c, (void**)&s->Vsave, np2))) return rc;
+      if (s->np < 1024 && (rc = dalloc(c, (void**)&s->oa_part, 4 * np2))) return rc;
     }
     {
       std::vector<cplx<double>> I((size_t)s->np * s->np, mk<double>(0.0, 0.0));
@@ -1161,7 +1164,7 @@ void itp_stateset_destroy(itp_stateset* s) {
   (void)hipDeviceSynchronize();  // no parent access: parents may already be gone
   void* ptrs[] = {s->buf[0], s->buf[1], s->part, s->A[0], s->A[1], s->V[0], s->V[1], s->Vinit, s->bjQ, s->bjD,
                   s->bjslots, s->bjident, s->oa_work, s->oa_lam, s->oa_slots, s->oa_state, s->cl_hdr, s->cl_pairs,
-                  s->cl_members, s->cl_offsets, s->Vsave, s->C, s->C32,
+                  s->cl_members, s->cl_offsets, s->Vsave, s->oa_part, s->C, s->C32,
                   s->lam, s->perm, s->info, s->bad, s->epart, s->dE, s->dmean};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);

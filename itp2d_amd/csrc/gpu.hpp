@@ -49,10 +49,12 @@ hipError_t launch_block_jacobi(cplx<double>* A, cplx<double>* V, cplx<double>* Q
 // Refinement of the eigenvector estimate X0 of A0: X0 is overwritten only on
 // convergence (iterates go to X1 and to work's fifth np x np slot); work holds
 // 5 np^2 complex values. x0_identity: non-null (an identity matrix) when X0
-// is the identity, which spares the first iteration's products
+// is the identity, which spares the first iteration's products. part:
+// 4 np^2 complex values for split-K products of small matrices (nullptr:
+// no split)
 hipError_t launch_oa(const cplx<double>* A0, cplx<double>* X0, cplx<double>* X1, cplx<double>* work, double* lam,
                      double* slots, int* state, const double* info, int np, int N, int max_iter, cplx<double>* Aout,
-                     hipStream_t st, const cplx<double>* x0_identity = nullptr);
+                     hipStream_t st, const cplx<double>* x0_identity = nullptr, cplx<double>* part = nullptr);
 // Strong pairs (i < j < N, |a_ij| > tau |a_ii - a_jj|) of the np x np matrix
 // A: hdr[0] counts them (the first cap are listed in pairs as (i, j)),
 // hdr[1] holds the bits of the largest ratio

@@ -534,7 +534,7 @@ hipError_t launch_block_jacobi(cplx<double>* A, cplx<double>* V, cplx<double>* Q
 // one 16 x 16 quarter per wavefront.
 __global__ __launch_bounds__(256) void zgemm_kernel(const cplx<double>* A, int ah, const cplx<double>* B,
                                                     cplx<double>* C, const cplx<double>* Cadd, int n,
-                                                    const int* state, int herm) {
+                                                    const int* state, int herm, cplx<double>* part) {
   if (state && state[0] != 0) return;
   // herm: C is Hermitian (A^H B with B = A, or with B = M A for Hermitian M):
   // the upper tiles only, each written with its mirror
@@ -545,22 +545,31 @@ __global__ __launch_bounds__(256) void zgemm_kernel(const cplx<double>* A, int a
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wi = w >> 1, wj = w & 1;
   const int li = lane & 15, lk = lane >> 4;
+  // split-K: slice blockIdx.z of gridDim.z equal K ranges (multiples of 16),
+  // its partial product to part[z] (summed by zgemm_reduce_kernel)
+  const int kn = n / (int)gridDim.z, kb = (int)blockIdx.z * kn, ke = kb + kn;
   d4 cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
-  for (int k0 = 0; k0 < n; k0 += 16) {
+  cplx<double> pa[2], pb[2];
+  auto fetch = [&](int k0) {
+#pragma unroll
     for (int q = 0; q < 2; q++) {
       const int e = tid + 256 * q;
-      // A tile: 32 rows of op(A) x 16 k
-      {
-        const int r = e >> 4, kk = e & 15;
-        As[r][kk] = ah ? conj(A[(long long)(k0 + kk) * n + i0 + r]) : A[(long long)(i0 + r) * n + k0 + kk];
-      }
-      // B tile: 16 k x 32 columns
-      {
-        const int kk = e >> 5, cidx = e & 31;
-        Bs[kk][cidx] = B[(long long)(k0 + kk) * n + j0 + cidx];
-      }
+      const int r = e >> 4, kk = e & 15;  // A tile: 32 rows of op(A) x 16 k
+      pa[q] = ah ? conj(A[(long long)(k0 + kk) * n + i0 + r]) : A[(long long)(i0 + r) * n + k0 + kk];
+      const int kb2 = e >> 5, cidx = e & 31;  // B tile: 16 k x 32 columns
+      pb[q] = B[(long long)(k0 + kb2) * n + j0 + cidx];
+    }
+  };
+  fetch(kb);
+  for (int k0 = kb; k0 < ke; k0 += 16) {
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int e = tid + 256 * q;
+      As[e >> 4][e & 15] = pa[q];
+      Bs[e >> 5][e & 31] = pb[q];
     }
     __syncthreads();
+    if (k0 + 16 < ke) fetch(k0 + 16);
 #pragma unroll
     for (int ks = 0; ks < 4; ks++) {
       const cplx<double> av = As[wi * 16 + li][ks * 4 + lk];
@@ -572,13 +581,37 @@ __global__ __launch_bounds__(256) void zgemm_kernel(const cplx<double>* A, int a
     }
     __syncthreads();
   }
+  cplx<double>* out = gridDim.z > 1 ? part + (long long)blockIdx.z * n * n : C;
   for (int v = 0; v < 4; v++) {
     const long long o = (long long)(i0 + wi * 16 + lk + 4 * v) * n + j0 + wj * 16 + li;
     cplx<double> r = mk<double>(cr[v], ci[v]);
+    if (gridDim.z > 1) {
+      out[o] = r;
+      continue;
+    }
     if (Cadd) r = Cadd[o] + r;
     C[o] = r;
     if (herm && blockIdx.y != blockIdx.x)
       C[(long long)(j0 + wj * 16 + li) * n + i0 + wi * 16 + lk + 4 * v] = conj(r);
+  }
+}
+
+// C = sum of the ksplit partial products (in slice order) [+ Cadd]; with herm
+// only the upper 32 x 32 tiles were computed and each is written with its
+// mirror.
+__global__ __launch_bounds__(256) void zgemm_reduce_kernel(const cplx<double>* part, int ksplit, cplx<double>* C,
+                                                           const cplx<double>* Cadd, int n, const int* state,
+                                                           int herm) {
+  if (state && state[0] != 0) return;
+  const long long n2 = (long long)n * n;
+  for (long long e = blockIdx.x * 256ll + threadIdx.x; e < n2; e += (long long)gridDim.x * 256) {
+    const int i = (int)(e / n), j = (int)(e - (long long)i * n);
+    if (herm && (i >> 5) > (j >> 5)) continue;
+    cplx<double> r = part[e];
+    for (int z = 1; z < ksplit; z++) r = r + part[(long long)z * n2 + e];
+    if (Cadd) r = Cadd[e] + r;
+    C[e] = r;
+    if (herm && (i >> 5) != (j >> 5)) C[(long long)j * n + i] = conj(r);
   }
 }
 
@@ -655,7 +688,7 @@ __global__ void oa_finalize_kernel(const int* state, const cplx<double>* X1, con
 
 hipError_t launch_oa(const cplx<double>* A0, cplx<double>* X0, cplx<double>* X1, cplx<double>* work, double* lam,
                      double* slots, int* state, const double* info, int np, int N, int max_iter, cplx<double>* Aout,
-                     hipStream_t st, const cplx<double>* x0_identity) {
+                     hipStream_t st, const cplx<double>* x0_identity, cplx<double>* part) {
   if (np % 32) return hipErrorInvalidValue;
   const long long n2 = (long long)np * np;
   cplx<double>* Y = work;
@@ -666,7 +699,19 @@ hipError_t launch_oa(const cplx<double>* A0, cplx<double>* X0, cplx<double>* X1,
   hipLaunchKernelGGL(oa_init_kernel, dim3(1), dim3(1), 0, st, info, state);
   hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<unsigned long long*>(slots),
                      (long long)max_iter);
-  const dim3 gg(np / 32, np / 32);
+  // split-K for small matrices (few 32 x 32 tiles): 4 slices when the tiles
+  // alone cannot fill the device and part is given
+  int ks = 1;
+  if (part && (np / 32) * (np / 32) < 1024) ks = np % 64 == 0 ? (np % 128 == 0 ? 4 : 2) : 1;
+  const dim3 gg(np / 32, np / 32, ks);
+  const int rb = (int)std::min<long long>(((long long)np * np + 255) / 256, 2048);
+  auto gemm = [&](const cplx<double>* a, int ah, const cplx<double>* b, cplx<double>* c, const cplx<double>* cadd,
+                  int herm) {
+    hipLaunchKernelGGL(zgemm_kernel, gg, dim3(256), 0, st, a, ah, b, c, cadd, np, state, herm, part);
+    if (ks > 1)
+      hipLaunchKernelGGL(zgemm_reduce_kernel, dim3(rb), dim3(256), 0, st, (const cplx<double>*)part, ks, c, cadd, np,
+                         state, herm);
+  };
   for (int k = 0; k < max_iter; k++) {
     // X0 is only read (iterates alternate between X1 and X2), so a failed
     // refinement leaves the starting estimate in place
@@ -678,13 +723,13 @@ hipError_t launch_oa(const cplx<double>* A0, cplx<double>* X0, cplx<double>* X1,
       hipLaunchKernelGGL(oa_update_kernel, dim3((np + 255) / 256, np), dim3(256), 0, st, A0, x0_identity, E, lam,
                          np, N, slot, state);
     } else {
-      hipLaunchKernelGGL(zgemm_kernel, gg, dim3(256), 0, st, A0, 0, X, Y, nullptr, np, state, 0);
-      hipLaunchKernelGGL(zgemm_kernel, gg, dim3(256), 0, st, X, 1, Y, S, nullptr, np, state, 1);
-      hipLaunchKernelGGL(zgemm_kernel, gg, dim3(256), 0, st, X, 1, X, G, nullptr, np, state, 1);
+      gemm(A0, 0, X, Y, nullptr, 0);
+      gemm(X, 1, Y, S, nullptr, 1);
+      gemm(X, 1, X, G, nullptr, 1);
       hipLaunchKernelGGL(oa_update_kernel, dim3((np + 255) / 256, np), dim3(256), 0, st, S, G, E, lam, np, N, slot,
                          state);
     }
-    hipLaunchKernelGGL(zgemm_kernel, gg, dim3(256), 0, st, X, 0, E, Xn, X, np, state, 0);
+    gemm(X, 0, E, Xn, X, 0);
     hipLaunchKernelGGL(oa_decide_kernel, dim3(1), dim3(1), 0, st, slot, k, state);
   }
   hipLaunchKernelGGL(oa_finalize_kernel, dim3(256), dim3(256), 0, st, state, X1, X2, X0, Aout, lam, np, N);
