@@ -98,7 +98,7 @@ ITP_HD long long bj_gidx(int b1, int b2, int i) {
 // element test decides the whole-matrix check and the skipping of
 // sub-problems, so no unconverged element can be left alone.
 ITP_HD double bj_ratio(cplx<double> aij, double aii, double ajj, double tol, double floor) {
-  const double ab = std::hypot(aij.x, aij.y);
+  const double ab = cabs_fast(aij);
   double den = tol * std::sqrt(std::fabs(aii * ajj));
   den = den > floor ? den : floor;
   return den > 0 ? ab / den : (ab > 0 ? 1e300 : 0.0);
@@ -132,7 +132,7 @@ ITP_HD void bj_reduce_max(Exec& ex, const BJLds& s) {
 // |a_ij| / |a_ii - a_jj|: small for every pair once the matrix is close
 // enough to diagonal for first-order (perturbative) eigenvector corrections.
 ITP_HD double bj_gap_ratio(cplx<double> aij, double aii, double ajj) {
-  const double ab = std::hypot(aij.x, aij.y);
+  const double ab = cabs_fast(aij);
   const double gap = std::fabs(aii - ajj);
   return gap > 0 ? ab / gap : 1e300;
 }

@@ -39,7 +39,7 @@ ITP_HD double cl_row_pairs(const cplx<double>* A, int np, int N, int i, int j0, 
   for (int j = j0; j < N; j += jstep) {
     if (j <= i) continue;
     const cplx<double> b = A[(long long)i * np + j];
-    const double ab = std::hypot(b.x, b.y);
+    const double ab = cabs_fast(b);
     if (ab == 0) continue;
     const double gap = std::fabs(aii - A[(long long)j * np + j].x);
     const double rho = gap > 0 ? ab / gap : 1e300;

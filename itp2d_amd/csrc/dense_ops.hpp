@@ -23,15 +23,21 @@ ITP_HD int rr_next(int pos, int m) {
   return (k + 1 < m) ? 2 * (k + 1) : 2 * (m - 1) + 1;
 }
 
+// |z| without the overflow scaling of hypot: the eigensolver's matrices have
+// entries far inside the FP64 range, and this sits on the critical path of
+// every rotation round.
+ITP_HD double cabs_fast(cplx<double> z) { return std::sqrt(z.x * z.x + z.y * z.y); }
+
 // Rotation J = [[c, s], [-conj(s), c]] with J^H [[a, b], [conj(b), d]] J diagonal.
 // off = |b| / sqrt(|a d|), the relative size of the element being removed.
+// For |tau| so large that tau^2 overflows, t = 0: no rotation, the limit.
 ITP_HD void jacobi_rotation(double a, double d, cplx<double> b, double& c, cplx<double>& s, double& off) {
-  const double ab = std::hypot(b.x, b.y);
+  const double ab = cabs_fast(b);
   const double scale = std::sqrt(std::fabs(a * d));
   off = scale > 0 ? ab / scale : ab;
   if (ab == 0 || off < 1e-17) { c = 1; s = mk<double>(0.0, 0.0); return; }
   const double tau = (d - a) / (2 * ab);
-  const double t = (tau >= 0 ? 1.0 : -1.0) / (std::fabs(tau) + std::hypot(1.0, tau));
+  const double t = (tau >= 0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
   c = 1 / std::sqrt(1 + t * t);
   const double sc = t * c / ab;
   s = mk<double>(b.x * sc, b.y * sc);
