@@ -1023,17 +1023,21 @@ int itp_potential_add_impurities(itp_potential* p, int kind, const double* data,
       p->V.assign(gi.M, 0.0);
       HIPCK(c, hipMemcpy(p->dV, p->V.data(), gi.M * sizeof(double), hipMemcpyHostToDevice));
     }
-    if (count) p->null = false;
     if (kind == kImpDelta) {
       // one grid point each (DeltaImpurities::add_noise, noise.cpp:80-88): on the host copy
       HIPCK(c, hipMemcpyAsync(p->V.data(), p->dV, gi.M * sizeof(double), hipMemcpyDeviceToHost, c->stream));
       HIPCK(c, hipStreamSynchronize(c->stream));
+      // all positions are checked before any value changes, so a rejected
+      // call leaves the host and device copies equal
+      std::vector<long long> at(count);
       for (size_t n = 0; n < count; n++) {
         const long long x = grid_nearest(data[3 * n], gi.sx, gi.dx), y = grid_nearest(data[3 * n + 1], gi.sy, gi.dx);
         if (x < 0 || y < 0) return set_err(c, ITP_E_INVALID, "delta impurity outside the grid");
-        p->V[(size_t)y * gi.sx + x] += data[3 * n + 2];
+        at[n] = y * (long long)gi.sx + x;
       }
+      for (size_t n = 0; n < count; n++) p->V[at[n]] += data[3 * n + 2];
       HIPCK(c, hipMemcpy(p->dV, p->V.data(), gi.M * sizeof(double), hipMemcpyHostToDevice));
+      if (count) p->null = false;
       return ITP_OK;
     }
     const int stride = kind == kImpGaussian ? 4 : 2;
@@ -1051,6 +1055,7 @@ int itp_potential_add_impurities(itp_potential* p, int kind, const double* data,
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(dd);
     if (e != hipSuccess) return set_err(c, ITP_E_HIP, std::string("impurities: ") + hipGetErrorString(e));
+    if (count) p->null = false;
     return ITP_OK;
   });
 }

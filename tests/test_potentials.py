@@ -128,6 +128,11 @@ def test_device_impurities(ctx):
     V = P.add_impurities(V, 2, pts, [0.8, 1.1], dx)
     V = P.add_impurities(V, 3, delta, None, dx)
     assert np.abs(pot.values() - V).max() <= 1e-13 * np.abs(V).max()
+    # a delta impurity off the grid is rejected and changes nothing
+    before = pot.values()
+    with pytest.raises(I.ITPError):
+        pot.add_impurities(I.api.IMP_DELTA, np.array([[0.0, 0.0, 1.0], [100.0, 0.0, 1.0]]))
+    assert np.array_equal(pot.values(), before)
     # the zero potential with impurities is no longer null: a split sees V
     z = I.Potential.of_type(g, "zero")
     z.add_impurities(I.api.IMP_GAUSSIAN, gauss[:3])
