@@ -242,7 +242,7 @@ int itp_ctx_set_eig_handoff(itp_ctx* ctx, double ratio);
 /* Calls finished by the refinement iteration and calls that fell back to Jacobi. */
 int itp_ctx_oa_stats(itp_ctx* ctx, long long* refined, long long* fallbacks);
 /* Cluster start of mode 0 (tried before any Jacobi sweep): the pairs with
- * |a_ij| > tau |a_ii - a_jj| are joined into clusters of at most 48 indices,
+ * |a_ij| > tau |a_ii - a_jj| are joined into clusters of at most 64 indices,
  * each diagonalised exactly, and the refinement iteration starts from those
  * rotations; with more than 8192 such pairs, a larger cluster or no
  * convergence the block-Jacobi route runs as before. tau in [0, 0.5); 0 turns

@@ -104,7 +104,7 @@ def test_block_jacobi_emulated(n):
     assert np.abs(V.conj().T @ V - np.eye(n)).max() < 1e-14 * n
 
 
-@pytest.mark.parametrize("k", [2, 7, 12, 48])
+@pytest.mark.parametrize("k", [2, 7, 12, 48, 63, 64])
 def test_cluster_solve_emulated(k):
     """Cluster start (cluster_eig.hpp): the exact eigenvectors of a principal
     sub-matrix on scattered indices, embedded in the identity, diagonalise that
