@@ -617,43 +617,61 @@ __global__ __launch_bounds__(256) void zgemm_reduce_kernel(const cplx<double>* p
 
 // E from S and G; indices >= N (padding) stay fixed. lam[i] = S_ii / G_ii.
 // The largest |E_ij| (i != j) goes to *slot (atomic max of the bit pattern).
-__global__ void oa_update_kernel(const cplx<double>* S, const cplx<double>* G, cplx<double>* E, double* lam,
-                                 int np, int N, unsigned long long* slot, const int* state) {
+constexpr int kOARows = 8;  // rows of E per workgroup of oa_update_kernel
+
+__global__ __launch_bounds__(256) void oa_update_kernel(const cplx<double>* S, const cplx<double>* G, cplx<double>* E,
+                                                        double* lam, int np, int N, unsigned long long* slot,
+                                                        const int* state) {
   if (state[0] != 0) return;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y;
-  if (j >= np) return;
-  const long long o = (long long)i * np + j;
-  cplx<double> e = mk<double>(0.0, 0.0);
   double m = 0;
-  if (i < N && j < N) {
-    const double li = S[(long long)i * np + i].x / G[(long long)i * np + i].x;
-    if (i == j) {
-      e = mk<double>(0.5 * (1.0 - G[o].x), 0.0);
-      lam[i] = li;
-    } else {
-      const double lj = S[(long long)j * np + j].x / G[(long long)j * np + j].x;
-      // S and G are Hermitian; their two computed triangles differ by rounding.
-      // Using the Hermitian part keeps that difference out of E's Hermitian
-      // part (which must be exactly R/2), where divided by a small gap it
-      // would stall the iteration at eps |A| / gap.
-      const long long ot = (long long)j * np + i;
-      const cplx<double> sij = (S[o] + conj(S[ot])) * 0.5;
-      const cplx<double> gij = (G[o] + conj(G[ot])) * 0.5;
-      const cplx<double> num = sij - gij * lj;  // S_ij + lambda_j R_ij with R_ij = -G_ij
-      const double gap = lj - li;
-      e = mk<double>(num.x / gap, num.y / gap);
-      m = std::hypot(e.x, e.y);
-      if (!(m <= 1e300)) m = 1e300;  // a zero gap or a NaN fails the iteration
+  for (int r = 0; r < kOARows; r++) {
+    const int i = blockIdx.y * kOARows + r;
+    if (i >= np || j >= np) continue;
+    const long long o = (long long)i * np + j;
+    cplx<double> e = mk<double>(0.0, 0.0);
+    if (i < N && j < N) {
+      const double li = S[(long long)i * np + i].x / G[(long long)i * np + i].x;
+      if (i == j) {
+        e = mk<double>(0.5 * (1.0 - G[o].x), 0.0);
+        lam[i] = li;
+      } else {
+        const double lj = S[(long long)j * np + j].x / G[(long long)j * np + j].x;
+        // S and G are Hermitian, but inside the diagonal 32 x 32 tiles of the
+        // products both triangles are computed and differ by rounding; their
+        // Hermitian part keeps that difference out of E's Hermitian part
+        // (which must be exactly R/2), where divided by a small gap it would
+        // stall the iteration at eps |A| / gap. Elsewhere the lower triangle
+        // is the exact mirror, and the Hermitian part is the element itself.
+        cplx<double> sij = S[o], gij = G[o];
+        if ((i >> 5) == (j >> 5)) {
+          const long long ot = (long long)j * np + i;
+          sij = (sij + conj(S[ot])) * 0.5;
+          gij = (gij + conj(G[ot])) * 0.5;
+        }
+        const cplx<double> num = sij - gij * lj;  // S_ij + lambda_j R_ij with R_ij = -G_ij
+        const double gap = lj - li;
+        e = mk<double>(num.x / gap, num.y / gap);
+        const double a = cabs_fast(e);
+        const double mm = (a <= 1e300) ? a : 1e300;  // a zero gap or a NaN fails the iteration
+        m = mm > m ? mm : m;
+      }
     }
+    E[o] = e;
   }
-  E[o] = e;
-  // wave maximum, then one atomic per wavefront
+  // workgroup maximum, then one atomic per workgroup
   for (int off = 32; off >= 1; off >>= 1) {
     const double t = __shfl_xor(m, off);
     m = t > m ? t : m;
   }
-  if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(slot, dbits(m));
+  __shared__ double wm[4];
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = wm[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) b = wm[w] > b ? wm[w] : b;
+    if (b > 0) atomicMax(slot, dbits(b));
+  }
 }
 
 // After iteration k: done once the correction fell below 1e-8 (the update
@@ -704,6 +722,7 @@ hipError_t launch_oa(const cplx<double>* A0, cplx<double>* X0, cplx<double>* X1,
   int ks = 1;
   if (part && (np / 32) * (np / 32) < 1024) ks = np % 64 == 0 ? (np % 128 == 0 ? 4 : 2) : 1;
   const dim3 gg(np / 32, np / 32, ks);
+  const dim3 ug((np + 255) / 256, (np + kOARows - 1) / kOARows);
   const int rb = (int)std::min<long long>(((long long)np * np + 255) / 256, 2048);
   auto gemm = [&](const cplx<double>* a, int ah, const cplx<double>* b, cplx<double>* c, const cplx<double>* cadd,
                   int herm) {
@@ -720,14 +739,12 @@ hipError_t launch_oa(const cplx<double>* A0, cplx<double>* X0, cplx<double>* X1,
     unsigned long long* slot = reinterpret_cast<unsigned long long*>(slots) + k;
     if (k == 0 && x0_identity) {
       // X = I: S = A0 and G = I, no products needed
-      hipLaunchKernelGGL(oa_update_kernel, dim3((np + 255) / 256, np), dim3(256), 0, st, A0, x0_identity, E, lam,
-                         np, N, slot, state);
+      hipLaunchKernelGGL(oa_update_kernel, ug, dim3(256), 0, st, A0, x0_identity, E, lam, np, N, slot, state);
     } else {
       gemm(A0, 0, X, Y, nullptr, 0);
       gemm(X, 1, Y, S, nullptr, 1);
       gemm(X, 1, X, G, nullptr, 1);
-      hipLaunchKernelGGL(oa_update_kernel, dim3((np + 255) / 256, np), dim3(256), 0, st, S, G, E, lam, np, N, slot,
-                         state);
+      hipLaunchKernelGGL(oa_update_kernel, ug, dim3(256), 0, st, S, G, E, lam, np, N, slot, state);
     }
     gemm(X, 0, E, Xn, X, 0);
     hipLaunchKernelGGL(oa_decide_kernel, dim3(1), dim3(1), 0, st, slot, k, state);
