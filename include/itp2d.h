@@ -227,6 +227,9 @@ int itp_ctx_eig_stats(itp_ctx* ctx, long long* calls, long long* sweeps);
 /* Upper bound on the element sweeps spent on each block sub-problem per round
  * of the block Jacobi eigensolver (default 1). */
 int itp_ctx_set_eig_inner(itp_ctx* ctx, int inner);
+/* Rotation of FP32 sets (mixed path): 1 (default) split-BF16x3 products on the
+ * bf16 matrix cores, 0 FP32 matrix cores; both accumulate in FP32. */
+int itp_ctx_set_fp32_rotation(itp_ctx* ctx, int kind);
 /* Workgroups of the cooperative eigensolver launch (default 0: one per 16
  * super-tile updates of a round, at least 64, at most what is co-resident). */
 int itp_ctx_set_eig_grid(itp_ctx* ctx, int workgroups);

@@ -88,6 +88,7 @@ def lib():
             "itp_ctx_set_eig_handoff": [_vp, C.c_double],
             "itp_ctx_oa_stats": [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
             "itp_ctx_set_eig_cluster": [_vp, C.c_double],
+            "itp_ctx_set_fp32_rotation": [_vp, C.c_int],
             "itp_ctx_cluster_stats": [_vp, C.POINTER(C.c_longlong)],
             "itp_grid_create": [_vp, _sz, _sz, C.c_double, C.c_int, C.POINTER(_vp)],
             "itp_grid_destroy": [_vp],
@@ -249,6 +250,10 @@ class Context:
         a, b = C.c_longlong(0), C.c_longlong(0)
         lib().itp_ctx_oa_stats(self.h, C.byref(a), C.byref(b))
         return a.value, b.value
+
+    def set_fp32_rotation(self, kind):
+        """Rotation of FP32 sets: 1 split-BF16x3 (default), 0 FP32 MFMA."""
+        self.check(lib().itp_ctx_set_fp32_rotation(self.h, int(kind)), "set_fp32_rotation")
 
     def set_eig_cluster(self, tau):
         """Cluster start threshold on |a_ij| / |a_ii - a_jj| (0: off)."""

@@ -42,6 +42,7 @@ struct itp_ctx {
   int eig_mode = 0;    // 0: Jacobi handing off to the refinement iteration, 1: Jacobi only
   double eig_handoff = 0.5;   // hand-off threshold on |a_ij| / |a_ii - a_jj|
   double eig_cluster = 0.1;   // cluster start: pairs above this |a_ij| / |a_ii - a_jj| are solved exactly (0: off)
+  int fp32_rotation = 1;      // FP32 sets' rotation: 0 FP32 MFMA, 1 split-BF16x3 MFMA
   long long oa_done = 0, oa_fallback = 0;
   long long cl_calls = 0, cl_clusters = 0, cl_largest = 0, cl_fallback = 0;
 };
@@ -645,7 +646,12 @@ int rotate_set(itp_stateset* s, const cplx<R>* in, cplx<R>* out) {
   const GridInfo& g = s->grid->g;
   if (sizeof(R) == 4) {
     HIPCK(c, launch_convert(s->C, s->C32, (long long)s->nc * s->nc, 1, c->stream));
-    HIPCK(c, launch_rotate_f32((const cplx<float>*)in, (cplx<float>*)out, s->C32, s->nc, g.M, (int)s->N, c->stream));
+    if (c->fp32_rotation == 1)
+      HIPCK(c, launch_rotate_bf16x3((const cplx<float>*)in, (cplx<float>*)out, s->C32, s->nc, g.M, (int)s->N,
+                                    c->stream));
+    else
+      HIPCK(c, launch_rotate_f32((const cplx<float>*)in, (cplx<float>*)out, s->C32, s->nc, g.M, (int)s->N,
+                                 c->stream));
     return ITP_OK;
   }
   HIPCK(c, launch_rotate<R>(in, out, s->C, s->nc, g.M, (int)s->N, c->stream));
@@ -850,6 +856,12 @@ int itp_ctx_cluster_stats(itp_ctx* c, long long* out) {
   out[1] = c->cl_clusters;
   out[2] = c->cl_largest;
   out[3] = c->cl_fallback;
+  return ITP_OK;
+}
+
+int itp_ctx_set_fp32_rotation(itp_ctx* c, int kind) {
+  if (!c || kind < 0 || kind > 1) return ITP_E_INVALID;
+  c->fp32_rotation = kind;
   return ITP_OK;
 }
 

@@ -39,6 +39,9 @@ hipError_t launch_gram_bf16x3(const cplx<float>* A, const cplx<float>* B, long l
                               int* bad, hipStream_t st, int full = 0);
 hipError_t launch_rotate_f32(const cplx<float>* in, cplx<float>* out, const cplx<float>* C, int nc, long long M, int N,
                              hipStream_t st);
+// the same rotation with split-BF16x3 products on the bf16 matrix cores
+hipError_t launch_rotate_bf16x3(const cplx<float>* in, cplx<float>* out, const cplx<float>* C, int nc, long long M,
+                                int N, hipStream_t st);
 template <class R>
 hipError_t launch_residual(const cplx<R>* W, const cplx<R>* Psi, const double* E, long long M, int N, double dx,
                            double* sigma, hipStream_t st);

@@ -265,6 +265,133 @@ __global__ __launch_bounds__(256) void rotate_f32_kernel(const cplx<float>* in, 
       }
 }
 
+// ---------------------------------------------------------------------------
+// Psi_out[r][t] = sum_c C[c][r] Psi_in[c][t] for complex FP32 fields and an
+// FP32 copy of the coefficients, on v_mfma_f32_16x16x32_bf16 with the
+// split-BF16x3 products of the overlap (p + q <= 4): 64 output states x 64
+// grid points per workgroup, K = 16 input states (32 interleaved reals) per
+// step. A[r][2c + q] holds C[c][r] (re, im); the states are staged as
+// (re, im) pairs and turned in registers into (re, -im) for the real part and
+// (im, re) for the imaginary part (negating a bfloat16 is exact). FP32
+// accumulation over all input states, as on the FP32 MFMA path.
+// ---------------------------------------------------------------------------
+constexpr int kR3Pad = 8;  // bf16 row padding in LDS
+
+// (b0, b1, b2, b3, ...) -> (b0, -b1, b2, -b3, ...): the real part's operand
+__device__ __forceinline__ bf8 negodd(bf8 b) {
+  bf8 r;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    r[2 * c] = b[2 * c];
+    r[2 * c + 1] = -b[2 * c + 1];
+  }
+  return r;
+}
+// (b0, b1, b2, b3, ...) -> (b1, b0, b3, b2, ...): the imaginary part's operand
+__device__ __forceinline__ bf8 swappairs(bf8 b) {
+  bf8 r;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    r[2 * c] = b[2 * c + 1];
+    r[2 * c + 1] = b[2 * c];
+  }
+  return r;
+}
+
+__global__ __launch_bounds__(256) void rotate_bf16x3_kernel(const cplx<float>* in, cplx<float>* out,
+                                                            const cplx<float>* C, int nc, long long M, int N) {
+  __shared__ __align__(16) __bf16 As[3][64][32 + kR3Pad];  // [part][output state r][interleaved input k]
+  __shared__ __align__(16) __bf16 Bs[3][64][32 + kR3Pad];  // [part][grid point t][interleaved input k]
+  int rt;
+  long long tt;
+  // block order by xcd_tile: each XCD takes whole grid-point slabs and all
+  // output-state tiles of a slab
+  if (!xcd_tile(blockIdx.x, nc / 64, (M + 63) / 64, rt, tt)) return;
+  const long long t0 = tt * 64;
+  const int r0 = rt * 64;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wr = wave >> 1, wt = wave & 1;
+  f4 re[2][2], im[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      re[a][b] = f4{0, 0, 0, 0};
+      im[a][b] = f4{0, 0, 0, 0};
+    }
+  for (int c0 = 0; c0 < N; c0 += 16) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int e = tid + 256 * q;
+      const int cc = e >> 6, x = e & 63;
+      const int c = c0 + cc;
+      // C is zero-padded to nc x nc; states c >= N contribute nothing
+      const cplx<float> cv = C[(long long)(c < nc ? c : nc - 1) * nc + r0 + x];
+      const cplx<float> a = (c < N) ? cv : mk<float>(0.f, 0.f);
+      const cplx<float> pv = (c < N && t0 + x < M) ? in[(long long)c * M + t0 + x] : mk<float>(0.f, 0.f);
+      __bf16 h, m, l, h2, m2, l2;
+      split3(a.x, h, m, l);
+      split3(a.y, h2, m2, l2);
+      As[0][x][2 * cc] = h; As[0][x][2 * cc + 1] = h2;
+      As[1][x][2 * cc] = m; As[1][x][2 * cc + 1] = m2;
+      As[2][x][2 * cc] = l; As[2][x][2 * cc + 1] = l2;
+      split3(pv.x, h, m, l);
+      split3(pv.y, h2, m2, l2);
+      Bs[0][x][2 * cc] = h; Bs[0][x][2 * cc + 1] = h2;
+      Bs[1][x][2 * cc] = m; Bs[1][x][2 * cc + 1] = m2;
+      Bs[2][x][2 * cc] = l; Bs[2][x][2 * cc + 1] = l2;
+    }
+    __syncthreads();
+    const int kcol = 8 * (lane >> 4);
+    bf8 a[3][2], br[3][2], bi[3][2];
+#pragma unroll
+    for (int sp = 0; sp < 3; sp++)
+#pragma unroll
+      for (int x = 0; x < 2; x++) {
+        a[sp][x] = *reinterpret_cast<const bf8*>(&As[sp][wr * 32 + x * 16 + (lane & 15)][kcol]);
+        const bf8 b = *reinterpret_cast<const bf8*>(&Bs[sp][wt * 32 + x * 16 + (lane & 15)][kcol]);
+        br[sp][x] = negodd(b);
+        bi[sp][x] = swappairs(b);
+      }
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+      for (int y = 0; y < 2; y++) {
+        // split products with p + q <= 4 (1-based), smallest first
+        re[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][x], br[0][y], re[x][y], 0, 0, 0);
+        re[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], br[2][y], re[x][y], 0, 0, 0);
+        re[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][x], br[1][y], re[x][y], 0, 0, 0);
+        re[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][x], br[0][y], re[x][y], 0, 0, 0);
+        re[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], br[1][y], re[x][y], 0, 0, 0);
+        re[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], br[0][y], re[x][y], 0, 0, 0);
+        im[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][x], bi[0][y], im[x][y], 0, 0, 0);
+        im[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], bi[2][y], im[x][y], 0, 0, 0);
+        im[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][x], bi[1][y], im[x][y], 0, 0, 0);
+        im[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][x], bi[0][y], im[x][y], 0, 0, 0);
+        im[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], bi[1][y], im[x][y], 0, 0, 0);
+        im[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], bi[0][y], im[x][y], 0, 0, 0);
+      }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int r = r0 + wr * 32 + x * 16 + 4 * (lane >> 4) + v;
+        const long long t = t0 + wt * 32 + y * 16 + (lane & 15);
+        if (r < N && t < M) out[(long long)r * M + t] = mk<float>(re[x][y][v], im[x][y][v]);
+      }
+}
+
+hipError_t launch_rotate_bf16x3(const cplx<float>* in, cplx<float>* out, const cplx<float>* C, int nc, long long M,
+                                int N, hipStream_t st) {
+  const long long blocks = xcd_grid(nc / 64, (M + 63) / 64);
+  if (blocks > 0x7fffffffLL || nc % 64) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(rotate_bf16x3_kernel, dim3((unsigned)blocks), dim3(256), 0, st, in, out, C, nc, M, N);
+  return hipGetLastError();
+}
+
 hipError_t launch_rotate_f32(const cplx<float>* in, cplx<float>* out, const cplx<float>* C, int nc, long long M, int N,
                              hipStream_t st) {
   const long long blocks = xcd_grid(nc / 64, (M + 63) / 64);

@@ -454,6 +454,31 @@ def test_orthonormalize_fp32(ctx):
     assert np.all(np.diff(lam) >= 0)
 
 
+@pytest.mark.parametrize("N,sx", [(40, 64), (130, 32), (500, 64)])
+def test_fp32_rotation_kinds_agree(ctx, N, sx):
+    """The split-BF16x3 rotation of FP32 sets against the FP32-MFMA one on the
+    same overlap and coefficients: the same orthonormalized set to FP32
+    accuracy (both accumulate in FP32)."""
+    say(f"fp32 rotations N={N}")
+    dx = 12.0 / sx
+    g = I.Grid(ctx, sx, sx, dx, 0)
+    f = rand_states(N, sx, sx, dx, 77 + N)
+    outs = []
+    try:
+        for kind in (0, 1):
+            ctx.set_fp32_rotation(kind)
+            S = I.StateSet(g, N, I.FP32)
+            S.upload(f)
+            S.orthonormalize()
+            outs.append(S.download())
+    finally:
+        ctx.set_fp32_rotation(1)
+    assert np.abs(outs[1] - outs[0]).max() < 3e-6 * np.abs(outs[0]).max()
+    A = outs[1].reshape(N, -1)
+    G = (A.conj() @ A.T) * dx * dx
+    assert np.abs(G - np.eye(N)).max() < 2e-5
+
+
 @pytest.mark.parametrize("bt,B", BRANCHES)
 def test_rayleigh_ritz_fp64(ctx, bt, B):
     """FP64 Rayleigh-Ritz against the oracle: Ritz values of the H-matrix built
