@@ -29,6 +29,7 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef double d4m __attribute__((ext_vector_type(4)));
 typedef __bf16 bf4v __attribute__((ext_vector_type(4)));
+typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
 
 constexpr int kG3Rows = 64;     // states per tile side
 constexpr int kG3KT = 64;       // reals per K step
@@ -328,17 +329,20 @@ __global__ __launch_bounds__(256) void rotate_bf16x3_kernel(const cplx<float>* i
       const cplx<float> cv = C[(long long)(c < nc ? c : nc - 1) * nc + r0 + x];
       const cplx<float> a = (c < N) ? cv : mk<float>(0.f, 0.f);
       const cplx<float> pv = (c < N && t0 + x < M) ? in[(long long)c * M + t0 + x] : mk<float>(0.f, 0.f);
-      __bf16 h, m, l, h2, m2, l2;
-      split3(a.x, h, m, l);
-      split3(a.y, h2, m2, l2);
-      As[0][x][2 * cc] = h; As[0][x][2 * cc + 1] = h2;
-      As[1][x][2 * cc] = m; As[1][x][2 * cc + 1] = m2;
-      As[2][x][2 * cc] = l; As[2][x][2 * cc + 1] = l2;
-      split3(pv.x, h, m, l);
-      split3(pv.y, h2, m2, l2);
-      Bs[0][x][2 * cc] = h; Bs[0][x][2 * cc + 1] = h2;
-      Bs[1][x][2 * cc] = m; Bs[1][x][2 * cc + 1] = m2;
-      Bs[2][x][2 * cc] = l; Bs[2][x][2 * cc + 1] = l2;
+      // each (re, im) pair of a split part goes to LDS as one 4-byte store
+      __bf16 s[4][3];
+      split3(a.x, s[0][0], s[0][1], s[0][2]);
+      split3(a.y, s[1][0], s[1][1], s[1][2]);
+      split3(pv.x, s[2][0], s[2][1], s[2][2]);
+      split3(pv.y, s[3][0], s[3][1], s[3][2]);
+      const bf2v ah = {s[0][0], s[1][0]}, am = {s[0][1], s[1][1]}, al = {s[0][2], s[1][2]};
+      const bf2v bh = {s[2][0], s[3][0]}, bm = {s[2][1], s[3][1]}, bl = {s[2][2], s[3][2]};
+      *reinterpret_cast<bf2v*>(&As[0][x][2 * cc]) = ah;
+      *reinterpret_cast<bf2v*>(&As[1][x][2 * cc]) = am;
+      *reinterpret_cast<bf2v*>(&As[2][x][2 * cc]) = al;
+      *reinterpret_cast<bf2v*>(&Bs[0][x][2 * cc]) = bh;
+      *reinterpret_cast<bf2v*>(&Bs[1][x][2 * cc]) = bm;
+      *reinterpret_cast<bf2v*>(&Bs[2][x][2 * cc]) = bl;
     }
     __syncthreads();
     const int kcol = 8 * (lane >> 4);
