@@ -186,11 +186,15 @@ int ensure_ws(itp_ctx* c, size_t bytes) {
 void flush_events(itp_ctx* c) {
   if (c->ev_used == 0) return;
   (void)hipEventSynchronize(c->evpool[c->ev_used - 1]);
-  for (size_t p = 0; p + 1 < c->ev_used; p += 2) {
+  // events 2k and 2k+1 bracket the phase pair_phase[k]
+  const size_t pairs = std::min(c->ev_used / 2, c->pair_phase.size());
+  for (size_t k = 0; k < pairs; k++) {
+    const int ph = c->pair_phase[k];
     float t = 0;
-    if (hipEventElapsedTime(&t, c->evpool[p], c->evpool[p + 1]) == hipSuccess) {
-      c->ms[c->pair_phase[p / 2]] += t;
-      c->cnt[c->pair_phase[p / 2]]++;
+    if (ph >= 0 && ph < ITP_NUM_PHASES &&
+        hipEventElapsedTime(&t, c->evpool[2 * k], c->evpool[2 * k + 1]) == hipSuccess) {
+      c->ms[ph] += t;
+      c->cnt[ph]++;
     }
   }
   c->ev_used = 0;
@@ -198,6 +202,9 @@ void flush_events(itp_ctx* c) {
 }
 
 // HIP-event timing of one phase on the context's stream (only when enabled).
+// The stop event is recorded by stop() or, on any other way out of the
+// scope (early returns included), by the destructor, so every start event
+// has its stop event next to it in the pool.
 struct Phase {
   itp_ctx* c;
   int ph;
@@ -207,6 +214,9 @@ struct Phase {
     if (c->ev_used + 2 > c->evpool.size()) flush_events(c);
     (void)hipEventRecord(c->evpool[c->ev_used++], c->stream);
   }
+  ~Phase() { stop(); }
+  Phase(const Phase&) = delete;
+  Phase& operator=(const Phase&) = delete;
   void stop() {
     if (!on) return;
     (void)hipEventRecord(c->evpool[c->ev_used++], c->stream);

@@ -338,6 +338,37 @@ def test_driver_abort_flag_and_printout(ctx, capfd):
     assert f"Total {s.total_steps} steps of ITP performed." in out
     assert "Ratios:" in out and "\tEnergies:" in out
     assert out.count("Step ") >= s.total_steps
+    ctx.enable_timing(False)  # verbosity 2 and above turned the context's phase timing on
+
+
+def test_phase_timing_counts(ctx):
+    """Phase timing with every kind of call that records phases (the
+    Rayleigh-Ritz pass ends its phases at scope exit): each phase is counted
+    once per run of it, also across a flush of the full event pool (2048
+    events, 1024 phases)."""
+    say("phase timing counts")
+    sx, N = 32, 12
+    dx = 12.0 / sx
+    g = I.Grid(ctx, sx, sx, dx, 0)
+    H = I.Hamiltonian(I.Potential(g, W.harmonic(sx, sx, dx)), 0.0)
+    S = I.StateSet(g, N)
+    S.upload(rand_states(N, sx, sx, dx, 5))
+    ctx.phase_times()
+    ctx.reset_phase_times()
+    ctx.enable_timing(True)
+    try:
+        for _ in range(3):
+            H.rayleigh_ritz(S)
+        for _ in range(400):
+            S.orthonormalize()
+        t = ctx.phase_times()
+    finally:
+        ctx.enable_timing(False)
+        ctx.reset_phase_times()
+    # per Rayleigh-Ritz pass: two overlaps, two eigensolves, two rotations, one H application
+    assert {k: v[1] for k, v in t.items()} == {"propagate": 0, "overlap": 406, "eigensolve": 406,
+                                              "lincomb": 406, "energies": 3}
+    assert all(v[0] > 0 for k, v in t.items() if k != "propagate")
 
 
 def test_c2_full_size_steps(ctx):
