@@ -111,7 +111,8 @@ def mixed_leg(I, ctx, cfg, V, args):
            "steps_per_solve": s0.total_steps, "fp32_steps_per_solve": s0.mixed_steps,
            "converged_per_solve": s0.num_converged, "converged_within_1e-8_of_fp64": good // len(systems),
            "fp64_converged": int(conv64.sum()), "max_rel_err_vs_fp64": worst,
-           "dtype": "c32 fields and sweeps, bf16x3 MFMA overlap (fp32 accumulate, fp64 sums), fp32 MFMA rotation, "
+           "dtype": "c32 fields and sweeps, bf16x3 MFMA overlap (fp32 accumulate, fp64 sums), "
+                    "split-bf16x3 MFMA rotation (fp32 accumulate), "
                     "fp64 eigensolver; fp64 final steps"}
     log(f"mixed leg: {out}")
     return out
