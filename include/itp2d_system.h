@@ -111,6 +111,18 @@ int itps_converged(const itps_system* s, int* flags);
  * reference. */
 int itps_rayleigh_ritz(itps_system* s);
 
+/* Step history for result files (the data the reference writes to its
+ * datafile during the run, itpsystem.cpp:113-114, 221-226, 399-403):
+ * itps_history_steps gives the number of rows (the last step that computed
+ * energies); row k holds the sorted energies and deviations of step k + 1
+ * (NaN for a step that computed none, e.g. one ended by a recovery), N
+ * values each. itps_time_step_history returns the number of time-step
+ * changes and fills up to cap (first step using the value, value) pairs; the
+ * first pair is the initial time step at step 1. */
+size_t itps_history_steps(const itps_system* s);
+int itps_history_row(const itps_system* s, size_t step, double* E, double* sigma);
+size_t itps_time_step_history(const itps_system* s, int* steps, double* eps, size_t cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -141,6 +141,9 @@ def lib():
             "itps_total_time": [_vp],
             "itps_recoveries": [_vp],
             "itps_converged": [_vp, C.POINTER(C.c_int)],
+            "itps_history_steps": [_vp],
+            "itps_history_row": [_vp, _sz, _dp, _dp],
+            "itps_time_step_history": [_vp, C.POINTER(C.c_int), _dp, _sz],
             "itps_rayleigh_ritz": [_vp],
             "itps_mixed_steps": [_vp],
             "itps_set_abort_flag": [_vp, C.POINTER(C.c_int)],
@@ -169,6 +172,8 @@ def lib():
         L.itps_total_time.restype = C.c_double
         L.itps_num_converged.restype = _sz
         L.itps_num_timestep_converged.restype = _sz
+        L.itps_history_steps.restype = _sz
+        L.itps_time_step_history.restype = _sz
         L.itp_batch_size.restype = _sz
         L.itp_batch_wall_time.restype = C.c_double
         _lib = L
@@ -607,6 +612,22 @@ class ITPSystem:
         f = np.zeros(int(self.cfg["N"]), dtype=np.int32)
         lib().itps_converged(self.h, f.ctypes.data_as(C.POINTER(C.c_int)))
         return f.astype(bool)
+
+    def history(self):
+        """Step history: (E, sigma) arrays [steps][N] of the sorted energies and
+        deviations of every step (NaN rows for steps that computed none)."""
+        steps, N = lib().itps_history_steps(self.h), int(self.cfg["N"])
+        E, sd = np.zeros((steps, N)), np.zeros((steps, N))
+        for k in range(steps):
+            self.ctx.check(lib().itps_history_row(self.h, k, _p(E[k]), _p(sd[k])), "itps_history_row")
+        return E, sd
+
+    def time_step_history(self):
+        """(first step using it, eps) pairs of every time step the run used."""
+        n = lib().itps_time_step_history(self.h, None, None, 0)
+        steps, eps = (C.c_int * max(n, 1))(), np.zeros(max(n, 1))
+        lib().itps_time_step_history(self.h, steps, _p(eps), n)
+        return [(int(steps[k]), float(eps[k])) for k in range(n)]
 
     def sorted(self):
         N = self.cfg["N"]

@@ -3,6 +3,10 @@
 Sources: csrc/kernels.hip, csrc/mixed.hip (kernels), csrc/capi.hip (C ABI of include/itp2d.h),
 host/itpsystem.cpp (ITP driver, C ABI of include/itp2d_system.h), host/batch.cpp (batches of
 instances on worker threads, C ABI of include/itp2d_batch.h).
+
+Also builds the host-only itp2d_amd/lib/libitp2d_datafile.so (host/datafile.cpp, the HDF5
+result file of include/itp2d_datafile.h) with g++ against the HDF5 C library under
+HDF5_DIR (default /opt/conda), where its headers are present.
 """
 import os
 import subprocess
@@ -12,6 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libitp2d_amd.so")
+DATAFILE_LIB = os.path.join(LIB_DIR, "libitp2d_datafile.so")
+HDF5_DIR = os.environ.get("HDF5_DIR", "/opt/conda")
 
 SOURCES = [
     os.path.join(HERE, "csrc", "kernels.hip"),
@@ -54,7 +60,25 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
+    build_datafile(verbose)
     return LIB
+
+
+def build_datafile(verbose=False):
+    """The HDF5 result-file library; returns its path, or None without HDF5 headers."""
+    inc, lib = os.path.join(HDF5_DIR, "include"), os.path.join(HDF5_DIR, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        print(f"itp2d_amd: no hdf5.h under {inc}; the datafile library is not built", flush=True)
+        return None
+    tmp = DATAFILE_LIB + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-Wall", "-I", os.path.join(ROOT, "include"),
+           "-I", inc, os.path.join(HERE, "host", "datafile.cpp"), "-o", tmp, "-L", lib, "-lhdf5",
+           "-Wl,-rpath," + lib]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, DATAFILE_LIB)
+    return DATAFILE_LIB
 
 
 if __name__ == "__main__":

@@ -36,6 +36,7 @@ class ITPSystem {
     if (!eps_values.empty()) { eps = eps_values.front(); eps_values.pop_front(); }
     else eps = 0.5;  // Parameters::default_initial_eps (parameters.cpp:49)
     T_.reset(new MultiProductSplit(p.halforder, *pot_, eps, p.B));
+    ts_hist.emplace_back(1, eps);
     mixed = p.precision == ITP_FP32;
     if (mixed) {
       if (!(params.mixed_tol > 0)) params.mixed_tol = 2e-7;
@@ -180,6 +181,13 @@ class ITPSystem {
   double total_time = 0;
   // abort flag polled at the start of every step (the reference's
   // abort_flagptr, raised by its SIGINT handler, itp2d.cpp:35-60)
+  // Per total step (row k: the sorted energies and deviations computed in
+  // step k + 1; empty for a step that computed none), and the time steps as
+  // (first step using it, eps) pairs: what a result file's histories hold
+  // (itpsystem.cpp:113-114, 221-226, 399-403).
+  std::vector<std::vector<double>> hist_E, hist_sd;
+  std::vector<std::pair<int, double>> ts_hist;
+  bool record_steps = true;
   const volatile int* abort_flag = nullptr;
   int verbosity = 0;  // 0 quiet; 1-3 as the reference's -v levels
   bool pot_null = false;
@@ -223,6 +231,7 @@ class ITPSystem {
       return;
     }
     T_->set_time_step(eps);
+    ts_hist.emplace_back(total_step_counter + 1, eps);  // first step with the new value
     if (verbosity >= 1) std::fprintf(out, "\tEpsilon changed to %e.\n", eps);
     step_counter = 0;
     all_ts = false;
@@ -316,6 +325,11 @@ class ITPSystem {
   // the Hermitian part of H in their span); converged states by the
   // eigenvalue error bound sigma^2 / gap <= accuracy |E|.
   void rayleigh_ritz_pass() {
+    record_steps = false;  // not a step: the step history keeps the last step's row
+    struct Restore {
+      bool& r;
+      ~Restore() { r = true; }
+    } restore{record_steps};
     std::vector<double> E, sd;
     H_->rayleigh_ritz(*states_, E, sd);
     esn.resize(params.N);
@@ -353,6 +367,12 @@ class ITPSystem {
     energies.emplace_back();
     stdevs.emplace_back();
     for (auto& t : esn) { energies.back().push_back(std::get<0>(t)); stdevs.back().push_back(std::get<1>(t)); }
+    if (record_steps && total_step_counter > 0) {  // row total_step_counter - 1 of the step history
+      hist_E.resize(total_step_counter);
+      hist_sd.resize(total_step_counter);
+      hist_E.back() = energies.back();
+      hist_sd.back() = stdevs.back();
+    }
   }
 
   // ConvergenceTest::test for sorted position n (convergence.cpp:75-200)
@@ -514,6 +534,30 @@ int itps_converged(const itps_system* s, int* flags) {
   if (!s || !flags) return ITP_E_INVALID;
   for (size_t n = 0; n < s->sys->params.N; n++) flags[n] = s->sys->converged_sorted(n) ? 1 : 0;
   return ITP_OK;
+}
+
+size_t itps_history_steps(const itps_system* s) { return s ? s->sys->hist_E.size() : 0; }
+
+int itps_history_row(const itps_system* s, size_t step, double* E, double* sigma) {
+  if (!s || step >= s->sys->hist_E.size()) return ITP_E_INVALID;
+  const size_t N = s->sys->params.N;
+  const auto& e = s->sys->hist_E[step];
+  const auto& d = s->sys->hist_sd[step];
+  for (size_t n = 0; n < N; n++) {
+    if (E) E[n] = n < e.size() ? e[n] : std::nan("");
+    if (sigma) sigma[n] = n < d.size() ? d[n] : std::nan("");
+  }
+  return ITP_OK;
+}
+
+size_t itps_time_step_history(const itps_system* s, int* steps, double* eps, size_t cap) {
+  if (!s) return 0;
+  const auto& h = s->sys->ts_hist;
+  for (size_t k = 0; k < h.size() && k < cap; k++) {
+    if (steps) steps[k] = h[k].first;
+    if (eps) eps[k] = h[k].second;
+  }
+  return h.size();
 }
 
 }  // extern "C"

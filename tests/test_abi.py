@@ -13,9 +13,9 @@ from oracle import pyoracle as O
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
+def declared_functions(headers=("itp2d.h", "itp2d_system.h", "itp2d_batch.h")):
     names = set()
-    for h in ("itp2d.h", "itp2d_system.h", "itp2d_batch.h"):
+    for h in headers:
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         for m in re.finditer(r"\b(itps?_[a-z0-9_]+)\s*\(", text):
@@ -27,6 +27,19 @@ def test_library_exports_all_declared_symbols():
     lib = ctypes.CDLL(itp2d_amd.library_path)
     names = declared_functions()
     assert len(names) > 40
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_datafile_library_exports_all_declared_symbols():
+    """The host-only HDF5 result-file library (include/itp2d_datafile.h)."""
+    import pytest
+    from itp2d_amd import datafile
+    if not os.path.exists(datafile.LIB_PATH):
+        pytest.skip("datafile library not built (no HDF5 C library)")
+    lib = ctypes.CDLL(datafile.LIB_PATH)
+    names = declared_functions(("itp2d_datafile.h",))
+    assert len(names) >= 14
     missing = [n for n in sorted(names) if not hasattr(lib, n)]
     assert not missing, missing
 
