@@ -191,60 +191,89 @@ def save_run(system, path, V=None, save="final_states", clobber=False, attribute
     if save not in ("final_states", "energies"):
         raise ValueError("save must be 'final_states' or 'energies'")
     cfg = system.cfg
-    sx, sy = int(cfg["sx"]), int(cfg["sy"])
-    dx = float(cfg["lenx"]) / sx
-    extra = dict(attributes or {})
-    with Datafile(path, sx, sy, dx, clobber) as f:
-        # run description (itpsystem.cpp:73-97, 113)
-        f.attr("program_version", extra.pop("program_version", PROGRAM_VERSION))
-        f.attr_ulong("random_seed", int(cfg["seed"]))
-        f.attr("start_time", extra.pop("start_time", time.strftime("%Y-%m-%d %H:%M:%S")))
-        f.attr("num_threads", int(extra.pop("num_threads", 1)))
-        f.attr("num_states", int(cfg["N"]))
-        f.attr("num_wanted_to_converge", int(cfg["n_wanted"]))
-        f.attr("ignore_lowest", int(cfg.get("ignore_lowest", 0)))
-        f.attr("grid_length", float(cfg["lenx"]))
-        f.attr("grid_boundary_type", "dirichlet" if int(cfg["boundary"]) else "periodic")
-        f.attr("operator_splitting_order", 2 * int(cfg["halforder"]))
-        for name in ("potential", "noise", "impurity_type", "impurity_distribution", "impurity_constraint"):
-            f.attr(name, extra.pop(name, "none"))
-        f.attr("timestep_convergence_test", convergence_test_description(cfg["timestep_test"]))
-        f.attr("final_convergence_test", convergence_test_description(cfg["final_test"]))
-        f.attr("magnetic_field_strength", float(cfg["B"]))
-        if V is not None and np.any(np.asarray(V) != 0):  # a null potential is not written
-            f.write_potential(np.asarray(V, dtype=np.float64).reshape(sy, sx))
-        if noise_data is not None:
-            f.write_noise(noise_data)
-        ts = system.time_step_history()
-        if ts:
-            f.attr("initial_time_step", ts[0][1])
-        for step, eps in ts:
-            f.write_time_step(step, eps)
-        # per-step energies and deviations (itpsystem.cpp:399-403)
-        E_hist, sd_hist = system.history()
-        for k in range(E_hist.shape[0]):
-            f.write_history_row(0, k, E_hist[k])
-            f.write_history_row(1, k, sd_hist[k])
-        # finish (itpsystem.cpp:512-536)
-        E, sd, idx = system.sorted()
-        if save == "final_states":
-            f.write_states(system.total_steps, system.states(), order=idx)
-        f.write_final(0, E)
-        f.write_final(1, sd)
-        f.attr("num_converged", int(system.num_converged))
-        f.attr("error_flag", int(system.error_flag))
-        f.attr("total_steps_done", int(system.total_steps))
-        times = system.ctx.phase_times()
-        for name, phase in (("propagation_time", "propagate"), ("orthonormalization_time", None),
-                            ("dotproduct_time", "overlap"), ("eigensolve_time", "eigensolve"),
-                            ("lincomb_time", "lincomb"), ("convtest_time", "energies")):
-            if phase is None:
-                ms = sum(times[p][0] for p in ("overlap", "eigensolve", "lincomb"))
-            else:
-                ms = times[phase][0]
-            f.attr(name, ms / 1e3)
-        f.attr("io_time", 0.0)
-        f.attr("total_time", float(system.total_time))
-        for name, value in extra.items():
-            f.attr(name, value)
+    with Datafile(path, int(cfg["sx"]), int(cfg["sy"]), float(cfg["lenx"]) / int(cfg["sx"]), clobber) as f:
+        write_run_attributes(f, system, V, attributes, noise_data)
+        write_run_results(f, system, final_states=save == "final_states")
     return path
+
+
+def run_saving_every_step(system, path, V=None, clobber=False, attributes=None, noise_data=None):
+    """Runs `system` to the end, saving as the reference's Everything mode does:
+    the initial states (unsorted, step 0, itpsystem.cpp:132-133), the sorted
+    states after every step (itpsystem.cpp:451-453), the histories and the
+    final energies. Returns the driver's status."""
+    cfg = system.cfg
+    rc = 0
+    with Datafile(path, int(cfg["sx"]), int(cfg["sy"]), float(cfg["lenx"]) / int(cfg["sx"]), clobber) as f:
+        write_run_attributes(f, system, V, attributes, noise_data)
+        f.write_states(0, system.states())
+        while not system.finished:
+            r = system.step()
+            rc = rc or r
+            if system.total_steps > 0 and not (system.finished and system.error_flag):
+                f.write_states(system.total_steps, system.states(), order=system.sorted()[2])
+        write_run_results(f, system, final_states=False)
+    return rc
+
+
+def write_run_attributes(f, system, V=None, attributes=None, noise_data=None):
+    """The run description written when the reference's ITPSystem starts
+    (itpsystem.cpp:70-114): run attributes, the potential, the noise data."""
+    cfg = system.cfg
+    sx, sy = int(cfg["sx"]), int(cfg["sy"])
+    extra = dict(attributes or {})
+    f.attr("program_version", extra.pop("program_version", PROGRAM_VERSION))
+    f.attr_ulong("random_seed", int(cfg["seed"]))
+    f.attr("start_time", extra.pop("start_time", time.strftime("%Y-%m-%d %H:%M:%S")))
+    f.attr("num_threads", int(extra.pop("num_threads", 1)))
+    f.attr("num_states", int(cfg["N"]))
+    f.attr("num_wanted_to_converge", int(cfg["n_wanted"]))
+    f.attr("ignore_lowest", int(cfg.get("ignore_lowest", 0)))
+    f.attr("grid_length", float(cfg["lenx"]))
+    f.attr("grid_boundary_type", "dirichlet" if int(cfg["boundary"]) else "periodic")
+    f.attr("operator_splitting_order", 2 * int(cfg["halforder"]))
+    for name in ("potential", "noise", "impurity_type", "impurity_distribution", "impurity_constraint"):
+        f.attr(name, extra.pop(name, "none"))
+    f.attr("timestep_convergence_test", convergence_test_description(cfg["timestep_test"]))
+    f.attr("final_convergence_test", convergence_test_description(cfg["final_test"]))
+    f.attr("magnetic_field_strength", float(cfg["B"]))
+    for name, value in extra.items():
+        f.attr(name, value)
+    if V is not None and np.any(np.asarray(V) != 0):  # a null potential is not written
+        f.write_potential(np.asarray(V, dtype=np.float64).reshape(sy, sx))
+    if noise_data is not None:
+        f.write_noise(noise_data)
+
+
+def write_run_results(f, system, final_states=True):
+    """What the reference writes during and at the end of the run
+    (itpsystem.cpp:113-114, 221-226, 399-403, 512-536)."""
+    ts = system.time_step_history()
+    if ts:
+        f.attr("initial_time_step", ts[0][1])
+    for step, eps in ts:
+        f.write_time_step(step, eps)
+    # per-step energies and deviations
+    E_hist, sd_hist = system.history()
+    for k in range(E_hist.shape[0]):
+        f.write_history_row(0, k, E_hist[k])
+        f.write_history_row(1, k, sd_hist[k])
+    E, sd, idx = system.sorted()
+    if final_states:
+        f.write_states(system.total_steps, system.states(), order=idx)
+    f.write_final(0, E)
+    f.write_final(1, sd)
+    f.attr("num_converged", int(system.num_converged))
+    f.attr("error_flag", int(system.error_flag))
+    f.attr("total_steps_done", int(system.total_steps))
+    times = system.ctx.phase_times()
+    for name, phase in (("propagation_time", "propagate"), ("orthonormalization_time", None),
+                        ("dotproduct_time", "overlap"), ("eigensolve_time", "eigensolve"),
+                        ("lincomb_time", "lincomb"), ("convtest_time", "energies")):
+        if phase is None:
+            ms = sum(times[p][0] for p in ("overlap", "eigensolve", "lincomb"))
+        else:
+            ms = times[phase][0]
+        f.attr(name, ms / 1e3)
+    f.attr("io_time", 0.0)
+    f.attr("total_time", float(system.total_time))
