@@ -86,7 +86,7 @@ def test_cli_run_matches_api(ctx, tmp_path):
     from itp2d_amd import datafile as D
     print("[gpu] command-line run", flush=True)
     argv = ["-s", "32", "-l", "10", "-n", "6", "-N", "8", "-d", "2", "-e", "0.1", "-p", "harmonic(1.5)",
-            "-T", "relchange(1e-7)", "-F", "relchange(1e-7)", "--rngseed", "11", "-q"]
+            "-T", "relchange(1e-10)", "-F", "relchange(1e-10)", "--maxsteps", "2000", "--rngseed", "11", "-q"]
     cfg, opts = cli.parse(argv)
     g = I.Grid(ctx, 32, 32, 10 / 32, 0)
     V = I.Potential.of_type(g, "harmonic", [1.5]).values()
