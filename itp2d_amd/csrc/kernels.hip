@@ -503,8 +503,11 @@ hipError_t launch_block_jacobi(cplx<double>* A, cplx<double>* V, cplx<double>* Q
     return hipErrorInvalidValue;
   // grid_wgs <= 0: one workgroup per 16 super-tile updates of a round, at
   // least 64 (below that the grid barrier is cheap and the sub-problems of a
-  // round are the critical path), at most what is co-resident
-  const int want = grid_wgs > 0 ? grid_wgs : std::max(64, bj_items(np) / 16);
+  // round are the critical path), at most what is co-resident; from one
+  // workgroup per CU on, whole workgroups per CU (with a partial second row,
+  // every round waits for the CUs holding two)
+  int want = grid_wgs > 0 ? grid_wgs : std::max(64, bj_items(np) / 16);
+  if (grid_wgs <= 0 && ncu > 0 && want >= ncu) want = ncu * (want / ncu);
   int grid = std::min(want, per_cu * ncu);
   grid = std::max(1, std::min(grid, bj_items(np)));
   void* args[] = {&a};
