@@ -156,6 +156,9 @@ def parse(argv):
         raise CLIError("--order has to be even")
     if ns.mineps < 0:
         raise CLIError("--mineps cannot be negative")
+    # unsigned in the reference (size_t, unsigned long)
+    if ns.ignore_lowest < 0 or (ns.rngseed is not None and ns.rngseed < 0):
+        raise CLIError("--ignore-lowest and --rngseed cannot be negative")
     if ns.noise == "impurities":
         raise CLIError("--noise impurities: the reference's random impurity realisations are not reproduced; "
                        "pass realisations through itp_potential_add_impurities")

@@ -47,7 +47,7 @@ def test_derived_values():
     ["-F", "nosuchtest"], ["-p", "nosuchpotential"], ["-p", "harmonic(1,2)"], ["-p", "harmonic 1"],
     ["--noise", "impurities", "--impurity-type", "gaussian", "--impurity-distribution", "uniform(1)"],
     ["--impurity-type", "gaussian"], ["--noise", "unknown"], ["--copy-states", "a.h5"], ["--no-such-option"],
-    ["-s", "x"]])
+    ["-s", "x"], ["--ignore-lowest", "-1"], ["--rngseed", "-5"]])
 def test_rejected(argv):
     with pytest.raises(cli.CLIError):
         cli.parse(argv)
