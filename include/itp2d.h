@@ -233,6 +233,11 @@ int itp_ctx_set_fp32_rotation(itp_ctx* ctx, int kind);
 /* Workgroups of the cooperative eigensolver launch (default 0: one per 16
  * super-tile updates of a round, at least 64, at most what is co-resident). */
 int itp_ctx_set_eig_grid(itp_ctx* ctx, int workgroups);
+/* Polls of a block-Jacobi grid barrier before the wait gives up (default
+ * 2^24). A wait that gives up makes every workgroup of the launch leave it, and
+ * the call returns ITP_E_EIGENSOLVER with the states untouched. 0 makes every
+ * workgroup but the last to arrive give up at once (used to test that path). */
+int itp_ctx_set_eig_spin_limit(itp_ctx* ctx, unsigned long long polls);
 /* Eigensolver of the overlap matrix (replaces zheev, eigensolver.hpp:80-84):
  * 0 (default) block Jacobi sweeps until every unconverged off-diagonal element
  * is below half of its diagonal gap, then the refinement iteration

@@ -5,6 +5,8 @@ StateArray order, statearray.cpp:21-75); potentials are float64 [sy, sx].
 """
 import ctypes as C
 import os
+import sys
+import weakref
 
 import numpy as np
 
@@ -84,6 +86,7 @@ def lib():
             "itp_ctx_eig_stats": [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
             "itp_ctx_set_eig_inner": [_vp, C.c_int],
             "itp_ctx_set_eig_grid": [_vp, C.c_int],
+            "itp_ctx_set_eig_spin_limit": [_vp, C.c_ulonglong],
             "itp_ctx_set_eig_mode": [_vp, C.c_int],
             "itp_ctx_set_eig_handoff": [_vp, C.c_double],
             "itp_ctx_oa_stats": [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
@@ -184,14 +187,46 @@ def _p(a):
     return None if a is None else a.ctypes.data_as(_dp)
 
 
+class _Owned:
+    """A handle of the library with an explicit close().
+
+    close() frees the handle at once (the owning Context closes its children
+    first, newest first). __del__ calls close() too, except while the
+    interpreter is finalizing: objects are then collected in no set order
+    and the HIP runtime may already be shutting down, so the handles are left
+    to the process exit instead of being freed through HIP calls.
+    """
+
+    _destroy = None  # name of the library's destroy function
+
+    def _register(self, ctx):
+        kids = ctx._children
+        if len(kids) >= 256:
+            kids[:] = [r for r in kids if r() is not None]
+        kids.append(weakref.ref(self))
+
+    def close(self):
+        h = getattr(self, "h", None)
+        if h and _lib is not None:
+            getattr(_lib, self._destroy)(h)
+        self.h = None
+
+    def __del__(self):
+        if sys.is_finalizing():
+            return
+        self.close()
+
+
 def _c128(a):
     return np.ascontiguousarray(a, dtype=np.complex128)
 
 
 class Context:
-    """One device and one stream."""
+    """One device and one stream. close() (or leaving a `with` block) frees
+    every object made on it, newest first, then the context itself."""
 
     def __init__(self, device=0):
+        self._children = []
         self.h = _vp()
         rc = lib().itp_ctx_create(device, C.byref(self.h))
         if rc:
@@ -242,6 +277,11 @@ class Context:
     def set_eig_grid(self, workgroups):
         self.check(lib().itp_ctx_set_eig_grid(self.h, int(workgroups)), "set_eig_grid")
 
+    def set_eig_spin_limit(self, polls):
+        """Polls of a block-Jacobi grid barrier before the wait gives up (default 2**24); a wait that
+        gives up ends the call with EigensolverError."""
+        self.check(lib().itp_ctx_set_eig_spin_limit(self.h, int(polls)), "set_eig_spin_limit")
+
     def set_eig_mode(self, mode):
         """0: Jacobi handing off to the refinement iteration (default); 1: Jacobi only."""
         self.check(lib().itp_ctx_set_eig_mode(self.h, int(mode)), "set_eig_mode")
@@ -270,19 +310,39 @@ class Context:
         self.check(lib().itp_ctx_cluster_stats(self.h, out), "cluster_stats")
         return tuple(out)
 
+    def close(self):
+        for ref in reversed(getattr(self, "_children", [])):
+            obj = ref()
+            if obj is not None:
+                obj.close()
+        self._children = []
+        h = getattr(self, "h", None)
+        if h and _lib is not None:
+            _lib.itp_ctx_destroy(h)
+        self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
     def __del__(self):
-        if getattr(self, "h", None) and _lib is not None:
-            _lib.itp_ctx_destroy(self.h)
-            self.h = None
+        if sys.is_finalizing():
+            return
+        self.close()
 
 
-class Grid:
+class Grid(_Owned):
     """DataLayout + Transformer (datalayout.hpp:30-54, transformer.hpp:30-63)."""
+
+    _destroy = "itp_grid_destroy"
 
     def __init__(self, ctx, sx, sy, dx, boundary=PERIODIC):
         self.ctx, self.sx, self.sy, self.dx, self.boundary = ctx, sx, sy, dx, boundary
         self.h = _vp()
         ctx.check(lib().itp_grid_create(ctx.h, sx, sy, dx, boundary, C.byref(self.h)), "itp_grid_create")
+        self._register(ctx)
 
     def kvalues(self):
         a = [np.zeros(n) for n in (self.sx, self.sy, self.sx, self.sy)]
@@ -312,10 +372,6 @@ class Grid:
                                                         f.shape[0]), "expkinetic")
         return f
 
-    def __del__(self):
-        if getattr(self, "h", None) and _lib is not None:
-            _lib.itp_grid_destroy(self.h)
-            self.h = None
 
 
 POTENTIAL_TYPES = ["zero", "harmonic", "elliptic", "prettyhardsquare", "softpentagon", "henonheiles", "gaussian",
@@ -335,8 +391,10 @@ def potential_type_params(name, params=()):
     return kind.value, out
 
 
-class Potential:
+class Potential(_Owned):
     """Potential (potential.hpp:31-58); V=None is the null potential."""
+
+    _destroy = "itp_potential_destroy"
 
     def __init__(self, grid, V=None, _handle=None):
         self.grid = grid
@@ -344,9 +402,11 @@ class Potential:
         if _handle is not None:
             self.h = _handle
             self.V = None
+            self._register(grid.ctx)
             return
         self.V = None if V is None else np.ascontiguousarray(V, dtype=np.float64)
         grid.ctx.check(lib().itp_potential_create(grid.h, _p(self.V), C.byref(self.h)), "itp_potential_create")
+        self._register(grid.ctx)
 
     @classmethod
     def of_type(cls, grid, name, params=()):
@@ -382,19 +442,18 @@ class Potential:
                                                               _p(f.view(np.float64)), f.shape[0]), "secondorder")
         return f
 
-    def __del__(self):
-        if getattr(self, "h", None) and _lib is not None:
-            _lib.itp_potential_destroy(self.h)
-            self.h = None
 
 
-class StateSet:
+class StateSet(_Owned):
     """StateSet (stateset.hpp): N fields on the device, FP64 or FP32."""
+
+    _destroy = "itp_stateset_destroy"
 
     def __init__(self, grid, N, precision=FP64):
         self.grid, self.N, self.precision = grid, N, precision
         self.h = _vp()
         grid.ctx.check(lib().itp_stateset_create(grid.h, N, precision, C.byref(self.h)), "itp_stateset_create")
+        self._register(grid.ctx)
 
     def upload(self, states):
         s = _c128(states)
@@ -432,20 +491,19 @@ class StateSet:
         self.grid.ctx.check(lib().itp_overlap(self.h, _p(out.view(np.float64))), "overlap")
         return out.reshape(self.N, self.N).T  # column-major -> S[i, j]
 
-    def __del__(self):
-        if getattr(self, "h", None) and _lib is not None:
-            _lib.itp_stateset_destroy(self.h)
-            self.h = None
 
 
-class MultiProductSplit:
+class MultiProductSplit(_Owned):
     """The evolution operator T (multiproductsplit.cpp:23-58)."""
+
+    _destroy = "itp_split_destroy"
 
     def __init__(self, potential, halforder, eps, B=0.0):
         self.pot = potential
         self.h = _vp()
         potential.grid.ctx.check(lib().itp_split_create(potential.h, halforder, eps, B, C.byref(self.h)),
                                  "itp_split_create")
+        self._register(potential.grid.ctx)
 
     @property
     def halforder(self):
@@ -457,20 +515,19 @@ class MultiProductSplit:
     def propagate(self, states):
         self.pot.grid.ctx.check(lib().itp_propagate(self.h, states.h), "propagate")
 
-    def __del__(self):
-        if getattr(self, "h", None) and _lib is not None:
-            _lib.itp_split_destroy(self.h)
-            self.h = None
 
 
-class Hamiltonian:
+class Hamiltonian(_Owned):
     """H = T + V (itpsystem.cpp:117-119)."""
+
+    _destroy = "itp_hamiltonian_destroy"
 
     def __init__(self, potential, B=0.0):
         self.pot = potential
         self.h = _vp()
         potential.grid.ctx.check(lib().itp_hamiltonian_create(potential.h, B, C.byref(self.h)),
                                  "itp_hamiltonian_create")
+        self._register(potential.grid.ctx)
 
     def energies(self, states):
         E = np.zeros(states.N)
@@ -497,10 +554,6 @@ class Hamiltonian:
                                 "hamiltonian")
         return f
 
-    def __del__(self):
-        if getattr(self, "h", None) and _lib is not None:
-            _lib.itp_hamiltonian_destroy(self.h)
-            self.h = None
 
 
 def eigh(ctx, A):
@@ -541,8 +594,10 @@ def _params(cfg):
     return p, eps
 
 
-class ITPSystem:
+class ITPSystem(_Owned):
     """The ITP driver (itpsystem.cpp) on the GPU engine."""
+
+    _destroy = "itps_destroy"
 
     def __init__(self, ctx, cfg, V=None, init_states=None):
         self.ctx, self.cfg = ctx, cfg
@@ -553,6 +608,7 @@ class ITPSystem:
         rc = lib().itps_create(ctx.h, C.byref(self._params), _p(self._V),
                                None if self._init is None else _p(self._init.view(np.float64)), C.byref(self.h))
         ctx.check(rc, "itps_create")
+        self._register(ctx)
 
     def step(self):
         return lib().itps_step(self.h)
@@ -642,10 +698,6 @@ class ITPSystem:
         self.ctx.check(lib().itps_states(self.h, _p(out.view(np.float64))), "itps_states")
         return out
 
-    def __del__(self):
-        if getattr(self, "h", None) and _lib is not None:
-            _lib.itps_destroy(self.h)
-            self.h = None
 
 
 class Batch:
@@ -700,7 +752,13 @@ class Batch:
             raise ITPError(rc, "itp_batch_sorted")
         return E, sd, f.astype(bool)
 
+    def close(self):
+        h = getattr(self, "h", None)
+        if h and _lib is not None:
+            _lib.itp_batch_destroy(h)
+        self.h = None
+
     def __del__(self):
-        if getattr(self, "h", None) and _lib is not None:
-            _lib.itp_batch_destroy(self.h)
-            self.h = None
+        if sys.is_finalizing():
+            return
+        self.close()

@@ -44,14 +44,18 @@ FLAGS = [
 
 def build(verbose=False):
     os.makedirs(LIB_DIR, exist_ok=True)
-    objs = []
+    objs, procs = [], []
+    # the translation units compile in parallel (one process each)
     for src in SOURCES:
         obj = os.path.join(LIB_DIR, os.path.basename(src) + ".o")
         cmd = [HIPCC] + FLAGS + ["-I", os.path.join(ROOT, "include"), "-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        procs.append((src, subprocess.Popen(cmd)))
         objs.append(obj)
+    failed = [src for src, p in procs if p.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, "hipcc " + " ".join(failed))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs
     if verbose:

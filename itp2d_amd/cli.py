@@ -198,10 +198,9 @@ def parse(argv):
 
 def run(cfg, opts, device=0):
     """Builds the potential, runs the driver and saves; returns the exit status."""
-    import numpy as np
-
     from . import api as I
     from . import datafile
+    ctx = None
     try:
         ctx = I.Context(device)
         dx = cfg["lenx"] / cfg["sx"]
@@ -213,6 +212,8 @@ def run(cfg, opts, device=0):
         s.set_verbosity(max(0, min(3, opts["verbosity"])))
     except Exception as e:  # set-up failed (itp2d.cpp:98-106)
         print(f"Error during initialization: {e}", file=sys.stderr)
+        if ctx is not None:
+            ctx.close()
         return 3
     attrs = {"potential": opts["potential_string"], "noise": "none", "impurity_type": "N/A",
              "impurity_distribution": "N/A", "impurity_constraint": opts["impurity_constraint"],
@@ -233,8 +234,11 @@ def run(cfg, opts, device=0):
                                   save="energies" if opts["save"] == "energies" else "final_states")
     except Exception as e:  # the run failed (itp2d.cpp:113-121)
         print(f"Error during the run: {e}", file=sys.stderr)
+        ctx.close()
         return 4
-    return 1 if s.error_flag else 0
+    status = 1 if s.error_flag else 0
+    ctx.close()  # frees the run's device objects before the interpreter exits
+    return status
 
 
 def main(argv=None):

@@ -36,12 +36,13 @@ struct BlockJacobiArgs {
   double* slots;         // [2 (max_sweeps + 1) + 1]: per whole-matrix check, the largest element ratio,
                          // then (second half) the largest coupling-to-gap ratio (atomic max);
                          // the last 8 bytes hold the grid barrier's counter and error flag
-  double* info;          // [0] sweeps, [1] 1 converged / 2 handed off / 0 neither
+  double* info;          // [0] sweeps, [1] 1 converged / 2 handed off / 0 neither / -1 grid barrier timed out
   int np;
   int max_sweeps;
   int max_inner;
   double tol;
   double handoff;        // stop early once every unconverged |a_ij| < handoff |a_ii - a_jj| (0: never)
+  unsigned spin_limit;   // polls of a grid barrier before it gives up (0: give up unless last to arrive)
 };
 
 // LDS for one workgroup: 4 JS x JS complex matrices + rotation parameters +

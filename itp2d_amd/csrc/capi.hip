@@ -43,6 +43,7 @@ struct itp_ctx {
   double eig_handoff = 0.5;   // hand-off threshold on |a_ij| / |a_ii - a_jj|
   double eig_cluster = 0.1;   // cluster start: pairs above this |a_ij| / |a_ii - a_jj| are solved exactly (0: off)
   int fp32_rotation = 1;      // FP32 sets' rotation: 0 FP32 MFMA, 1 split-BF16x3 MFMA
+  unsigned eig_spin_limit = 1u << 24;  // polls of a block-Jacobi grid barrier before it reports a timeout
   long long oa_done = 0, oa_fallback = 0;
   long long cl_calls = 0, cl_clusters = 0, cl_largest = 0, cl_fallback = 0;
 };
@@ -422,6 +423,7 @@ int strong_clusters(const std::vector<int>& pairs, int N, std::vector<int>& memb
 // Result of one eigensolver attempt, read back after the stream drained.
 struct EigRead {
   double info[4] = {0, 0, 0, 0};
+  unsigned barrier_err = 0;
   int state[2] = {3, 0};
   int bad = 0;
 };
@@ -544,15 +546,26 @@ int eigen_stage(itp_stateset* s, size_t* bad_n, double* bad_lambda, bool scale, 
     if (done) c->oa_done++;
     else if (rd.state[0] != 3) try_clusters = false;  // its refinement failed: no more attempts
   }
-  while (!done && sweeps < kMaxSweeps) {
+  // every launch either sweeps, converges, hands off or fails; the launch
+  // count bounds the loop as well
+  for (int launches = 0; !done && sweeps < kMaxSweeps && launches <= kMaxSweeps; launches++) {
     // one sweep per launch while cluster starts are tried, else up to the budget
     {
       std::unique_lock<std::mutex> coop(coop_mutex());
-      HIPCK(c, launch_block_jacobi(s->A[0], s->V[0], s->bjQ, s->bjD, s->bjident, s->bjslots, s->np,
-                                   try_clusters ? 1 : kMaxSweeps - sweeps, c->eig_inner, 1e-14,
-                                   handoff ? c->eig_handoff : 0.0, s->info, c->eig_grid, c->stream));
+      const int budget = try_clusters ? 1 : kMaxSweeps - sweeps;
+      HIPCK(c, launch_block_jacobi(s->A[0], s->V[0], s->bjQ, s->bjD, s->bjident, s->bjslots, s->np, budget,
+                                   c->eig_inner, 1e-14, handoff ? c->eig_handoff : 0.0, s->info, c->eig_grid,
+                                   c->eig_spin_limit, c->stream));
       HIPCK(c, hipMemcpyAsync(rd.info, s->info, sizeof(rd.info), hipMemcpyDeviceToHost, c->stream));
+      // the grid barrier's error flag (after the per-check maxima and the arrival counter)
+      HIPCK(c, hipMemcpyAsync(&rd.barrier_err, (const char*)(s->bjslots + 2 * (budget + 1)) + sizeof(unsigned),
+                              sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
       HIPCK(c, hipStreamSynchronize(c->stream));
+    }
+    if (rd.barrier_err || rd.info[1] < 0) {
+      // a grid barrier gave up waiting: A and V are in an unknown state
+      c->eig_calls++;
+      return set_err(c, ITP_E_EIGENSOLVER, "block Jacobi eigensolver: a grid barrier timed out");
     }
     sweeps += (int)rd.info[0];
     c->eig_sweeps += (long long)rd.info[0];
@@ -875,6 +888,12 @@ int itp_ctx_set_fp32_rotation(itp_ctx* c, int kind) {
   return ITP_OK;
 }
 
+int itp_ctx_set_eig_spin_limit(itp_ctx* c, unsigned long long polls) {
+  if (!c || polls > 0xffffffffull) return ITP_E_INVALID;
+  c->eig_spin_limit = (unsigned)polls;
+  return ITP_OK;
+}
+
 int itp_ctx_set_eig_grid(itp_ctx* c, int workgroups) {
   if (!c || workgroups < 0 || workgroups > 4096) return ITP_E_INVALID;
   c->eig_grid = workgroups;
@@ -1100,6 +1119,8 @@ int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out
   if (!g || !out || N < 1 || (precision != ITP_FP64 && precision != ITP_FP32)) return ITP_E_INVALID;
   itp_ctx* c = g->ctx;
   if (N > 4096) return set_err(c, ITP_E_UNSUPPORTED, "more than 4096 states per set is not supported");
+  if (precision == ITP_FP32 && (g->g.M % 2))
+    return set_err(c, ITP_E_UNSUPPORTED, "FP32 sets need an even number of grid points (sx * sy)");
   if (N > 65535) return set_err(c, ITP_E_UNSUPPORTED, "too many states");
   return guarded(c, [&]() -> int {
     std::unique_ptr<itp_stateset> s(new itp_stateset);

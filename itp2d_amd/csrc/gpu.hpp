@@ -48,7 +48,7 @@ hipError_t launch_residual(const cplx<R>* W, const cplx<R>* Psi, const double* E
 // grid_wgs <= 0 sizes the cooperative grid by the matrix
 hipError_t launch_block_jacobi(cplx<double>* A, cplx<double>* V, cplx<double>* Q, cplx<double>* D, int* ident,
                                double* slots, int np, int max_sweeps, int max_inner, double tol, double handoff,
-                               double* info, int grid_wgs, hipStream_t st);
+                               double* info, int grid_wgs, unsigned spin_limit, hipStream_t st);
 // Refinement of the eigenvector estimate X0 of A0: X0 is overwritten only on
 // convergence (iterates go to X1 and to work's fifth np x np slot); work holds
 // 5 np^2 complex values. x0_identity: non-null (an identity matrix) when X0

@@ -407,25 +407,34 @@ template hipError_t launch_gram<float>(const cplx<float>*, const cplx<float>*, l
 // ---------------------------------------------------------------------------
 // Grid-wide barrier of a cooperative launch: an arrival counter (zeroed
 // before the launch) with agent-scope release/acquire fences. Barrier number
-// k of every workgroup waits for k * gridDim.x arrivals. A spin bound turns a
-// barrier that never completes into an error flag instead of a hang.
-__device__ __forceinline__ void grid_barrier(unsigned* cnt, unsigned& epoch, unsigned* err) {
+// k of every workgroup waits for k * gridDim.x arrivals. A workgroup that is
+// not the last to arrive polls the counter at most `limit` times; past that
+// it raises the error flag instead of waiting on. Returns true when the flag
+// is raised (by this or any other workgroup): the caller then leaves the
+// kernel, and every workgroup still waiting on a later barrier gives up after
+// its own bounded wait, so the launch always drains.
+__device__ __forceinline__ bool grid_barrier(unsigned* cnt, unsigned& epoch, unsigned* err, unsigned limit) {
+  __shared__ int failed;
   __syncthreads();
   const unsigned target = ++epoch * gridDim.x;
   if (threadIdx.x == 0) {
     __threadfence();
-    atomicAdd(cnt, 1u);
-    unsigned spins = 0;
-    while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      __builtin_amdgcn_s_sleep(1);
-      if (++spins > (1u << 24)) {
-        atomicOr(err, 1u);
-        break;
+    const unsigned arrived = atomicAdd(cnt, 1u) + 1u;
+    if (arrived < target) {
+      for (unsigned spins = 0;; spins++) {
+        if (spins >= limit) {
+          atomicOr(err, 1u);
+          break;
+        }
+        if (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
+        __builtin_amdgcn_s_sleep(1);
       }
     }
     __threadfence();
+    failed = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
   }
   __syncthreads();
+  return failed != 0;
 }
 
 __global__ __launch_bounds__(256) void block_jacobi_kernel(BlockJacobiArgs a) {
@@ -443,7 +452,7 @@ __global__ __launch_bounds__(256) void block_jacobi_kernel(BlockJacobiArgs a) {
   const long long e0 = (long long)blockIdx.x * per;
   const long long e1 = e0 + per < tot ? e0 + per : tot;
   int sweeps = 0;
-  bool conv = false, handed = false;
+  bool conv = false, handed = false, failed = false;
   for (int chk = 0;; chk++) {
     // whole-matrix convergence check
     unsigned long long* slot = reinterpret_cast<unsigned long long*>(a.slots + chk);
@@ -453,7 +462,7 @@ __global__ __launch_bounds__(256) void block_jacobi_kernel(BlockJacobiArgs a) {
       atomicMax(slot, dbits(s.red[nt]));
       atomicMax(gslot, dbits(s.red[nt + 1]));
     }
-    grid_barrier(bar, epoch, bar_err);
+    if ((failed = grid_barrier(bar, epoch, bar_err, a.spin_limit))) break;
     const double worst = bitsd(__hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     const double rho = bitsd(__hip_atomic_load(gslot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     if (worst < 1.0) { conv = true; break; }
@@ -461,18 +470,22 @@ __global__ __launch_bounds__(256) void block_jacobi_kernel(BlockJacobiArgs a) {
     if (chk == a.max_sweeps) break;
     // one sweep of nb-1 rounds
     bj_reset_positions(ex, s, nb);
-    for (int r = 0; r < nb - 1; r++) {
+    for (int r = 0; r < nb - 1 && !failed; r++) {
       for (int k = blockIdx.x; k < P; k += gridDim.x) bj_inner(ex, a, k, s.pos2blk[2 * k], s.pos2blk[2 * k + 1], s);
-      grid_barrier(bar, epoch, bar_err);
+      if ((failed = grid_barrier(bar, epoch, bar_err, a.spin_limit))) break;
       for (int t = blockIdx.x; t < items; t += gridDim.x) bj_apply(ex, a, t, s);
-      grid_barrier(bar, epoch, bar_err);
+      if ((failed = grid_barrier(bar, epoch, bar_err, a.spin_limit))) break;
       bj_next_positions(ex, s, nb);
     }
+    if (failed) break;
     sweeps++;
   }
+  // the host also reads the error flag itself after the launch has drained:
+  // a workgroup that timed out after workgroup 0 wrote these words still
+  // marks the call as failed
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     a.info[0] = sweeps;
-    a.info[1] = (*bar_err != 0) ? 0.0 : (conv ? 1.0 : (handed ? 2.0 : 0.0));
+    a.info[1] = failed ? -1.0 : (conv ? 1.0 : (handed ? 2.0 : 0.0));
     a.info[2] = 0;
   }
 }
@@ -483,11 +496,12 @@ __global__ void fill_u64_kernel(unsigned long long* p, long long n) {
 
 hipError_t launch_block_jacobi(cplx<double>* A, cplx<double>* V, cplx<double>* Q, cplx<double>* D, int* ident,
                                double* slots, int np, int max_sweeps, int max_inner, double tol, double handoff,
-                               double* info, int grid_wgs, hipStream_t st) {
+                               double* info, int grid_wgs, unsigned spin_limit, hipStream_t st) {
   if (np % JS || np / JB > kMaxBlocks) return hipErrorInvalidValue;
   BlockJacobiArgs a;
   a.A = A; a.V = V; a.Q = Q; a.D = D; a.ident = ident; a.slots = slots; a.info = info;
   a.np = np; a.max_sweeps = max_sweeps; a.max_inner = max_inner; a.tol = tol; a.handoff = handoff;
+  a.spin_limit = spin_limit;
   // the two maxima per check, then the barrier counter and its error flag
   hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(256), 0, st, reinterpret_cast<unsigned long long*>(slots),
                      (long long)(2 * (max_sweeps + 1) + 1));

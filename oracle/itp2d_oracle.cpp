@@ -1133,6 +1133,39 @@ int orc_system_sorted(void* s, double* E, double* sigma, size_t* index) {
   }
   return int(sys->esn.size());
 }
+// State pointwise and BLAS-1 operations on one field (state.hpp:180-240):
+// kind 0 pointwise_multiply, 1 pointwise_multiply_imaginary_shiftx,
+// 2 pointwise_multiply_y, 3 pointwise_multiply_and_add (with addstate).
+int orc_pointwise(int kind, size_t sx, size_t sy, double* state, const double* values, const double* addstate) {
+  cd* s = reinterpret_cast<cd*>(state);
+  const size_t M = sx * sy;
+  switch (kind) {
+    case 0: pw_mul(s, values, M); return 0;
+    case 1: pw_mul_imag_shiftx(s, values, sx, sy); return 0;
+    case 2: pw_mul_y(s, values, sx, sy); return 0;
+    case 3: {
+      if (!addstate) return 1;
+      const cd* a = reinterpret_cast<const cd*>(addstate);
+      pw_mul(s, values, M);
+      axpy(s, a, M);
+      return 0;
+    }
+    default: return 1;
+  }
+}
+// State::dot (out[0], out[1]) and State::norm (return value)
+double orc_dot_norm(size_t M, double dx, const double* a, const double* b, double* out) {
+  const cd* x = reinterpret_cast<const cd*>(a);
+  const cd z = dot(x, reinterpret_cast<const cd*>(b), M, dx);
+  out[0] = z.real();
+  out[1] = z.imag();
+  return norm2(x, M, dx);
+}
+int orc_system_converged(void* s, int* flags) {
+  System* sys = static_cast<System*>(s);
+  for (size_t n = 0; n < sys->esn.size(); n++) flags[n] = sys->fin_conv[std::get<2>(sys->esn[n])] ? 1 : 0;
+  return int(sys->esn.size());
+}
 int orc_system_states(void* s, double* out) {
   System* sys = static_cast<System*>(s);
   std::memcpy(out, sys->states.data(), sys->states.size() * sizeof(cd));

@@ -140,6 +140,12 @@ size_t orc_system_num_converged(void* sys);
 size_t orc_system_num_timestep_converged(void* sys);
 // sorted energies / deviations / slot indices of the latest step (length N)
 int orc_system_sorted(void* sys, double* E, double* sigma, size_t* index);
+// State pointwise operations (kind: 0 multiply, 1 imaginary multiply with the
+// x shift, 2 multiply by a y-table, 3 multiply and add) and dot/norm
+int orc_pointwise(int kind, size_t sx, size_t sy, double* state, const double* values, const double* addstate);
+double orc_dot_norm(size_t M, double dx, const double* a, const double* b, double* out);
+// final-convergence flag of each sorted position
+int orc_system_converged(void* sys, int* flags);
 int orc_system_states(void* sys, double* out);
 double orc_system_total_time(void* sys);
 int orc_system_history_len(void* sys);

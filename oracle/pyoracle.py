@@ -94,6 +94,10 @@ def _setup(L):
     L.orc_system_num_timestep_converged.restype = C.c_size_t
     L.orc_system_sorted.argtypes = [C.c_void_p, _dp, _dp, C.POINTER(_sz)]
     L.orc_system_states.argtypes = [C.c_void_p, _dp]
+    L.orc_system_converged.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+    L.orc_pointwise.argtypes = [C.c_int, _sz, _sz, _dp, _dp, _dp]
+    L.orc_dot_norm.argtypes = [_sz, C.c_double, _dp, _dp, _dp]
+    L.orc_dot_norm.restype = C.c_double
     L.orc_set_num_threads.argtypes = [C.c_int]
 
 
@@ -251,6 +255,28 @@ def energies(states, dx, boundary, B, V):
     return E, sd, mc.view(np.complex128), ms.view(np.complex128)
 
 
+POINTWISE = {"multiply": 0, "multiply_imaginary_shiftx": 1, "multiply_y": 2, "multiply_and_add": 3}
+
+
+def pointwise(state, kind, values, addstate=None):
+    """A State pointwise operation (state.hpp:180-211) on one [sy][sx] field; returns the new field."""
+    s = _c128(state).copy()
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    a = None if addstate is None else _c128(addstate)
+    sy, sx = s.shape
+    _check(lib().orc_pointwise(POINTWISE[kind], sx, sy, _ptr(s.view(np.float64)), _ptr(v),
+                               None if a is None else _ptr(a.view(np.float64))), "pointwise")
+    return s
+
+
+def dot_norm(a, b, dx):
+    """(State::dot(a, b), State::norm(a)) (state.hpp:213-240)."""
+    x, y = _c128(a), _c128(b)
+    out = np.zeros(2)
+    nrm = lib().orc_dot_norm(x.size, dx, _ptr(x.view(np.float64)), _ptr(y.view(np.float64)), _ptr(out))
+    return complex(out[0], out[1]), nrm
+
+
 def how_orthonormal(states, dx):
     s = _c128(states)
     n, sy, sx = s.shape
@@ -337,6 +363,12 @@ class System:
         idx = (_sz * N)()
         lib().orc_system_sorted(self.h, _ptr(E), _ptr(sd), idx)
         return E, sd, np.array(list(idx))
+
+    def converged(self):
+        """Final-convergence flag of each sorted position."""
+        f = np.zeros(int(self.cfg["N"]), dtype=np.int32)
+        lib().orc_system_converged(self.h, f.ctypes.data_as(C.POINTER(C.c_int)))
+        return f.astype(bool)
 
     def states(self):
         N, sy, sx = self.cfg["N"], self.cfg["sy"], self.cfg["sx"]

@@ -16,4 +16,6 @@ def pytest_configure(config):
 def ctx():
     import itp2d_amd
     print(f"[gpu] creating context on device 0", flush=True)
-    return itp2d_amd.Context(0)
+    c = itp2d_amd.Context(0)
+    yield c
+    c.close()

@@ -163,6 +163,30 @@ def test_eigh(ctx, n):
     assert np.abs(V @ np.diag(w) @ V.conj().T - A).max() < 1e-12 * np.abs(A).max()
 
 
+def test_eigh_barrier_timeout_is_an_error(ctx):
+    """The block-Jacobi grid barrier's bounded wait, forced from the host: with
+    a spin limit of 0 every workgroup but the last to arrive gives up at once,
+    every workgroup leaves the launch, and the call reports EigensolverError
+    (not "not converged yet", which would relaunch). The context then works as
+    before."""
+    say("eigh barrier timeout")
+    n = 200
+    rng = np.random.default_rng(77)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    A = A + A.conj().T
+    ctx.set_eig_mode(1)
+    try:
+        ctx.set_eig_spin_limit(0)
+        with pytest.raises(I.EigensolverError) as e:
+            I.eigh(ctx, A)
+        assert "timed out" in str(e.value)
+    finally:
+        ctx.set_eig_spin_limit(1 << 24)
+        ctx.set_eig_mode(0)
+    w, V = I.eigh(ctx, A)
+    assert np.abs(w - np.linalg.eigvalsh(A)).max() < 1e-12 * np.abs(w).max()
+
+
 @pytest.mark.parametrize("n,cluster", [(100, False), (300, False), (120, True)])
 def test_eigh_refinement(ctx, n, cluster):
     """Nearly diagonal Hermitian matrices like the late overlap matrices of an
