@@ -146,6 +146,19 @@ int itp_split_halforder(const itp_split* sp);
  * Operator::operate (operators.hpp:33-52) batched over the whole set. */
 int itp_propagate(itp_split* sp, itp_stateset* s);
 
+/* itp_propagate without any allocation: no counterpart in the reference. For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The intermediate
+ * fields live in a fixed device scratch array of
+ * itp_propagate_noalloc_max_bytes() bytes (64 MiB) that each device holds for
+ * the life of the process; a set needing more (N * sx * sy * 16 bytes for
+ * ITP_FP64, 8 for ITP_FP32) is rejected with ITP_E_UNSUPPORTED before any
+ * work is queued. Results equal itp_propagate's. Every call on one device
+ * shares that device's array: calls that overlap on one device (two streams,
+ * two host threads) race. */
+int itp_propagate_noalloc(itp_split* sp, itp_stateset* s);
+size_t itp_propagate_noalloc_max_bytes(void);
+
 /* ---- StateSet::orthonormalize (stateset.cpp:136-232): overlap matrix,
  * Hermitian eigendecomposition, eigenvalue checks, Psi <- Psi U Lambda^(-1/2).
  * On ITP_E_NONPOSITIVE_EIG / ITP_E_NONNORMAL_EIG the states are left as they

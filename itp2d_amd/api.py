@@ -113,6 +113,8 @@ def lib():
             "itp_split_set_time_step": [_vp, C.c_double],
             "itp_split_halforder": [_vp],
             "itp_propagate": [_vp, _vp],
+            "itp_propagate_noalloc": [_vp, _vp],
+            "itp_propagate_noalloc_max_bytes": [],
             "itp_orthonormalize": [_vp, C.POINTER(_sz), _dp],
             "itp_overlap": [_vp, _dp],
             "itp_hamiltonian_create": [_vp, C.c_double, C.POINTER(_vp)],
@@ -171,6 +173,7 @@ def lib():
         L.itp_grid_normalization_factor.restype = C.c_double
         L.itp_stateset_device_ptr.restype = _vp
         L.itp_stateset_size.restype = _sz
+        L.itp_propagate_noalloc_max_bytes.restype = _sz
         L.itps_eps.restype = C.c_double
         L.itps_total_time.restype = C.c_double
         L.itps_num_converged.restype = _sz
@@ -515,6 +518,11 @@ class MultiProductSplit(_Owned):
     def propagate(self, states):
         self.pot.grid.ctx.check(lib().itp_propagate(self.h, states.h), "propagate")
 
+    def propagate_noalloc(self, states):
+        """propagate() with its intermediate fields in the library's fixed device scratch array
+        (no allocation; sets larger than noalloc_max_bytes() are rejected)."""
+        self.pot.grid.ctx.check(lib().itp_propagate_noalloc(self.h, states.h), "propagate_noalloc")
+
 
 
 class Hamiltonian(_Owned):
@@ -554,6 +562,11 @@ class Hamiltonian(_Owned):
                                 "hamiltonian")
         return f
 
+
+
+def noalloc_max_bytes():
+    """Size of the fixed device scratch array of propagate_noalloc."""
+    return lib().itp_propagate_noalloc_max_bytes()
 
 
 def eigh(ctx, A):

@@ -24,6 +24,7 @@ using namespace itp;
 struct itp_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  void* scratch = nullptr;  // this device's address of the fixed scratch array
   std::string err;
   int num_cus = 0;
   void* ws = nullptr;
@@ -129,7 +130,33 @@ struct itp_hamiltonian {
   KinHDev d;
 };
 
+// Fixed device scratch of itp_propagate_noalloc: the intermediate fields of
+// one propagation for callers that cannot allocate. Each device holds its own
+// copy for the life of the process; its address on a device comes from
+// hipGetSymbolAddress (the array's name is only a host-side stand-in).
+constexpr size_t kNoAllocScratchBytes = 64ull << 20;
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Address of the scratch array on `device`, looked up once per device with
+// that device current (called from itp_ctx_create).
+hipError_t scratch_address(int device, void** out) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = ptr;
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
 
 constexpr int kMaxSweeps = 40;
 constexpr int kOAMaxIter = 8;
@@ -304,19 +331,32 @@ int build_split_tables(itp_split* sp) {
   return ITP_OK;
 }
 
+// One propagation of the whole set. The intermediate fields go to the
+// context's workspace (grown with hipMalloc when needed) or, for the
+// no-allocation entry point, to the fixed scratch array.
 template <class R>
-int propagate_impl(itp_split* sp, itp_stateset* s) {
+int propagate_impl(itp_split* sp, itp_stateset* s, bool noalloc = false) {
   itp_grid* gr = s->grid;
   itp_ctx* c = gr->ctx;
   const GridInfo& g = gr->g;
   const size_t fbytes = s->N * (size_t)g.M * sizeof(cplx<R>);
-  int rc = ensure_ws(c, fbytes);
-  if (rc) return rc;
+  cplx<R>* work = nullptr;
+  if (noalloc) {
+    if (fbytes > kNoAllocScratchBytes)
+      return set_err(c, ITP_E_UNSUPPORTED, "itp_propagate_noalloc: the set needs " + std::to_string(fbytes) +
+                                               " bytes of scratch, more than the fixed " +
+                                               std::to_string(kNoAllocScratchBytes));
+    if (!c->scratch) return set_err(c, ITP_E_HIP, "itp_propagate_noalloc: no scratch address for this device");
+    work = (cplx<R>*)c->scratch;
+  } else {
+    int rc = ensure_ws(c, fbytes);
+    if (rc) return rc;
+    work = (cplx<R>*)c->ws;
+  }
   const Branch br = branch_of(sp->B, g.bt);
   const bool has_v = !sp->pot->null;
   const cplx<R>* in = (const cplx<R>*)s->buf[s->cur];
   cplx<R>* out = (cplx<R>*)s->buf[s->cur ^ 1];
-  cplx<R>* work = (cplx<R>*)c->ws;
   std::vector<Sweep<R>> prog;
   for (int t = 1; t <= sp->halforder; t++)
     member_program<R>(br, sp->members[t - 1]->d, has_v ? t : 1, has_v, in, work, out, t > 1, prog);
@@ -795,6 +835,7 @@ int itp_ctx_create(int device, itp_ctx** out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) return ITP_E_HIP;
   c->num_cus = prop.multiProcessorCount;
+  if (scratch_address(device, &c->scratch) != hipSuccess) return ITP_E_HIP;
   c->evpool.resize(2048);
   for (auto& e : c->evpool)
     if (hipEventCreate(&e) != hipSuccess) return ITP_E_HIP;
@@ -1300,6 +1341,16 @@ int itp_propagate(itp_split* sp, itp_stateset* s) {
   itp_ctx* c = s->grid->ctx;
   return guarded(c, [&]() -> int { return s->prec == ITP_FP64 ? propagate_impl<double>(sp, s) : propagate_impl<float>(sp, s); });
 }
+
+int itp_propagate_noalloc(itp_split* sp, itp_stateset* s) {
+  if (!sp || !s || sp->pot->grid != s->grid) return ITP_E_INVALID;
+  itp_ctx* c = s->grid->ctx;
+  return guarded(c, [&]() -> int {
+    return s->prec == ITP_FP64 ? propagate_impl<double>(sp, s, true) : propagate_impl<float>(sp, s, true);
+  });
+}
+
+size_t itp_propagate_noalloc_max_bytes(void) { return kNoAllocScratchBytes; }
 
 int itp_orthonormalize(itp_stateset* s, size_t* bad_n, double* bad_lambda) {
   if (!s) return ITP_E_INVALID;
