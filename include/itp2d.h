@@ -165,6 +165,25 @@ size_t itp_propagate_noalloc_max_bytes(void);
  * were and *bad_n, *bad_lambda (nullable) name the offending eigenvalue in
  * ascending order, like NonPositiveEigenvalue(n, eval, ...). ---- */
 int itp_orthonormalize(itp_stateset* s, size_t* bad_n, double* bad_lambda);
+/* Cholesky orthonormalization Psi <- Psi R^-1 with S = R^H R (no counterpart
+ * in the reference, which goes through the eigendecomposition of S,
+ * stateset.cpp:136-232): the same span as itp_orthonormalize, another basis
+ * (the CholeskyQR step of the mixed path; the states are not sorted by the
+ * overlap eigenvalues). When the factorization meets a pivot below 1e-5
+ * (ITP_FP32 sets) or 1e-12 (ITP_FP64) of its diagonal, the eigendecomposition
+ * route of itp_orthonormalize runs instead (*used_eigen = 1, nullable) with
+ * its error codes. */
+int itp_orthonormalize_cholesky(itp_stateset* s, int* used_eigen, size_t* bad_n, double* bad_lambda);
+/* Condition estimate of the overlap matrix of the last
+ * itp_orthonormalize_cholesky: max R_ii^2 / min R_ii^2 of its factor (0 if it
+ * fell back to the eigendecomposition or none ran). */
+int itp_stateset_cholesky_condition(const itp_stateset* s, double* kappa);
+/* FP64 Rayleigh quotients E[n] = Re<psi_n|H psi_n> / <psi_n|psi_n> of the
+ * states of src (either precision), evaluated in FP64 on a copy made into dst
+ * (an ITP_FP64 set of the same grid and size, whose fields are overwritten with
+ * src's). No counterpart in the reference: the mixed path's check of its FP32
+ * states against FP64 energies. */
+int itp_rayleigh_quotients(itp_hamiltonian* h, const itp_stateset* src, itp_stateset* dst, double* E);
 /* Overlap matrix S_ij = <psi_i|psi_j> as computed by itp_orthonormalize
  * (stateset.cpp:145-154); out is N*N complex, column-major, full Hermitian. */
 int itp_overlap(itp_stateset* s, double* out);
@@ -251,6 +270,9 @@ int itp_ctx_set_eig_grid(itp_ctx* ctx, int workgroups);
  * the call returns ITP_E_EIGENSOLVER with the states untouched. 0 makes every
  * workgroup but the last to arrive give up at once (used to test that path). */
 int itp_ctx_set_eig_spin_limit(itp_ctx* ctx, unsigned long long polls);
+/* Pivot threshold of itp_orthonormalize_cholesky relative to the diagonal
+ * (0, the default: 1e-5 for ITP_FP32 sets, 1e-12 for ITP_FP64). */
+int itp_ctx_set_cholesky_tol(itp_ctx* ctx, double tol);
 /* Eigensolver of the overlap matrix (replaces zheev, eigensolver.hpp:80-84):
  * 0 (default) block Jacobi sweeps until every unconverged off-diagonal element
  * is below half of its diagonal gap, then the refinement iteration

@@ -46,20 +46,38 @@ typedef struct itps_params {
   itps_test timestep_test, final_test;
   int precision;            /* ITP_FP64: the reference algorithm in FP64.
                                ITP_FP32: the mixed path (no counterpart in the
-                               reference): FP32 fields, split-BF16x3 overlap,
-                               FP64 eigensolver, energies estimated from the
-                               overlap eigenvalues, E_n = -ln(lambda_n) / (2 eps),
-                               until their relative change per step is below
-                               mixed_tol for the wanted states, or its largest
-                               value has stopped falling for 6 steps (the FP32
-                               rounding floor); then FP64 steps
-                               of the reference algorithm on an FP64 copy until
-                               the configured tests pass (the reference's energies
-                               belong to the fixed point of the split propagator,
-                               so parity needs that fixed point). Uses the first
-                               eps for the FP32 phase. */
-  double mixed_tol;         /* 0: 2e-7 */
+                               reference). FP32 phase: FP32 fields and sweeps,
+                               split-BF16x3 overlap, orthonormalization by
+                               CholeskyQR (FP64 Cholesky factor and inverse on
+                               the GPU, split-BF16x3 rotation), and every
+                               checked step (mixed_check_every) a Rayleigh-Ritz step
+                               (the overlap eigendecomposition, as the
+                               reference orthonormalizes) followed by the FP64
+                               Rayleigh quotients of the states
+                               (itp_rayleigh_quotients). The FP32 phase ends
+                               when those energies change by less than
+                               mixed_tol (relative) for every wanted state
+                               between consecutive checks, or stop improving for
+                               3 checks (the FP32 floor). Then the reference
+                               algorithm continues in FP64 on the checked FP64
+                               copy until the configured tests pass (the
+                               reference's energies belong to the fixed point of
+                               the split propagator, so parity needs that fixed
+                               point). Uses the first eps for the FP32 phase. */
+  double mixed_tol;         /* 0: the time-step test's tolerance if it is
+                               relchange, else 1e-9 */
   double accuracy;          /* error bound of itps_rayleigh_ritz; 0: 1e-8 */
+  int mixed_check_every;    /* FP32 steps per check of the mixed path; 0:
+                               adaptive (first check after 16 steps, then the
+                               step count predicted from the decay of the
+                               energy changes, within 2..8) */
+  int mixed_skip_ortho;     /* mixed path: 1 (default) lets a CholeskyQR step
+                               be skipped when mixed_tol >= 1e-10 and the last
+                               factor's condition per propagation step is
+                               below 30, so two
+                               propagations share one orthonormalization (the
+                               span is the same; the overlap's condition stays
+                               below ~1e3); -1 orthonormalizes every step */
 } itps_params;
 
 typedef struct itps_system itps_system;
@@ -89,6 +107,10 @@ double itps_total_time(const itps_system* s);
 int itps_recoveries(const itps_system* s);
 /* Steps of the mixed path's FP32 phase (included in itps_total_steps). */
 int itps_mixed_steps(const itps_system* s);
+/* out[4]: FP32-phase steps, FP64 checks made in it, CholeskyQR steps that fell
+ * back to the eigendecomposition route, steps whose orthonormalization was
+ * skipped. */
+int itps_mixed_stats(const itps_system* s, int* out);
 /* Abort flag polled at the start of every step (the reference's
  * abort_flagptr, itpsystem.cpp:407-416, raised by its SIGINT handler,
  * itp2d.cpp:35-60): when *flag is non-zero the run finishes with the error

@@ -63,7 +63,7 @@ class _Params(C.Structure):
                 ("exhaust_eps", C.c_int), ("max_steps", C.c_int), ("min_time_step", C.c_double),
                 ("recover", C.c_int), ("seed", C.c_uint64), ("timestep_test", _Test),
                 ("final_test", _Test), ("precision", C.c_int), ("mixed_tol", C.c_double),
-                ("accuracy", C.c_double)]
+                ("accuracy", C.c_double), ("mixed_check_every", C.c_int), ("mixed_skip_ortho", C.c_int)]
 
 
 def lib():
@@ -87,6 +87,7 @@ def lib():
             "itp_ctx_set_eig_inner": [_vp, C.c_int],
             "itp_ctx_set_eig_grid": [_vp, C.c_int],
             "itp_ctx_set_eig_spin_limit": [_vp, C.c_ulonglong],
+            "itp_ctx_set_cholesky_tol": [_vp, C.c_double],
             "itp_ctx_set_eig_mode": [_vp, C.c_int],
             "itp_ctx_set_eig_handoff": [_vp, C.c_double],
             "itp_ctx_oa_stats": [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
@@ -151,6 +152,10 @@ def lib():
             "itps_time_step_history": [_vp, C.POINTER(C.c_int), _dp, _sz],
             "itps_rayleigh_ritz": [_vp],
             "itps_mixed_steps": [_vp],
+            "itps_mixed_stats": [_vp, C.POINTER(C.c_int)],
+            "itp_orthonormalize_cholesky": [_vp, C.POINTER(C.c_int), C.POINTER(_sz), _dp],
+            "itp_rayleigh_quotients": [_vp, _vp, _vp, _dp],
+            "itp_stateset_cholesky_condition": [_vp, _dp],
             "itps_set_abort_flag": [_vp, C.POINTER(C.c_int)],
             "itps_set_verbosity": [_vp, C.c_int],
             "itp_potential_create_type": [_vp, C.c_char_p, _dp, _sz, C.POINTER(_vp)],
@@ -284,6 +289,10 @@ class Context:
         """Polls of a block-Jacobi grid barrier before the wait gives up (default 2**24); a wait that
         gives up ends the call with EigensolverError."""
         self.check(lib().itp_ctx_set_eig_spin_limit(self.h, int(polls)), "set_eig_spin_limit")
+
+    def set_cholesky_tol(self, tol):
+        """Pivot threshold of the Cholesky orthonormalization relative to the diagonal (0: default)."""
+        self.check(lib().itp_ctx_set_cholesky_tol(self.h, float(tol)), "set_cholesky_tol")
 
     def set_eig_mode(self, mode):
         """0: Jacobi handing off to the refinement iteration (default); 1: Jacobi only."""
@@ -474,6 +483,16 @@ class StateSet(_Owned):
         rc = lib().itp_orthonormalize(self.h, C.byref(n), C.byref(v))
         self.grid.ctx.check(rc, "orthonormalize", bad=(n.value, v.value))
 
+    def orthonormalize_cholesky(self):
+        """Cholesky orthonormalization Psi <- Psi R^-1 (S = R^H R); returns True when it fell back
+        to the eigendecomposition route."""
+        used = C.c_int(0)
+        n = _sz(0)
+        v = C.c_double(0)
+        rc = lib().itp_orthonormalize_cholesky(self.h, C.byref(used), C.byref(n), C.byref(v))
+        self.grid.ctx.check(rc, "orthonormalize_cholesky", bad=(n.value, v.value))
+        return bool(used.value)
+
     def assign(self, other):
         """Copy the fields of another set of the same grid and size (converting precision)."""
         self.grid.ctx.check(lib().itp_stateset_convert(self.h, other.h), "convert")
@@ -550,6 +569,13 @@ class Hamiltonian(_Owned):
         self.pot.grid.ctx.check(lib().itp_rayleigh_ritz(self.h, states.h, _p(E), _p(sd)), "rayleigh_ritz")
         return E, sd
 
+    def rayleigh_quotients(self, states, dst):
+        """FP64 Rayleigh quotients of the states (either precision), evaluated on a copy in dst
+        (an FP64 set of the same grid and size)."""
+        E = np.zeros(states.N)
+        self.pot.grid.ctx.check(lib().itp_rayleigh_quotients(self.h, states.h, dst.h, _p(E)), "rayleigh_quotients")
+        return E
+
     def energies_complex(self, states):
         m = np.zeros(2 * states.N)
         q = np.zeros(2 * states.N)
@@ -599,6 +625,8 @@ def _params(cfg):
     p.precision = int(cfg.get("precision", FP64))
     p.mixed_tol = float(cfg.get("mixed_tol", 0.0))
     p.accuracy = float(cfg.get("accuracy", 0.0))
+    p.mixed_check_every = int(cfg.get("mixed_check_every", 0))
+    p.mixed_skip_ortho = int(cfg.get("mixed_skip_ortho", 0))
     p.eps_values = _p(eps) if eps.size else None
     p.n_eps = eps.size
     for name in ["timestep_test", "final_test"]:
@@ -661,6 +689,13 @@ class ITPSystem(_Owned):
     def mixed_steps(self):
         """Steps of the mixed path's FP32 phase."""
         return lib().itps_mixed_steps(self.h)
+
+    def mixed_stats(self):
+        """(FP32-phase steps, FP64 checks, CholeskyQR steps that fell back to the eigendecomposition,
+        steps whose orthonormalization was skipped)."""
+        out = (C.c_int * 4)()
+        self.ctx.check(lib().itps_mixed_stats(self.h, out), "itps_mixed_stats")
+        return tuple(out)
 
     def set_abort_flag(self, flag):
         """Poll a ctypes.c_int at the start of every step; non-zero ends the run with the error flag

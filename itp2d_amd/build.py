@@ -22,6 +22,7 @@ HDF5_DIR = os.environ.get("HDF5_DIR", "/opt/conda")
 SOURCES = [
     os.path.join(HERE, "csrc", "kernels.hip"),
     os.path.join(HERE, "csrc", "mixed.hip"),
+    os.path.join(HERE, "csrc", "chol.hip"),
     os.path.join(HERE, "csrc", "capi.hip"),
     os.path.join(HERE, "host", "itpsystem.cpp"),
     os.path.join(HERE, "host", "batch.cpp"),

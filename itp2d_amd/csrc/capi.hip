@@ -45,6 +45,7 @@ struct itp_ctx {
   double eig_cluster = 0.1;   // cluster start: pairs above this |a_ij| / |a_ii - a_jj| are solved exactly (0: off)
   int fp32_rotation = 1;      // FP32 sets' rotation: 0 FP32 MFMA, 1 split-BF16x3 MFMA
   unsigned eig_spin_limit = 1u << 24;  // polls of a block-Jacobi grid barrier before it reports a timeout
+  double chol_tol = 0;        // Cholesky pivot threshold relative to the diagonal (0: 1e-5 FP32 sets, 1e-12 FP64)
   long long oa_done = 0, oa_fallback = 0;
   long long cl_calls = 0, cl_clusters = 0, cl_largest = 0, cl_fallback = 0;
 };
@@ -101,6 +102,12 @@ struct itp_stateset {
   int* perm = nullptr;
   double* info = nullptr;
   int* bad = nullptr;
+  // Cholesky orthonormalization: original diagonal, failure flag; field norms
+  double* diag0 = nullptr;
+  int* chol_fail = nullptr;
+  double* chol_mm = nullptr;     // largest and smallest R_ii^2 of the last factorization
+  double chol_kappa = 0;         // their ratio (0: no factorization yet or it failed)
+  double* norms = nullptr;
   // energies
   cplx<double>* epart = nullptr;  // [2][N][max(sx,sy)]
   double* dE = nullptr;           // E[N], sigma[N]
@@ -371,7 +378,8 @@ int propagate_impl(itp_split* sp, itp_stateset* s, bool noalloc = false) {
 }
 
 template <class R>
-int energies_impl(itp_hamiltonian* h, itp_stateset* s, double* E, double* sigma, double* mean, double* msq) {
+int energies_impl(itp_hamiltonian* h, itp_stateset* s, double* E, double* sigma, double* mean, double* msq,
+                  bool mean_only = false) {
   itp_grid* gr = s->grid;
   itp_ctx* c = gr->ctx;
   const GridInfo& g = gr->g;
@@ -390,13 +398,14 @@ int energies_impl(itp_hamiltonian* h, itp_stateset* s, double* E, double* sigma,
   std::vector<Sweep<R>> prog;
   Phase ph(c, ITP_PHASE_ENERGIES);
   const int nb = hamiltonian_program<R>(g, lg, br, h->d, V, psi, w1, w2, psi, p1, prog);
-  hamiltonian_program<R>(g, lg, br, h->d, V, w2, w1, nullptr, psi, p2, prog);
+  // mean_only: <H> alone (one application; sigma and <H^2> are then meaningless)
+  if (!mean_only) hamiltonian_program<R>(g, lg, br, h->d, V, w2, w1, nullptr, psi, p2, prog);
   GpuBackend be;
   be.st = c->stream;
   run_program<R>(be, g, lg, prog, (int)s->N);
   if (be.err != hipSuccess) return set_err(c, ITP_E_HIP, std::string("energies: ") + hipGetErrorString(be.err));
-  HIPCK(c, launch_energy_reduce(p1, p2, nb, (int)s->N, g.dx, s->dE, s->dE + s->N, s->dmean, s->dmean + s->N,
-                                c->stream));
+  HIPCK(c, launch_energy_reduce(p1, mean_only ? p1 : p2, nb, (int)s->N, g.dx, s->dE, s->dE + s->N, s->dmean,
+                                s->dmean + s->N, c->stream));
   ph.stop();
   if (E) HIPCK(c, hipMemcpyAsync(E, s->dE, s->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   if (sigma) HIPCK(c, hipMemcpyAsync(sigma, s->dE + s->N, s->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -684,7 +693,7 @@ int gram_impl(itp_stateset* s) {
 }
 
 template <class R>
-int rotate_set(itp_stateset* s, const cplx<R>* in, cplx<R>* out);
+int rotate_set(itp_stateset* s, const cplx<R>* in, cplx<R>* out, bool upper = false);
 
 template <class R>
 int ortho_impl(itp_stateset* s, size_t* bad_n, double* bad_lambda) {
@@ -701,17 +710,57 @@ int ortho_impl(itp_stateset* s, size_t* bad_n, double* bad_lambda) {
   return ITP_OK;
 }
 
+// Cholesky orthonormalization (the CholeskyQR step of the mixed path; no
+// counterpart in the reference): S = R^H R from the same Gram kernels,
+// Psi <- Psi R^-1 with the same rotation kernels. When the factorization
+// meets a pivot below the precision's threshold relative to its diagonal
+// (nearly dependent states), the eigendecomposition route of ortho_impl runs
+// on the same overlap matrix instead and *used_eigen is set.
+template <class R>
+int cholqr_impl(itp_stateset* s, int* used_eigen, size_t* bad_n, double* bad_lambda) {
+  itp_ctx* c = s->grid->ctx;
+  if (used_eigen) *used_eigen = 0;
+  int rc = gram_impl<R>(s);
+  if (rc) return rc;
+  int fail[2] = {0, 0};
+  {
+    Phase pe(c, ITP_PHASE_EIGENSOLVE);
+    // FP32 fields give an overlap with ~1e-7 relative errors: pivots below
+    // 1e-5 of their diagonal would be dominated by them
+    const double tol = c->chol_tol > 0 ? c->chol_tol : (sizeof(R) == 4 ? 1e-5 : 1e-12);
+    HIPCK(c, launch_cholinv(s->A[0], s->np, (int)s->N, s->A[1], s->diag0, s->C, s->nc, s->chol_fail, tol, c->stream));
+    HIPCK(c, launch_chol_diag_range(s->A[1], s->np, (int)s->N, s->chol_mm, c->stream));
+    double mm[2] = {0, 0};
+    HIPCK(c, hipMemcpyAsync(&fail[0], s->chol_fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipMemcpyAsync(&fail[1], s->bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipMemcpyAsync(mm, s->chol_mm, sizeof(mm), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    s->chol_kappa = (fail[0] || fail[1] || !(mm[1] > 0)) ? 0.0 : mm[0] / mm[1];
+  }
+  if (fail[0] || fail[1]) {
+    if (used_eigen) *used_eigen = 1;
+    if ((rc = eigen_stage(s, bad_n, bad_lambda, true))) return rc;
+  }
+  Phase pl(c, ITP_PHASE_LINCOMB);
+  // R^-1 is upper triangular (the eigendecomposition's C is not)
+  rc = rotate_set<R>(s, (const cplx<R>*)s->buf[s->cur], (cplx<R>*)s->buf[s->cur ^ 1], !(fail[0] || fail[1]));
+  pl.stop();
+  if (rc) return rc;
+  s->cur ^= 1;
+  return ITP_OK;
+}
+
 // out = in C for N fields of the set's precision: FP64 MFMA for FP64 sets,
 // FP32 MFMA (with an FP32 copy of C) for FP32 sets.
 template <class R>
-int rotate_set(itp_stateset* s, const cplx<R>* in, cplx<R>* out) {
+int rotate_set(itp_stateset* s, const cplx<R>* in, cplx<R>* out, bool upper) {
   itp_ctx* c = s->grid->ctx;
   const GridInfo& g = s->grid->g;
   if (sizeof(R) == 4) {
     HIPCK(c, launch_convert(s->C, s->C32, (long long)s->nc * s->nc, 1, c->stream));
     if (c->fp32_rotation == 1)
       HIPCK(c, launch_rotate_bf16x3((const cplx<float>*)in, (cplx<float>*)out, s->C32, s->nc, g.M, (int)s->N,
-                                    c->stream));
+                                    c->stream, upper ? 1 : 0));
     else
       HIPCK(c, launch_rotate_f32((const cplx<float>*)in, (cplx<float>*)out, s->C32, s->nc, g.M, (int)s->N,
                                  c->stream));
@@ -932,6 +981,12 @@ int itp_ctx_set_fp32_rotation(itp_ctx* c, int kind) {
 int itp_ctx_set_eig_spin_limit(itp_ctx* c, unsigned long long polls) {
   if (!c || polls > 0xffffffffull) return ITP_E_INVALID;
   c->eig_spin_limit = (unsigned)polls;
+  return ITP_OK;
+}
+
+int itp_ctx_set_cholesky_tol(itp_ctx* c, double tol) {
+  if (!c || !(tol >= 0) || !(tol < 1)) return ITP_E_INVALID;
+  c->chol_tol = tol;
   return ITP_OK;
 }
 
@@ -1236,6 +1291,10 @@ int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out
     if (precision == ITP_FP32 && (rc = dalloc(c, (void**)&s->C32, (size_t)s->nc * s->nc * 8))) return rc;
     HIPCK(c, hipMemset(s->C, 0, (size_t)s->nc * s->nc * 16));
     if ((rc = dalloc(c, (void**)&s->lam, N * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, (void**)&s->diag0, (size_t)s->np * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, (void**)&s->chol_fail, sizeof(int)))) return rc;
+    if ((rc = dalloc(c, (void**)&s->chol_mm, 2 * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, (void**)&s->norms, N * sizeof(double)))) return rc;
     if ((rc = dalloc(c, (void**)&s->perm, N * sizeof(int)))) return rc;
     if ((rc = dalloc(c, (void**)&s->info, 8 * sizeof(double)))) return rc;
     if ((rc = dalloc(c, (void**)&s->bad, sizeof(int)))) return rc;
@@ -1254,7 +1313,7 @@ void itp_stateset_destroy(itp_stateset* s) {
   void* ptrs[] = {s->buf[0], s->buf[1], s->part, s->A[0], s->A[1], s->V[0], s->V[1], s->Vinit, s->bjQ, s->bjD,
                   s->bjslots, s->bjident, s->oa_work, s->oa_lam, s->oa_slots, s->oa_state, s->cl_hdr, s->cl_pairs,
                   s->cl_members, s->cl_offsets, s->Vsave, s->oa_part, s->C, s->C32,
-                  s->lam, s->perm, s->info, s->bad, s->epart, s->dE, s->dmean};
+                  s->lam, s->perm, s->info, s->bad, s->epart, s->dE, s->dmean, s->diag0, s->chol_fail, s->chol_mm, s->norms};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete s;
@@ -1357,6 +1416,43 @@ int itp_orthonormalize(itp_stateset* s, size_t* bad_n, double* bad_lambda) {
   itp_ctx* c = s->grid->ctx;
   return guarded(c, [&]() -> int {
     return s->prec == ITP_FP64 ? ortho_impl<double>(s, bad_n, bad_lambda) : ortho_impl<float>(s, bad_n, bad_lambda);
+  });
+}
+
+int itp_orthonormalize_cholesky(itp_stateset* s, int* used_eigen, size_t* bad_n, double* bad_lambda) {
+  if (!s) return ITP_E_INVALID;
+  itp_ctx* c = s->grid->ctx;
+  return guarded(c, [&]() -> int {
+    return s->prec == ITP_FP64 ? cholqr_impl<double>(s, used_eigen, bad_n, bad_lambda)
+                               : cholqr_impl<float>(s, used_eigen, bad_n, bad_lambda);
+  });
+}
+
+int itp_stateset_cholesky_condition(const itp_stateset* s, double* kappa) {
+  if (!s || !kappa) return ITP_E_INVALID;
+  *kappa = s->chol_kappa;
+  return ITP_OK;
+}
+
+int itp_rayleigh_quotients(itp_hamiltonian* h, const itp_stateset* src, itp_stateset* dst, double* E) {
+  if (!h || !src || !dst || !E || dst->prec != ITP_FP64 || dst->N != src->N || dst->grid != src->grid ||
+      h->pot->grid != src->grid)
+    return ITP_E_INVALID;
+  itp_ctx* c = src->grid->ctx;
+  return guarded(c, [&]() -> int {
+    const GridInfo& g = src->grid->g;
+    const long long n = (long long)src->N * g.M;
+    void* to = dst->buf[dst->cur];
+    if (src->prec == ITP_FP64) HIPCK(c, hipMemcpyAsync(to, src->buf[src->cur], n * 16, hipMemcpyDeviceToDevice, c->stream));
+    else HIPCK(c, launch_convert(src->buf[src->cur], to, n, 0, c->stream));
+    HIPCK(c, launch_field_norms((const cplx<double>*)to, g.M, (int)src->N, dst->norms, c->stream));
+    int rc = energies_impl<double>(h, dst, E, nullptr, nullptr, nullptr, true);
+    if (rc) return rc;
+    std::vector<double> nrm(src->N);
+    HIPCK(c, hipMemcpy(nrm.data(), dst->norms, src->N * sizeof(double), hipMemcpyDeviceToHost));
+    const double dx2 = g.dx * g.dx;
+    for (size_t k = 0; k < src->N; k++) E[k] /= nrm[k] * dx2;
+    return ITP_OK;
   });
 }
 

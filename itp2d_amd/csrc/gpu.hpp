@@ -39,9 +39,10 @@ hipError_t launch_gram_bf16x3(const cplx<float>* A, const cplx<float>* B, long l
                               int* bad, hipStream_t st, int full = 0);
 hipError_t launch_rotate_f32(const cplx<float>* in, cplx<float>* out, const cplx<float>* C, int nc, long long M, int N,
                              hipStream_t st);
-// the same rotation with split-BF16x3 products on the bf16 matrix cores
+// the same rotation with split-BF16x3 products on the bf16 matrix cores;
+// upper: C is upper triangular (output state r takes inputs c <= r only)
 hipError_t launch_rotate_bf16x3(const cplx<float>* in, cplx<float>* out, const cplx<float>* C, int nc, long long M,
-                                int N, hipStream_t st);
+                                int N, hipStream_t st, int upper = 0);
 template <class R>
 hipError_t launch_residual(const cplx<R>* W, const cplx<R>* Psi, const double* E, long long M, int N, double dx,
                            double* sigma, hipStream_t st);
@@ -70,6 +71,15 @@ hipError_t launch_cluster_solve(const cplx<double>* A, int np, const int* member
                                 cplx<double>* X, hipStream_t st);
 hipError_t launch_eig_finish(const cplx<double>* A, const cplx<double>* V, int np, int N, double* lam_sorted,
                              int* perm, cplx<double>* C, int nc, int scale, hipStream_t st);
+// Cholesky S = R^H R of the np x np overlap (chol.hip): R (np x np scratch)
+// and W = R^-1 into the nc x nc coefficient matrix; *fail set (and the rest
+// skipped) on a pivot below tol times its original diagonal. diag0: np doubles.
+hipError_t launch_cholinv(const cplx<double>* S, int np, int N, cplx<double>* R, double* diag0, cplx<double>* W,
+                          int nc, int* fail, double tol, hipStream_t st);
+// mm[0], mm[1] = largest and smallest R_ii^2 (i < N) of the factor R
+hipError_t launch_chol_diag_range(const cplx<double>* R, int np, int N, double* mm, hipStream_t st);
+// out[n] = sum_i |f_n[i]|^2 for N fields of M elements
+hipError_t launch_field_norms(const cplx<double>* f, long long M, int N, double* out, hipStream_t st);
 template <class R>
 hipError_t launch_rotate(const cplx<R>* in, cplx<R>* out, const cplx<double>* C, int nc, long long M, int N,
                          hipStream_t st);

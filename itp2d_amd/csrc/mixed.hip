@@ -300,7 +300,8 @@ __device__ __forceinline__ bf8 swappairs(bf8 b) {
 }
 
 __global__ __launch_bounds__(256) void rotate_bf16x3_kernel(const cplx<float>* in, cplx<float>* out,
-                                                            const cplx<float>* C, int nc, long long M, int N) {
+                                                            const cplx<float>* C, int nc, long long M, int N,
+                                                            int upper) {
   __shared__ __align__(16) __bf16 As[3][64][32 + kR3Pad];  // [part][output state r][interleaved input k]
   __shared__ __align__(16) __bf16 Bs[3][64][32 + kR3Pad];  // [part][grid point t][interleaved input k]
   int rt;
@@ -319,7 +320,9 @@ __global__ __launch_bounds__(256) void rotate_bf16x3_kernel(const cplx<float>* i
       re[a][b] = f4{0, 0, 0, 0};
       im[a][b] = f4{0, 0, 0, 0};
     }
-  for (int c0 = 0; c0 < N; c0 += 16) {
+  // upper triangular C: output states r0..r0+63 take inputs c < r0 + 64 only
+  const int cend = upper ? (N < r0 + 64 ? N : r0 + 64) : N;
+  for (int c0 = 0; c0 < cend; c0 += 16) {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const int e = tid + 256 * q;
@@ -389,10 +392,10 @@ __global__ __launch_bounds__(256) void rotate_bf16x3_kernel(const cplx<float>* i
 }
 
 hipError_t launch_rotate_bf16x3(const cplx<float>* in, cplx<float>* out, const cplx<float>* C, int nc, long long M,
-                                int N, hipStream_t st) {
+                                int N, hipStream_t st, int upper) {
   const long long blocks = xcd_grid(nc / 64, (M + 63) / 64);
   if (blocks > 0x7fffffffLL || nc % 64) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(rotate_bf16x3_kernel, dim3((unsigned)blocks), dim3(256), 0, st, in, out, C, nc, M, N);
+  hipLaunchKernelGGL(rotate_bf16x3_kernel, dim3((unsigned)blocks), dim3(256), 0, st, in, out, C, nc, M, N, upper);
   return hipGetLastError();
 }
 

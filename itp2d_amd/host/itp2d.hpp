@@ -143,6 +143,24 @@ class StateSet {
     if (rc == ITP_E_NONNORMAL_EIG) throw NonNormalEigenvalue(n, v);
     tr_.context().check(rc, "orthonormalize");
   }
+  // Cholesky orthonormalization (the mixed path's CholeskyQR step); returns
+  // true when it fell back to the eigendecomposition route
+  bool orthonormalize_cholesky() {
+    size_t n = 0;
+    double v = 0;
+    int used = 0;
+    const int rc = itp_orthonormalize_cholesky(s_, &used, &n, &v);
+    if (rc == ITP_E_NONPOSITIVE_EIG) throw NonPositiveEigenvalue(n, v);
+    if (rc == ITP_E_NONNORMAL_EIG) throw NonNormalEigenvalue(n, v);
+    tr_.context().check(rc, "orthonormalize_cholesky");
+    return used != 0;
+  }
+  // max R_ii^2 / min R_ii^2 of the last Cholesky factor (0: none)
+  double cholesky_condition() const {
+    double k = 0;
+    tr_.context().check(itp_stateset_cholesky_condition(s_, &k), "cholesky_condition");
+    return k;
+  }
   // Ascending overlap eigenvalues of the last orthonormalization.
   void overlap_eigenvalues(std::vector<double>& lam) const {
     lam.resize(N);
@@ -199,6 +217,12 @@ class Hamiltonian {
   ~Hamiltonian() { itp_hamiltonian_destroy(h_); }
   Hamiltonian(const Hamiltonian&) = delete;
   Hamiltonian& operator=(const Hamiltonian&) = delete;
+  // FP64 Rayleigh quotients of src's states, evaluated on a copy in dst (FP64)
+  void rayleigh_quotients(const StateSet& src, StateSet& dst, std::vector<double>& E) {
+    E.resize(src.N);
+    pot_.transformer().context().check(itp_rayleigh_quotients(h_, src.get(), dst.get(), E.data()),
+                                       "rayleigh_quotients");
+  }
   // real parts of mean and standard deviation for every state, by slot
   void mean_and_standard_deviation(StateSet& s, std::vector<double>& E, std::vector<double>& sigma) {
     E.resize(s.N);

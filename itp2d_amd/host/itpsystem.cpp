@@ -38,9 +38,13 @@ class ITPSystem {
     T_.reset(new MultiProductSplit(p.halforder, *pot_, eps, p.B));
     ts_hist.emplace_back(1, eps);
     mixed = p.precision == ITP_FP32;
+    if (mixed && p.mixed_check_every > 0) next_check = p.mixed_check_every;
     if (mixed) {
-      if (!(params.mixed_tol > 0)) params.mixed_tol = 2e-7;
+      if (!(params.mixed_tol > 0))
+        params.mixed_tol = (p.timestep_test.kind == ITPS_TEST_RELCHANGE && p.timestep_test.p0 > 0) ? p.timestep_test.p0
+                                                                                                   : 1e-9;
       if (!(params.accuracy > 0)) params.accuracy = 1e-8;
+
     }
     rng = p.seed;
     if (init) states_->upload(reinterpret_cast<const comp*>(init));
@@ -75,9 +79,9 @@ class ITPSystem {
       std::fprintf(out, "Step %d (step %d with eps = %e) starting at %s\n", total_step_counter, step_counter, eps,
                    timestring().c_str());
     T_->operate(*states_);          // propagate (itpsystem.cpp:243-256)
+    if (mixed) { mixed_step(); return; }
     orthonormalize();               // itpsystem.cpp:258-289
     if (finished) return;
-    if (mixed) { mixed_energies_and_tests(); return; }
     calculate_energies();           // itpsystem.cpp:473-511
     if (verbosity >= 3) print_energies();
     check_timestep_convergence();
@@ -171,7 +175,11 @@ class ITPSystem {
 
   itps_params params;
   bool mixed = false, just_recovered = false;
-  int mixed_steps = 0, stall = 0;
+  int mixed_steps = 0, mixed_checks = 0, chol_fallbacks = 0, stall = 0;
+  int next_check = 16, last_check = 0;  // mixed path: FP32 step of the next check, and of the last one
+  int since_ortho = 1, skipped = 0;     // propagations since the last orthonormalization; steps skipped
+  double last_kappa = 0;                // condition per propagation of the last CholeskyQR (0: unknown)
+  double prev_rate = 0;                // per-step energy change measured at the last check
   double best_change = 1e300;
   std::vector<double> eps_list;
   double eps = 0;
@@ -238,9 +246,13 @@ class ITPSystem {
     for (size_t n = 0; n < params.N; n++) set_ts(n, false);
   }
 
-  void orthonormalize() {
+  // cholesky: the mixed path's CholeskyQR step instead of the reference's
+  // eigendecomposition route (which it falls back to on nearly dependent
+  // states, with the same errors)
+  void orthonormalize(bool cholesky = false) {
     try {
-      states_->orthonormalize();
+      if (!cholesky) states_->orthonormalize();
+      else if (states_->orthonormalize_cholesky()) chol_fallbacks++;
     } catch (NonPositiveEigenvalue&) {
       if (params.recover) {
         recoveries++;
@@ -267,22 +279,84 @@ class ITPSystem {
     push_history();
   }
 
-  // Mixed path: energies from the overlap eigenvalues of this step's
-  // orthonormalization, lambda_n = |T psi_n|^2 ~ exp(-2 eps E_n) for the
-  // Ritz-like states the canonical orthonormalization produces; slot r of the
-  // set holds the eigenvector of the r-th smallest lambda.
-  void mixed_energies_and_tests() {
-    if (just_recovered) {  // fresh noise: no energies this step
+  // Mixed path, FP32 phase (no counterpart in the reference): the step
+  // after propagation. The checked steps (schedule_next_check) orthonormalize
+  // as the reference does (the overlap eigendecomposition: a Rayleigh-Ritz step on
+  // the propagator, so slot r holds the r-th Ritz vector) and checks the
+  // states' FP64 Rayleigh quotients; the other steps use CholeskyQR, which
+  // keeps the same span at a fraction of the cost.
+  void mixed_step() {
+    mixed_steps++;
+    const bool check = mixed_steps >= next_check;
+    // two propagations may share one CholeskyQR step while the overlap stays
+    // well conditioned (kappa per step below kSkipKappa, so below ~1e3 for
+    // two). Only when the hand-off tolerance is within FP32 reach (1e-10):
+    // below it the FP64 steps after the hand-off still converge the states,
+    // and they then start from states orthonormalized at every step, as the
+    // reference's are.
+    constexpr double kSkipKappa = 30;
+    if (!check && params.mixed_skip_ortho >= 0 && params.mixed_tol >= 1e-10 && since_ortho == 1 && last_kappa > 0 &&
+        last_kappa < kSkipKappa) {
+      since_ortho++;
+      skipped++;
+      return;
+    }
+    orthonormalize(!check);
+    if (finished) return;
+    if (!check) {
+      const double k = states_->cholesky_condition();
+      last_kappa = k > 0 ? std::pow(k, 1.0 / since_ortho) : 0.0;
+    } else {
+      last_kappa = 0;
+    }
+    since_ortho = 1;
+    if (just_recovered) {  // fresh noise: nothing to check this step
       just_recovered = false;
       return;
     }
-    std::vector<double> lam;
-    states_->overlap_eigenvalues(lam);
+    if (check) mixed_check();
+  }
+
+  // Steps until the next check: fixed (mixed_check_every), or predicted from
+  // the geometric decay of the per-step energy change between the last two
+  // checks (the step count at which it falls below mixed_tol), within 2..8.
+  void schedule_next_check(double worst) {
+    const int interval = mixed_steps - last_check;
+    last_check = mixed_steps;
+    int next = 4;
+    if (params.mixed_check_every > 0) {
+      next = params.mixed_check_every;
+    } else if (prev_rate > 0 && worst > 0 && interval > 0) {
+      const double rate = worst / interval;  // per-step change
+      const double rho = std::pow(rate / prev_rate, 1.0 / interval);
+      if (rho < 0.999 && rho > 0) {
+        const double n = std::log(params.mixed_tol / rate) / std::log(rho);
+        next = (int)std::ceil(std::max(2.0, std::min(8.0, n)));
+      }
+      prev_rate = rate;
+    } else if (worst > 0 && interval > 0) {
+      prev_rate = worst / interval;
+    }
+    next_check = mixed_steps + next;
+  }
+
+  // FP64 energies of the FP32 states (evaluated on an FP64 copy: quadratic in
+  // the states' errors, so the FP32 rounding enters at ~1e-13) and the end of
+  // the FP32 phase: every wanted state's energy changed by less than
+  // mixed_tol since the last check, or the largest change has not reached a
+  // new low for kStall checks (the FP32 fixed point is reached). The reference
+  // algorithm then continues in FP64 on the checked copy until its own tests
+  // pass: the reference's energies are those of the FP64 fixed point of the
+  // split propagator, which the FP32 states are within rounding-level terms of.
+  void mixed_check() {
+    if (!states64_) states64_.reset(new StateSet(*tr_, params.N, FP64));
+    std::vector<double> E;
+    H_->rayleigh_quotients(*states_, *states64_, E);
     esn.resize(params.N);
-    for (size_t r = 0; r < params.N; r++) esn[r] = Esn_tuple(-std::log(lam[r]) / (2 * eps), 0.0, r);
+    for (size_t r = 0; r < params.N; r++) esn[r] = Esn_tuple(E[r], 0.0, r);
     std::sort(esn.begin(), esn.end());
     push_history();
-    mixed_steps++;
+    mixed_checks++;
     if (energies.size() < 2) return;
     const auto& a = energies[energies.size() - 1];
     const auto& b = energies[energies.size() - 2];
@@ -291,25 +365,14 @@ class ITPSystem {
       const double r = std::fabs((a[n] - b[n]) / b[n]);
       worst = std::isfinite(r) ? std::max(worst, r) : 1e300;
     }
-    // switch when every wanted state changes by less than mixed_tol, or when
-    // the largest change has not reached a new low for kStall steps (the
-    // FP32 rounding floor)
-    constexpr int kStall = 6;
+    constexpr int kStall = 3;
     if (worst < best_change) { best_change = worst; stall = 0; }
     else stall++;
+    schedule_next_check(worst);
     if (!(worst < params.mixed_tol) && stall < kStall) return;
-    // FP32 phase done: continue with FP64 steps of the reference algorithm on
-    // an FP64 copy of the states until its own tests pass. The energies the
-    // reference reports are those of the fixed point of the split propagator
-    // (they carry its O(eps^(2h)) splitting error), so parity needs that fixed
-    // point; from FP32-converged states its energies are within second-order
-    // terms, and the FP64 steps only confirm convergence.
-    // The FP64 set is allocated here and the FP32 one released after the
-    // copy, so the two precisions' sets are never both resident longer than
-    // the copy (a 1024^2 grid with 4000 states holds 62.5 GiB per FP32 set
-    // pair and 125 GiB per FP64 pair).
-    states64_.reset(new StateSet(*tr_, params.N, FP64));
-    states64_->assign(*states_);
+    // FP32 phase done: the FP64 copy becomes the state set (the FP32 one is
+    // released, so the two precisions' sets are resident together only
+    // during the checks)
     std::swap(states_, states64_);
     states64_.reset();
     mixed = false;
@@ -426,7 +489,7 @@ class ITPSystem {
   std::unique_ptr<Transformer> tr_;
   std::unique_ptr<Potential> pot_;
   std::unique_ptr<StateSet> states_;
-  std::unique_ptr<StateSet> states64_;  // mixed path: the FP64 set, alive only while the FP32 states are copied
+  std::unique_ptr<StateSet> states64_;  // mixed path: the FP64 copy of the FP32 states checked last
   std::unique_ptr<Hamiltonian> H_;
   std::unique_ptr<MultiProductSplit> T_;
   uint64_t rng = 0;
@@ -508,6 +571,14 @@ int itps_states(itps_system* s, double* out) {
 double itps_total_time(const itps_system* s) { return s ? s->sys->total_time : 0; }
 int itps_recoveries(const itps_system* s) { return s ? s->sys->recoveries : 0; }
 int itps_mixed_steps(const itps_system* s) { return s ? s->sys->mixed_steps : 0; }
+int itps_mixed_stats(const itps_system* s, int* out) {
+  if (!s || !out) return ITP_E_INVALID;
+  out[0] = s->sys->mixed_steps;
+  out[1] = s->sys->mixed_checks;
+  out[2] = s->sys->chol_fallbacks;
+  out[3] = s->sys->skipped;
+  return ITP_OK;
+}
 int itps_rayleigh_ritz(itps_system* s) {
   if (!s) return ITP_E_INVALID;
   try {

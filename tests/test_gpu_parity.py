@@ -112,6 +112,54 @@ def test_large_grids(ctx, n, bt, B, h, V):
         assert np.abs(E - Eo).max() < (1e-12 if prec == I.FP64 else 1e-5) * np.abs(Eo).max(), prec
 
 
+def test_laplacian_of_gaussian(ctx):
+    """The reference's Laplacian known answer (test_laplacian.cpp:38-58) on the GPU's H with
+    the null potential (H = -1/2 Laplacian at B = 0): exp(-r^2) -> 4 (r^2 - 1) exp(-r^2),
+    rms within 100 eps and max within 1000 eps, on 48x48 (the reference's 42 has a factor 7,
+    which the GPU transforms do not take)."""
+    say("laplacian of a gaussian")
+    n, L = 48, 12.0
+    dx = L / n
+    g = I.Grid(ctx, n, n, dx, I.PERIODIC)
+    H = I.Hamiltonian(I.Potential(g, None), 0.0)
+    px, py = W.positions(n, n, dx)
+    X, Y = np.meshgrid(px, py)
+    G = np.exp(-(X * X + Y * Y)) + 0j
+    R = 4 * (X * X + Y * Y - 1) * np.exp(-(X * X + Y * Y))
+    lap = -2 * H.apply(G[None])[0]
+    eps = np.finfo(float).eps
+    assert np.sqrt(np.mean(np.abs(lap - R) ** 2)) < 100 * eps
+    assert np.abs(lap - R).max() < 1000 * eps
+
+
+@pytest.mark.parametrize("case", ["box", "harmonic", "harmonic_magnetic"])
+def test_hamiltonian_ground_state_stable(ctx, case):
+    """The reference's Hamiltonian stability test (test_hamiltonian.cpp:40-106): the ITP
+    ground state of the GPU driver, applied H / E0 three times, stays within rms 5e-3."""
+    say(f"hamiltonian stability {case}")
+    n = 40
+    if case == "box":
+        L, bt, B, V = np.pi, I.DIRICHLET, 0.0, None
+    else:
+        L, bt = 10.0, I.PERIODIC
+        B = 1.0 if case == "harmonic_magnetic" else 0.0
+        V = W.harmonic(n, n, L / n)
+    cfg = dict(sx=n, sy=n, lenx=L, boundary=bt, N=1, n_wanted=1, halforder=5, B=B, eps_values=[1.0],
+               eps_divisor=5.0, max_steps=50, min_time_step=1e-9, seed=W.SEED_BASE,
+               timestep_test=("relstdev", 1e-3, 1e-4), final_test=("relstdev", 1e-3, 0.0))
+    s = I.ITPSystem(ctx, cfg, V)
+    s.run()
+    assert not s.error_flag
+    E, _, idx = s.sorted()
+    gs = s.states()[idx[0]]
+    g = I.Grid(ctx, n, n, L / n, bt)
+    H = I.Hamiltonian(I.Potential(g, V), B)
+    t = gs.copy()
+    for _ in range(3):
+        t = H.apply(t[None])[0] / E[0]
+        assert np.sqrt(np.mean(np.abs(t - gs) ** 2)) < 5e-3
+
+
 def test_null_potential_split(ctx):
     sx = sy = 32
     dx = 0.2
@@ -161,6 +209,24 @@ def test_eigh(ctx, n):
     w, V = I.eigh(ctx, A)
     assert np.abs(w - np.linalg.eigvalsh(A)).max() < 1e-12 * np.abs(w).max()
     assert np.abs(V @ np.diag(w) @ V.conj().T - A).max() < 1e-12 * np.abs(A).max()
+
+
+def test_eigh_reference_known_answers(ctx):
+    """The reference's EigenSolver known answers (test_eigensolver.cpp:25-74) on the GPU:
+    [[1, 2], [2, 4]] -> 0, 5 with eigenvectors (-2, 1)/sqrt5, (1, 2)/sqrt5 up to a unit phase,
+    and the 8x8 spectral decomposition to 300 eps."""
+    say("eigh reference known answers")
+    eps = np.finfo(float).eps
+    w, V = I.eigh(ctx, np.array([[1, 2], [2, 4]], complex))
+    assert abs(w[0]) < 20 * eps and abs(w[1] - 5) < 20 * eps
+    for n, e in enumerate([np.array([-2, 1]) / np.sqrt(5), np.array([1, 2]) / np.sqrt(5)]):
+        ph = np.vdot(e, V[:, n])
+        assert abs(abs(ph) - 1) < 20 * eps
+        assert np.abs(V[:, n] - ph * e).max() < 20 * eps
+    A = np.array([[1 if i == j else (1j * abs(i - j) if j > i else -1j * abs(i - j)) for j in range(8)]
+                  for i in range(8)])
+    w, V = I.eigh(ctx, A)
+    assert np.linalg.norm(A - (V * w) @ V.conj().T) < 300 * eps
 
 
 def test_eigh_barrier_timeout_is_an_error(ctx):

@@ -98,8 +98,9 @@ def test_reference_dirac_delta_ifft_and_unit_wave_fft():
     assert np.sqrt(np.mean(np.abs(O.transform(W_, "FFT") * g["norms"][0] - T) ** 2)) < 10 * EPS
 
 
-# test_eigensolver.cpp:25-74
-def test_reference_eigensolver_2x2():
+# a complex Hermitian 2x2 with eigenvalues 0 and 2 (the reference's own real 2x2 and
+# 8x8 known answers, test_eigensolver.cpp:25-74, are in test_oracle_pins.py)
+def test_eigensolver_2x2_complex():
     A = np.array([[1, 1j], [-1j, 1]])
     w, V = O.eigh(A)
     assert w == pytest.approx([0, 2], abs=1e-15)
