@@ -1,17 +1,21 @@
-"""Benchmark: converged states/s of the ITP loop on one MI355X.
+"""Benchmark: converged states/s of the ITP loop on one MI355X, mixed vs FP64.
 
-A step is one complete ITP solve of the workload (seeded initial states,
-propagate -> orthonormalize -> energies until the convergence tests pass),
-with all inputs resident in HBM and the per-solve setup (tables, allocations,
-uploads) done before the timed region, as the reference starts its timer at
-the first step (itpsystem.cpp:416-421). value = converged states of all timed
-solves / wall time of the timed solves.
+Default workload (the batched configuration): 64 independent quantum-dot
+instances (grids 128^2/192^2/256^2, 50-400 states, oscillator plus 40 Gaussian
+impurities, B in [0, 2), periodic, 4th-order multi-product split, eps 0.05,
+energy-change tests at 1e-9), all solved together by the batch API (worker
+threads, one HIP stream each). A step is one complete batched solve of all 64
+instances (seeded noise initial states, propagate -> orthonormalize ->
+energies until every instance's tests pass), inputs resident in HBM, per-solve
+setup inside the timed region as part of each instance (the reference's own
+timer starts at the first step, itpsystem.cpp:416-421; here the whole batch
+wall time counts).
 
-Default workload: the 256x256 Dirichlet grid, N = 500, oscillator + 40
-Gaussian impurities, B = 1, 4th-order multi-product split, eps = 0.05, on the
-plain FP64 GPU path (value). The same workload also runs on the mixed-precision
-path ("mixed" in the output line): its converged states count only if their
-energies agree with the FP64 path's to 1e-8 relative.
+value: converged states per second of the mixed-precision path, counting a
+state only if the FP64 path converges it too and the two energies agree to
+1e-8 relative. The FP64 path runs the same batch once (timed, its converged
+states/s in "fp64"); speedup_vs_fp64 is the ratio. The 256x256 Dirichlet
+single-instance workload (N = 500, B = 1) is reported beside it ("c2").
 
 Prints one JSON line. Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -131,62 +135,193 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes": n * 4, "dispatches": 3}), flush=True)
 
 
-def batch_bench(I, W, args):
-    """The batched configuration: 64 independent instances run by the batch
-    API (worker threads with one stream each), full solves; value = converged
-    states of all instances / wall time of the batch run. The same instances
-    run by one worker give the no-overlap reference."""
+def batch_solve(I, insts, prec, workers):
+    """One batched solve of all instances on one path: (rc, wall s, per-instance results)."""
+    b = I.Batch(0, workers=workers)
+    for c, V in insts:
+        b.add(dict(c, precision=prec), V)
+    rc = b.run()
+    res = []
+    for i in range(len(insts)):
+        r = b.result(i)
+        E, _, f = b.sorted(i)
+        r.update(E=E, conv=f, mixed=b.mixed_stats(i))
+        res.append(r)
+    wall = b.wall_time
+    b.close()
+    return rc, wall, res
+
+
+def agreement(res, ref, tol=1e-8):
+    """(states converged on both paths within tol of the reference's energies,
+    converged on both but farther, converged here only, largest relative error
+    among the states converged on both)."""
+    good = far = extra = 0
+    worst = 0.0
+    for r, q in zip(res, ref):
+        both = r["conv"] & q["conv"]
+        err = np.abs(r["E"] - q["E"]) / np.abs(q["E"])
+        good += int((both & (err <= tol)).sum())
+        far += int((both & (err > tol)).sum())
+        extra += int((r["conv"] & ~q["conv"]).sum())
+        if both.any():
+            worst = max(worst, float(err[both].max()))
+    return good, far, extra, worst
+
+
+def sweep_roofline(I, cfg, V, calls=20):
+    """Roofline of the dominant kernel group (the FP32 sweeps of propagation) at
+    the shape of one instance: `calls` propagations of its state set on the
+    context's stream, timed with the library's HIP events around each call."""
+    prec = I.FP32
+    N, M = cfg["N"], cfg["sx"] * cfg["sy"]
+    with I.Context(0) as ctx:
+        dx = cfg["lenx"] / cfg["sx"]
+        g = I.Grid(ctx, cfg["sx"], cfg["sy"], dx, cfg["boundary"])
+        pot = I.Potential(g, V)
+        sp = I.MultiProductSplit(pot, cfg["halforder"], cfg["eps_values"][0], cfg["B"])
+        S = I.StateSet(g, N, prec)
+        S.init_noise(cfg["seed"], dx)
+        sp.propagate(S)
+        ctx.synchronize()
+        ctx.enable_timing(True)
+        ctx.reset_phase_times()
+        for _ in range(calls):
+            sp.propagate(S)
+        ctx.synchronize()
+        ms, cnt = ctx.phase_times()["propagate"]
+        ctx.enable_timing(False)
+        h = sp.halforder
+    p_prop = 2 * h * h + 5 * h - 1
+    launches = h * h + 2 * h
+    field = N * M * 8
+    per_launch_bytes = p_prop * field / launches
+    avg_ms = ms / (cnt * launches)
+    achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
+    return {"kernel": "reg_sweep_kernel<float> (propagation sweeps)", "bound": "hbm",
+            "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "avg_launch_ms": round(avg_ms, 5), "algorithmic_bytes_per_launch": round(per_launch_bytes),
+            "shape": f"{cfg['sx']}x{cfg['sy']}, N={N}, h={h}, complex FP32"}
+
+
+def c4_cpu_baseline(insts, index=0):
+    """The FP64 C++ oracle's full solve of one instance (the smallest) on the host cores."""
+    from oracle import pyoracle as O
+    cfg, V = insts[index]
+    threads = O.num_threads()
+    sysc = O.System(cfg, V)
+    t0 = time.perf_counter()
+    sysc.run()
+    dt = time.perf_counter() - t0
+    conv = sysc.num_converged
+    return {"value": round(conv / dt, 4), "unit": "converged states/s", "cores": threads, "kind": "port",
+            "sample": f"full FP64 solve of instance {index} ({cfg['sx']}x{cfg['sy']}, N={cfg['N']}, "
+                      f"{sysc.total_steps} ITP iterations, {conv} converged) with the FP64 C++ oracle, "
+                      f"{dt:.2f} s on {threads} OpenMP threads"}
+
+
+def c2_summary(I, W):
+    """The 256x256 Dirichlet workload, one solve per path (the FP64 solve is the reference)."""
+    cfg, V = W.config("c2_dirichlet256")
+    out = {}
+    with I.Context(0) as ctx:
+        walls, res = {}, {}
+        for name, prec in [("fp64", I.FP64), ("mixed", I.FP32)]:
+            s = I.ITPSystem(ctx, dict(cfg, precision=prec), V)
+            s.run()  # warm-up
+            s.close()
+            s = I.ITPSystem(ctx, dict(cfg, precision=prec), V)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            s.run()
+            ctx.synchronize()
+            walls[name] = time.perf_counter() - t0
+            res[name] = {"E": s.sorted()[0], "conv": s.converged(), "steps": s.total_steps,
+                         "mixed": s.mixed_stats()}
+            s.close()
+    good, far, extra, worst = agreement([res["mixed"]], [res["fp64"]])
+    v64 = int(res["fp64"]["conv"].sum()) / walls["fp64"]
+    vm = good / walls["mixed"]
+    out = {"workload": "c2_dirichlet256", "grid": "256x256 dirichlet", "N": cfg["N"], "B": cfg["B"],
+           "halforder": cfg["halforder"],
+           "fp64": {"value": round(v64, 3), "ms_per_solve": round(walls["fp64"] * 1e3, 3),
+                    "steps": res["fp64"]["steps"], "converged": int(res["fp64"]["conv"].sum())},
+           "mixed": {"value": round(vm, 3), "ms_per_solve": round(walls["mixed"] * 1e3, 3),
+                     "steps": res["mixed"]["steps"], "fp32_steps": res["mixed"]["mixed"][0],
+                     "converged_within_1e-8": good, "converged_farther": far, "converged_extra": extra,
+                     "max_rel_err_vs_fp64": worst},
+           "speedup_vs_fp64": round(vm / v64, 3)}
+    log(f"c2: {out}")
+    return out
+
+
+def c4_headline(I, W, args):
     insts = W.c4_instances(64)
-    prec = I.FP32 if args.precision == "fp32" else I.FP64
-    insts = [(dict(c, precision=prec), V) for c, V in insts]
-    log(f"batch: {len(insts)} instances, {sum(c['N'] for c, _ in insts)} states, "
-        f"workers {args.batch_workers}, {args.precision}")
-    warm = I.Batch(0, workers=args.batch_workers)
-    for c, V in insts[: args.batch_workers]:
-        warm.add(c, V)
-    warm.run()
-    log(f"warmup batch {warm.wall_time:.2f}s")
-    del warm
-
-    def one(workers):
-        b = I.Batch(0, workers=workers)
-        for c, V in insts:
-            b.add(c, V)
-        rc = b.run()
-        res = [b.result(i) for i in range(len(insts))]
-        return rc, b.wall_time, res
-
-    rc, wall, res = one(args.batch_workers)
-    conv = sum(r["converged"] for r in res)
-    iters = sum(r["steps"] for r in res)
-    bad = [i for i, r in enumerate(res) if r["status"] or r["error_flag"]]
-    log(f"batch run rc={rc}: {wall:.2f}s, {conv} converged, {iters} iterations, failed instances {bad}")
-    seq = None
-    if not args.no_batch_serial:
-        _, wall1, res1 = one(1)
-        seq = {"workers": 1, "wall_s": round(wall1, 3),
-               "value": round(sum(r["converged"] for r in res1) / wall1, 3)}
-        log(f"one worker: {wall1:.2f}s")
+    states = sum(c["N"] for c, _ in insts)
+    log(f"c4: {len(insts)} instances, {states} states, workers {args.batch_workers}")
+    # the FP64 path once: timed, and the reference energies of the mixed path
+    rc64, wall64, res64 = batch_solve(I, insts, I.FP64, args.batch_workers)
+    conv64 = sum(int(r["conv"].sum()) for r in res64)
+    it64 = sum(r["steps"] for r in res64)
+    log(f"fp64 batch rc={rc64}: {wall64:.2f}s, {conv64} converged, {it64} ITP iterations")
+    for w in range(args.warmup):
+        rc, wall, _ = batch_solve(I, insts, I.FP32, args.batch_workers)
+        log(f"mixed warm-up {w}: rc={rc} {wall:.2f}s")
+    walls, goods, fars, extras, worst, its, fp32, checks, skips, fb = [], [], [], [], 0.0, 0, 0, 0, 0, 0
+    for k in range(args.steps):
+        rc, wall, res = batch_solve(I, insts, I.FP32, args.batch_workers)
+        good, far, extra, w_ = agreement(res, res64)
+        walls.append(wall)
+        goods.append(good)
+        fars.append(far)
+        extras.append(extra)
+        worst = max(worst, w_)
+        its = sum(r["steps"] for r in res)
+        fp32 = sum(r["mixed"][0] for r in res)
+        checks = sum(r["mixed"][1] for r in res)
+        fb = sum(r["mixed"][2] for r in res)
+        skips = sum(r["mixed"][3] for r in res)
+        failed = [i for i, r in enumerate(res) if r["status"] or r["error_flag"]]
+        log(f"mixed step {k}: rc={rc} {wall:.2f}s, {good} converged within 1e-8 of fp64 ({far} farther, "
+            f"{extra} converged only here), failed instances {failed}")
+    total = sum(walls)
+    value = sum(goods) / total
+    v64 = conv64 / wall64
+    roof = sweep_roofline(I, *max(insts, key=lambda x: x[0]["N"] * x[0]["sx"] * x[0]["sy"]))
+    log(f"roofline: {roof}")
     line = {
         "metric": METRIC,
-        "value": round(conv / wall, 3),
+        "value": round(value, 3),
         "unit": "converged states/s",
         "n_gpus": 1,
-        "steps": 1,
-        "warmup": 1,
-        "ms_per_step": round(wall * 1e3, 3),
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * total / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "c32-fields/f64-dense",
+        "dtype": "c32 fields; bf16x3 MFMA Gram and rotation with fp32 accumulation; f64 Cholesky, "
+                 "eigensolver, checks and final steps",
         "data": "synthetic (64 seeded instances: grids 128^2/192^2/256^2, 50-400 states, B in [0,2), "
-                "oscillator plus impurity potentials, seeded noise initial states)",
-        "config": {"workload": "c4_batch64", "instances": len(insts), "states": sum(c["N"] for c, _ in insts),
-                   "path": args.precision, "workers": args.batch_workers, "converged": conv,
-                   "itp_iterations": iters, "itp_iterations_per_s": round(iters / wall, 3),
-                   "failed_instances": bad},
-        "serial": seq,
+                "oscillator plus 40 Gaussian impurities, seeded noise initial states)",
+        "config": {"workload": "c4_batch64", "instances": len(insts), "states": states,
+                   "workers": args.batch_workers, "path": "mixed",
+                   "converged_within_1e-8_of_fp64": goods[-1], "converged_farther_than_1e-8": fars[-1],
+                   "converged_not_by_fp64": extras[-1], "max_rel_err_vs_fp64": worst,
+                   "itp_iterations": its, "fp32_iterations": fp32, "fp64_checks": checks,
+                   "skipped_orthonormalizations": skips, "cholesky_fallbacks": fb,
+                   "itp_iterations_per_s": round(its / (total / args.steps), 3)},
+        "fp64": {"value": round(v64, 3), "unit": "converged states/s", "ms_per_step": round(wall64 * 1e3, 3),
+                 "converged": conv64, "itp_iterations": it64, "dtype": "f64"},
+        "speedup_vs_fp64": round(value / v64, 3),
+        "roofline": roof,
     }
+    if not args.no_c2:
+        line["c2"] = c2_summary(I, W)
+    if not args.no_cpu_baseline:
+        log("cpu baseline sample ...")
+        line["cpu_baseline"] = c4_cpu_baseline(insts)
     print(json.dumps(line), flush=True)
 
 
@@ -195,7 +330,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2_dirichlet256")
+    ap.add_argument("--workload", default="c4_batch64")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--cpu-sample-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -208,8 +343,7 @@ def main():
                          "FETCH_SIZE / WRITE_SIZE counters against a known byte count) and exit")
     ap.add_argument("--batch-workers", type=int, default=4,
                     help="batched configuration (--workload c4_batch64): worker threads, one stream each")
-    ap.add_argument("--no-batch-serial", action="store_true",
-                    help="batched configuration: skip the one-worker comparison run")
+    ap.add_argument("--no-c2", action="store_true", help="batched configuration: skip the c2 summary")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -223,7 +357,7 @@ def main():
     from itp2d_amd import workloads as W
 
     if args.workload == "c4_batch64":
-        batch_bench(I, W, args)
+        c4_headline(I, W, args)
         return
     cfg, V = W.config(args.workload)
     cfg = dict(cfg, precision=I.FP32 if args.precision == "fp32" else I.FP64)
@@ -251,6 +385,7 @@ def main():
                           "iteration_ms": [round(w * 1e3, 2) for w in walls],
                           "phase_ms": ctx.phase_times(), "eigensolver": ctx.eig_stats(),
                           "refinement": ctx.oa_stats()}), flush=True)
+        ctx.close()
         return
 
     mixed = None
@@ -355,6 +490,7 @@ def main():
     if not args.no_cpu_baseline:
         log("cpu baseline sample ...")
         line["cpu_baseline"] = cpu_baseline(W.config(args.workload)[0], V, iters[0], conv[0], args.cpu_sample_steps)
+    ctx.close()  # device objects freed before the interpreter exits
     print(json.dumps(line), flush=True)
 
 

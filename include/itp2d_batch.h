@@ -47,6 +47,8 @@ double itp_batch_wall_time(const itp_batch* b);
  * Any pointer may be NULL. */
 int itp_batch_result(const itp_batch* b, size_t i, int* status, int* steps, size_t* converged, int* error_flag,
                      double* time_s);
+/* itps_mixed_stats of instance i (zeros on the FP64 path). */
+int itp_batch_mixed_stats(const itp_batch* b, size_t i, int* out);
 /* Sorted energies, deviations and final-convergence flags of instance i
  * (N values each, nullable), as itps_sorted / itps_converged. */
 int itp_batch_sorted(const itp_batch* b, size_t i, double* E, double* sigma, int* converged_flags);

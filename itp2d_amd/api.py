@@ -171,6 +171,7 @@ def lib():
             "itp_batch_result": [_vp, _sz, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(_sz),
                                  C.POINTER(C.c_int), _dp],
             "itp_batch_sorted": [_vp, _sz, _dp, _dp, C.POINTER(C.c_int)],
+            "itp_batch_mixed_stats": [_vp, _sz, C.POINTER(C.c_int)],
         }
         for name, args in sig.items():
             getattr(L, name).argtypes = args
@@ -790,6 +791,14 @@ class Batch:
             raise ITPError(rc, "itp_batch_result")
         return dict(status=st.value, steps=steps.value, converged=conv.value, error_flag=bool(err.value),
                     time=float(t[0]))
+
+    def mixed_stats(self, i):
+        """ITPSystem.mixed_stats of instance i."""
+        out = (C.c_int * 4)()
+        rc = lib().itp_batch_mixed_stats(self.h, int(i), out)
+        if rc:
+            raise ITPError(rc, "itp_batch_mixed_stats")
+        return tuple(out)
 
     def sorted(self, i):
         N = int(self._cfgs[i]["N"])

@@ -27,6 +27,7 @@ struct Instance {
   double time_s = 0;
   std::vector<double> E, sigma;
   std::vector<int> flags;
+  int mixed[4] = {0, 0, 0, 0};
 };
 
 }  // namespace
@@ -57,6 +58,7 @@ int run_one(itp_ctx* ctx, Instance& in) {
   in.flags.assign(N, 0);
   itps_sorted(sys, in.E.data(), in.sigma.data(), nullptr);
   itps_converged(sys, in.flags.data());
+  itps_mixed_stats(sys, in.mixed);
   itps_destroy(sys);
   return rc;
 }
@@ -157,6 +159,12 @@ int itp_batch_result(const itp_batch* b, size_t i, int* status, int* steps, size
   if (converged) *converged = in.converged;
   if (error_flag) *error_flag = in.error_flag;
   if (time_s) *time_s = in.time_s;
+  return ITP_OK;
+}
+
+int itp_batch_mixed_stats(const itp_batch* b, size_t i, int* out) {
+  if (!b || !out || i >= b->inst.size()) return ITP_E_INVALID;
+  std::copy(b->inst[i].mixed, b->inst[i].mixed + 4, out);
   return ITP_OK;
 }
 
