@@ -1,4 +1,5 @@
-// Register-resident line engine for power-of-two line lengths (32..1024).
+// Register-resident line engine for the line lengths 32..1024 that are powers
+// of two, and 192.
 //
 // Same sweep contract as sweep_program (line_ops.hpp): a workgroup takes a
 // block of whole lines of one field, runs the line-operation program and
@@ -39,11 +40,13 @@ template <> struct RegShape<128>  { static constexpr int P = 8,  Q = 16, NST = 2
 template <> struct RegShape<256>  { static constexpr int P = 16, Q = 16, NST = 2, R0 = 16, R1 = 16, R2 = 1; };
 template <> struct RegShape<512>  { static constexpr int P = 32, Q = 16, NST = 3, R0 = 16, R1 = 16, R2 = 2; };
 template <> struct RegShape<1024> { static constexpr int P = 64, Q = 16, NST = 3, R0 = 16, R1 = 16, R2 = 4; };
+// 192 = 12 * 4 * 4: twelve elements per thread, radix 12 (3 x 4) then two radix-4 stages
+template <> struct RegShape<192>  { static constexpr int P = 16, Q = 12, NST = 3, R0 = 12, R1 = 4,  R2 = 4; };
 
 constexpr int kRegThreads = 256;
 
 ITP_HD bool reg_supported(int n) {
-  return n == 32 || n == 64 || n == 128 || n == 256 || n == 512 || n == 1024;
+  return n == 32 || n == 64 || n == 128 || n == 192 || n == 256 || n == 512 || n == 1024;
 }
 // Lines per workgroup for a line length (kRegThreads / P).
 ITP_HD int reg_lines_per_wg(int n) {
@@ -51,6 +54,7 @@ ITP_HD int reg_lines_per_wg(int n) {
     case 32: return 64;
     case 64: return 32;
     case 128: return 32;
+    case 192: return 16;
     case 256: return 16;
     case 512: return 8;
     case 1024: return 4;
@@ -146,11 +150,52 @@ ITP_HD void dft16(cplx<R>* x) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// radix 12 in registers: x[m], m = 4 m1 + m2 (m1 < 3, m2 < 4)  ->  X[k1 + 3 k2]
+// ---------------------------------------------------------------------------
+template <class R, bool INV>
+ITP_HD cplx<R> w12_mul(cplx<R> a, int e) {
+  // a * e^{-+2 pi i e/12} for the exponents m2 k1 of the 3 x 4 split
+  const R s3 = R(0.86602540378443864676);
+  cplx<R> w;
+  switch (e) {
+    case 0: return a;
+    case 3: return INV ? mul_i(a) : mul_ni(a);
+    case 6: return -a;
+    case 1: w = mk<R>(s3, R(-0.5)); break;
+    case 2: w = mk<R>(R(0.5), -s3); break;
+    case 4: w = mk<R>(R(-0.5), -s3); break;
+    default: w = mk<R>(R(0), R(0)); break;  // not reached by dft12
+  }
+  if (INV) w.y = -w.y;
+  return a * w;
+}
+
+template <class R, bool INV>
+ITP_HD void dft12(cplx<R>* x) {
+  cplx<R> y[4][3];
+#pragma unroll
+  for (int m2 = 0; m2 < 4; m2++) {
+    cplx<R> a[3] = {x[m2], x[4 + m2], x[8 + m2]};
+    dft3<R, INV>(a);
+#pragma unroll
+    for (int k1 = 0; k1 < 3; k1++) y[m2][k1] = w12_mul<R, INV>(a[k1], m2 * k1);
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < 3; k1++) {
+    cplx<R> a[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+    dft4<R, INV>(a);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; k2++) x[k1 + 3 * k2] = a[k2];
+  }
+}
+
 template <class R, int RAD, bool INV>
 ITP_HD void dft_reg(cplx<R>* v) {
   if constexpr (RAD == 2) dft2<R, INV>(v);
   else if constexpr (RAD == 4) dft4<R, INV>(v);
   else if constexpr (RAD == 8) dft8<R, INV>(v);
+  else if constexpr (RAD == 12) dft12<R, INV>(v);
   else dft16<R, INV>(v);
 }
 
@@ -167,7 +212,7 @@ ITP_HD void reg_stage(cplx<R>* A, int t, cplx<R>* line, const cplx<double>* tw) 
 #pragma unroll
   for (int b = 0; b < NB; b++) {
     const int j = t + P * b;
-    const int k = opaque(j & (NS - 1));
+    const int k = opaque(j % NS);
     cplx<R> x[RAD];
 #pragma unroll
     for (int r = 0; r < RAD; r++) x[r] = A[b + r * NB];
@@ -297,7 +342,7 @@ ITP_HD cplx<R> pend_value(const cplx<R>* line, int p, long long g, const cplx<do
     const int k = K == PK_R2F_DST ? n - 1 - p : p;
     const cplx<double> wd = mkt[opaque(k)];
     const cplx<R> w = mk<R>(R(wd.x), R(wd.y));
-    v = w * line[lpad(k)] + conj(w) * line[lpad((n - k) & (n - 1))];
+    v = w * line[lpad(k)] + conj(w) * line[lpad(k == 0 ? 0 : n - k)];
     if constexpr (TAG::elem) v = v * R(ft[g * fsg + p * fsp]);
   } else if constexpr (K == PK_PERM_DCT || K == PK_PERM_DST) {
     v = line[lpad(perm_src(n, p))];
@@ -634,6 +679,7 @@ inline bool reg_dispatch(int n, int orient, bool xs, F&& f) {
     case 32: return go(std::integral_constant<int, 32>{});
     case 64: return go(std::integral_constant<int, 64>{});
     case 128: return go(std::integral_constant<int, 128>{});
+    case 192: return go(std::integral_constant<int, 192>{});
     case 256: return go(std::integral_constant<int, 256>{});
     case 512: return go(std::integral_constant<int, 512>{});
     case 1024: return go(std::integral_constant<int, 1024>{});

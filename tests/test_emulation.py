@@ -52,7 +52,7 @@ BRANCHES = [(0, 0.0), (1, 0.0), (0, 1.3), (1, 1.3)]
 
 
 @pytest.mark.parametrize("bt,B", BRANCHES)
-@pytest.mark.parametrize("sx,sy", [(64, 64), (40, 40), (64, 128), (256, 32)])
+@pytest.mark.parametrize("sx,sy", [(64, 64), (40, 40), (64, 128), (256, 32), (192, 32), (32, 192)])
 def test_operators(bt, B, sx, sy, engine):
     dx, V, f = _setup(sx, sy)
     assert rel(E.expkinetic(f, dx, bt, B, 0.07, -1.0, 0.9), O.apply_expkinetic(f, dx, bt, B, 0.07, -1.0, 0.9)) < 1e-14
