@@ -178,6 +178,10 @@ int itp_orthonormalize_cholesky(itp_stateset* s, int* used_eigen, size_t* bad_n,
  * itp_orthonormalize_cholesky: max R_ii^2 / min R_ii^2 of its factor (0 if it
  * fell back to the eigendecomposition or none ran). */
 int itp_stateset_cholesky_condition(const itp_stateset* s, double* kappa);
+/* R_ii^2 (i < N) of the last itp_orthonormalize_cholesky's factor, in the
+ * states' order: the squared norms of the successive Gram-Schmidt residuals.
+ * ITP_E_INVALID if the last call fell back to the eigendecomposition. */
+int itp_stateset_cholesky_diagonal(const itp_stateset* s, double* d);
 /* FP64 Rayleigh quotients E[n] = Re<psi_n|H psi_n> / <psi_n|psi_n> of the
  * states of src (either precision), evaluated in FP64 on a copy made into dst
  * (an ITP_FP64 set of the same grid and size, whose fields are overwritten with

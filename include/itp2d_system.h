@@ -68,7 +68,10 @@ typedef struct itps_params {
                                relchange, else 1e-9 */
   double accuracy;          /* error bound of itps_rayleigh_ritz; 0: 1e-8 */
   int mixed_check_every;    /* FP32 steps per check of the mixed path; 0:
-                               adaptive (first check after 16 steps, then the
+                               adaptive (first check once the log-determinant of the
+                               leading overlap from the CholeskyQR factors
+                               changes by < 1e-6 per step, or stops falling,
+                               or after 64 steps; then the
                                step count predicted from the decay of the
                                energy changes, within 2..8) */
   int mixed_skip_ortho;     /* mixed path: 1 (default) lets a CholeskyQR step

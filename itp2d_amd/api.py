@@ -156,6 +156,7 @@ def lib():
             "itp_orthonormalize_cholesky": [_vp, C.POINTER(C.c_int), C.POINTER(_sz), _dp],
             "itp_rayleigh_quotients": [_vp, _vp, _vp, _dp],
             "itp_stateset_cholesky_condition": [_vp, _dp],
+            "itp_stateset_cholesky_diagonal": [_vp, _dp],
             "itps_set_abort_flag": [_vp, C.POINTER(C.c_int)],
             "itps_set_verbosity": [_vp, C.c_int],
             "itp_potential_create_type": [_vp, C.c_char_p, _dp, _sz, C.POINTER(_vp)],

@@ -105,8 +105,9 @@ struct itp_stateset {
   // Cholesky orthonormalization: original diagonal, failure flag; field norms
   double* diag0 = nullptr;
   int* chol_fail = nullptr;
-  double* chol_mm = nullptr;     // largest and smallest R_ii^2 of the last factorization
+  double* chol_mm = nullptr;     // largest and smallest R_ii^2 of the last factorization, then every R_ii^2
   double chol_kappa = 0;         // their ratio (0: no factorization yet or it failed)
+  std::vector<double> chol_diag; // R_ii^2 of the last factorization (empty if it failed)
   double* norms = nullptr;
   // energies
   cplx<double>* epart = nullptr;  // [2][N][max(sx,sy)]
@@ -730,12 +731,15 @@ int cholqr_impl(itp_stateset* s, int* used_eigen, size_t* bad_n, double* bad_lam
     const double tol = c->chol_tol > 0 ? c->chol_tol : (sizeof(R) == 4 ? 1e-5 : 1e-12);
     HIPCK(c, launch_cholinv(s->A[0], s->np, (int)s->N, s->A[1], s->diag0, s->C, s->nc, s->chol_fail, tol, c->stream));
     HIPCK(c, launch_chol_diag_range(s->A[1], s->np, (int)s->N, s->chol_mm, c->stream));
-    double mm[2] = {0, 0};
+    std::vector<double> mm(2 + s->N, 0.0);
     HIPCK(c, hipMemcpyAsync(&fail[0], s->chol_fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCK(c, hipMemcpyAsync(&fail[1], s->bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCK(c, hipMemcpyAsync(mm, s->chol_mm, sizeof(mm), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipMemcpyAsync(mm.data(), s->chol_mm, mm.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCK(c, hipStreamSynchronize(c->stream));
-    s->chol_kappa = (fail[0] || fail[1] || !(mm[1] > 0)) ? 0.0 : mm[0] / mm[1];
+    const bool ok = !(fail[0] || fail[1]) && mm[1] > 0;
+    s->chol_kappa = ok ? mm[0] / mm[1] : 0.0;
+    if (ok) s->chol_diag.assign(mm.begin() + 2, mm.end());
+    else s->chol_diag.clear();
   }
   if (fail[0] || fail[1]) {
     if (used_eigen) *used_eigen = 1;
@@ -1293,7 +1297,7 @@ int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out
     if ((rc = dalloc(c, (void**)&s->lam, N * sizeof(double)))) return rc;
     if ((rc = dalloc(c, (void**)&s->diag0, (size_t)s->np * sizeof(double)))) return rc;
     if ((rc = dalloc(c, (void**)&s->chol_fail, sizeof(int)))) return rc;
-    if ((rc = dalloc(c, (void**)&s->chol_mm, 2 * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, (void**)&s->chol_mm, (2 + N) * sizeof(double)))) return rc;
     if ((rc = dalloc(c, (void**)&s->norms, N * sizeof(double)))) return rc;
     if ((rc = dalloc(c, (void**)&s->perm, N * sizeof(int)))) return rc;
     if ((rc = dalloc(c, (void**)&s->info, 8 * sizeof(double)))) return rc;
@@ -1431,6 +1435,13 @@ int itp_orthonormalize_cholesky(itp_stateset* s, int* used_eigen, size_t* bad_n,
 int itp_stateset_cholesky_condition(const itp_stateset* s, double* kappa) {
   if (!s || !kappa) return ITP_E_INVALID;
   *kappa = s->chol_kappa;
+  return ITP_OK;
+}
+
+int itp_stateset_cholesky_diagonal(const itp_stateset* s, double* d) {
+  if (!s || !d) return ITP_E_INVALID;
+  if (s->chol_diag.size() != s->N) return ITP_E_INVALID;
+  std::copy(s->chol_diag.begin(), s->chol_diag.end(), d);
   return ITP_OK;
 }
 

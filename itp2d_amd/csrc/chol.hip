@@ -255,7 +255,7 @@ __global__ __launch_bounds__(256) void field_norms_kernel(const cplx<double>* f,
   if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
-// mm[0] = max, mm[1] = min of R_ii^2 over i < N (one workgroup)
+// mm[0] = max, mm[1] = min of R_ii^2 over i < N, mm[2 + i] = R_ii^2 (one workgroup)
 __global__ __launch_bounds__(256) void chol_diag_range_kernel(const cplx<double>* R, int np, int N, double* mm) {
   __shared__ double hi[256], lo[256];
   double h = 0, l = 1e300;
@@ -263,6 +263,7 @@ __global__ __launch_bounds__(256) void chol_diag_range_kernel(const cplx<double>
     const double r = R[(long long)i * np + i].x;
     h = fmax(h, r * r);
     l = fmin(l, r * r);
+    mm[2 + i] = r * r;
   }
   hi[threadIdx.x] = h;
   lo[threadIdx.x] = l;

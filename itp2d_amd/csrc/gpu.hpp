@@ -76,7 +76,8 @@ hipError_t launch_eig_finish(const cplx<double>* A, const cplx<double>* V, int n
 // skipped) on a pivot below tol times its original diagonal. diag0: np doubles.
 hipError_t launch_cholinv(const cplx<double>* S, int np, int N, cplx<double>* R, double* diag0, cplx<double>* W,
                           int nc, int* fail, double tol, hipStream_t st);
-// mm[0], mm[1] = largest and smallest R_ii^2 (i < N) of the factor R
+// mm[0], mm[1] = largest and smallest R_ii^2 (i < N) of the factor R, then
+// mm[2 + i] = R_ii^2
 hipError_t launch_chol_diag_range(const cplx<double>* R, int np, int N, double* mm, hipStream_t st);
 // out[n] = sum_i |f_n[i]|^2 for N fields of M elements
 hipError_t launch_field_norms(const cplx<double>* f, long long M, int N, double* out, hipStream_t st);

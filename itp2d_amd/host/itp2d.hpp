@@ -161,6 +161,11 @@ class StateSet {
     tr_.context().check(itp_stateset_cholesky_condition(s_, &k), "cholesky_condition");
     return k;
   }
+  // R_ii^2 of the last Cholesky factor; false if it fell back
+  bool cholesky_diagonal(std::vector<double>& d) const {
+    d.resize(N);
+    return itp_stateset_cholesky_diagonal(s_, d.data()) == ITP_OK;
+  }
   // Ascending overlap eigenvalues of the last orthonormalization.
   void overlap_eigenvalues(std::vector<double>& lam) const {
     lam.resize(N);

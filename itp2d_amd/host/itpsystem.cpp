@@ -176,7 +176,9 @@ class ITPSystem {
   itps_params params;
   bool mixed = false, just_recovered = false, after_handoff = false;
   int mixed_steps = 0, mixed_checks = 0, chol_fallbacks = 0, stall = 0, stall_any = 0;
-  int next_check = 16, last_check = 0;  // mixed path: FP32 step of the next check, and of the last one
+  int next_check = 1 << 30, last_check = 0;  // mixed path: FP32 step of the next check, and of the last one
+  int trace_steps = 0, trace_step = 0, trace_stall = 0;  // the Cholesky trace watch before the first check
+  double last_trace = 0, best_trace_rate = 1e300;
   int since_ortho = 1, skipped = 0;     // propagations since the last orthonormalization; steps skipped
   double last_kappa = 0;                // condition per propagation of the last CholeskyQR (0: unknown)
   double prev_rate = 0;                // per-step energy change measured at the last check
@@ -303,18 +305,48 @@ class ITPSystem {
     }
     orthonormalize(!check);
     if (finished) return;
+    if (just_recovered) {  // fresh noise: nothing to check this step
+      just_recovered = false;
+      since_ortho = 1;
+      last_kappa = 0;
+      trace_steps = 0;
+      return;
+    }
     if (!check) {
       const double k = states_->cholesky_condition();
       last_kappa = k > 0 ? std::pow(k, 1.0 / since_ortho) : 0.0;
+      watch_cholesky_trace();
     } else {
       last_kappa = 0;
     }
     since_ortho = 1;
-    if (just_recovered) {  // fresh noise: nothing to check this step
-      just_recovered = false;
-      return;
-    }
     if (check) mixed_check();
+  }
+
+  // Before the first FP64 check: the Cholesky factor's diagonal tracks the
+  // span of the leading states. sum_{wanted n} ln R_nn^2 (per propagation) is
+  // the log-determinant of the propagator's overlap on the span of the first
+  // wanted states, whatever basis it has, so it converges smoothly as that
+  // span does. The first check comes once it changes by less than 1e-6 per
+  // step (relative), or stops falling for 4 factorizations, or after 64 steps.
+  void watch_cholesky_trace() {
+    if (params.mixed_check_every > 0 || mixed_checks > 0 || next_check <= mixed_steps + 1) return;
+    std::vector<double> d;
+    if (!states_->cholesky_diagonal(d)) { trace_steps = 0; return; }
+    double tr = 0;
+    for (size_t n = params.ignore_lowest; n < params.ignore_lowest + params.n_wanted; n++) tr += std::log(d[n]);
+    tr /= since_ortho;
+    bool start = mixed_steps >= 64;
+    if (trace_steps > 0 && std::isfinite(tr) && tr != 0) {
+      const double rate = std::fabs(tr - last_trace) / std::fabs(tr) / (mixed_steps - trace_step);
+      if (rate < best_trace_rate) { best_trace_rate = rate; trace_stall = 0; }
+      else trace_stall++;
+      if (rate < 1e-6 || (trace_stall >= 4 && mixed_steps >= 8)) start = true;
+    }
+    last_trace = tr;
+    trace_step = mixed_steps;
+    trace_steps++;
+    if (start) next_check = mixed_steps + 1;
   }
 
   // Steps until the next check: fixed (mixed_check_every), or predicted from
@@ -370,7 +402,7 @@ class ITPSystem {
     // counts only near the FP32 floor: far above it a non-monotone stretch
     // (states entering the span, levels reordering) is no floor
     constexpr int kStall = 3;
-    constexpr double kFloorZone = 1e-7;
+    constexpr double kFloorZone = 1e-9;
     const double rate = worst / std::max(1, mixed_steps - last_check);
     if (rate < best_change) { best_change = rate; stall = 0; stall_any = 0; }
     else {
