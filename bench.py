@@ -341,7 +341,7 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="profiling mode: run 3 plain device copies of this many MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against a known byte count) and exit")
-    ap.add_argument("--batch-workers", type=int, default=4,
+    ap.add_argument("--batch-workers", type=int, default=8,
                     help="batched configuration (--workload c4_batch64): worker threads, one stream each")
     ap.add_argument("--no-c2", action="store_true", help="batched configuration: skip the c2 summary")
     args = ap.parse_args()

@@ -56,8 +56,8 @@ typedef struct itps_params {
                                Rayleigh quotients of the states
                                (itp_rayleigh_quotients). The FP32 phase ends
                                when those energies change by less than
-                               mixed_tol (relative) for every wanted state
-                               between consecutive checks, or stop improving for
+                               mixed_tol (relative, per step) for every wanted
+                               state between consecutive checks, or stop improving for
                                3 checks (the FP32 floor). Then the reference
                                algorithm continues in FP64 on the checked FP64
                                copy until the configured tests pass (the
@@ -70,17 +70,17 @@ typedef struct itps_params {
   int mixed_check_every;    /* FP32 steps per check of the mixed path; 0:
                                adaptive (first check once the log-determinant of the
                                leading overlap from the CholeskyQR factors
-                               changes by < 1e-6 per step, or stops falling,
-                               or after 64 steps; then the
-                               step count predicted from the decay of the
-                               energy changes, within 2..8) */
-  int mixed_skip_ortho;     /* mixed path: 1 (default) lets a CholeskyQR step
-                               be skipped when mixed_tol >= 1e-10 and the last
-                               factor's condition per propagation step is
-                               below 30, so two
-                               propagations share one orthonormalization (the
-                               span is the same; the overlap's condition stays
-                               below ~1e3); -1 orthonormalizes every step */
+                               changes by < 1e-5 per step or stops falling,
+                               not before 16 steps and at the latest after
+                               64; then the step count predicted from the
+                               decay of the energy changes, within 2..16) */
+  int mixed_skip_ortho;     /* mixed path: 1 (default) lets CholeskyQR steps
+                               be skipped when mixed_tol >= 1e-10, while the
+                               last factor's condition per propagation step,
+                               raised to the number of propagations sharing
+                               one orthonormalization, stays below 1e3 (the
+                               span is the same); -1 orthonormalizes every
+                               step */
 } itps_params;
 
 typedef struct itps_system itps_system;

@@ -181,7 +181,8 @@ class ITPSystem {
   double last_trace = 0, best_trace_rate = 1e300;
   int since_ortho = 1, skipped = 0;     // propagations since the last orthonormalization; steps skipped
   double last_kappa = 0;                // condition per propagation of the last CholeskyQR (0: unknown)
-  double prev_rate = 0;                // per-step energy change measured at the last check
+  double prev_worst = 0, rho_est = -1; // largest energy change over the last check interval; decay per step
+  int prev_interval = 0;
   double best_change = 1e300;
   std::vector<double> eps_list;
   double eps = 0;
@@ -255,17 +256,22 @@ class ITPSystem {
     try {
       if (!cholesky) states_->orthonormalize();
       else if (states_->orthonormalize_cholesky()) chol_fallbacks++;
-    } catch (NonPositiveEigenvalue&) {
+    } catch (NonPositiveEigenvalue& e) {
+      // the reference reports the failure and the recovery (itpsystem.cpp:264-280)
+      if (verbosity >= 0) std::fprintf(err, "ERROR: %s\n", e.what());
       if (params.recover) {
+        if (verbosity >= 0) std::fprintf(err, "Trying to recover: Changing time step and resetting states.\n");
         recoveries++;
         just_recovered = true;
         change_time_step();
         init_noise();
+        if (verbosity >= 1) std::fprintf(out, "States reset. Resuming propagation.\n");
       } else {
         error_flag = true;
         finish();
       }
-    } catch (std::exception&) {
+    } catch (std::exception& e) {
+      if (verbosity >= 0) std::fprintf(err, "ERROR: \n%s\nQuitting...\n", e.what());  // itpsystem.cpp:282-287
       error_flag = true;
       finish();
     }
@@ -290,15 +296,18 @@ class ITPSystem {
   void mixed_step() {
     mixed_steps++;
     const bool check = mixed_steps >= next_check;
-    // two propagations may share one CholeskyQR step while the overlap stays
-    // well conditioned (kappa per step below kSkipKappa, so below ~1e3 for
-    // two). Only when the hand-off tolerance is within FP32 reach (1e-10):
-    // below it the FP64 steps after the hand-off still converge the states,
-    // and they then start from states orthonormalized at every step, as the
-    // reference's are.
-    constexpr double kSkipKappa = 30;
-    if (!check && params.mixed_skip_ortho >= 0 && params.mixed_tol >= 1e-10 && since_ortho == 1 && last_kappa > 0 &&
-        last_kappa < kSkipKappa) {
+    // several propagations may share one CholeskyQR step while the overlap
+    // stays well conditioned: kappa per step to the power of the propagations
+    // since the last orthonormalization kept below kSkipKappa. Only when the
+    // hand-off tolerance is within FP32 reach (1e-10): below it the FP64 steps
+    // after the hand-off still converge the states, and they then start from
+    // states orthonormalized at every step, as the reference's are.
+    // The condition estimate (the spread of the factor's diagonal) is only
+    // trusted once the noise's high-energy content has decayed (8 steps), and
+    // at most four propagations share an orthonormalization.
+    constexpr double kSkipKappa = 1e3;
+    if (!check && params.mixed_skip_ortho >= 0 && params.mixed_tol >= 1e-10 && mixed_steps > 8 && since_ortho < 4 &&
+        last_kappa > 1 && std::pow(last_kappa, since_ortho + 1) < kSkipKappa) {
       since_ortho++;
       skipped++;
       return;
@@ -323,12 +332,15 @@ class ITPSystem {
     if (check) mixed_check();
   }
 
-  // Before the first FP64 check: the Cholesky factor's diagonal tracks the
-  // span of the leading states. sum_{wanted n} ln R_nn^2 (per propagation) is
-  // the log-determinant of the propagator's overlap on the span of the first
-  // wanted states, whatever basis it has, so it converges smoothly as that
-  // span does. The first check comes once it changes by less than 1e-6 per
-  // step (relative), or stops falling for 4 factorizations, or after 64 steps.
+  // Before the first FP64 check (whose Rayleigh-Ritz step is expensive while
+  // the basis is far from the eigenbasis): the Cholesky factor's diagonal
+  // tracks the span of the leading states. sum_{wanted n} ln R_nn^2 (per
+  // propagation) is the log-determinant of the propagator's overlap on the
+  // span of the first wanted states, whatever basis it has, and it settles
+  // more slowly than the wanted energies (that span converges at the gap
+  // after the last wanted state, the energies at the gap after the whole
+  // set). The first check comes once it changes by less than 1e-5 per step
+  // (relative), or stops falling for 4 factorizations, or after 64 steps.
   void watch_cholesky_trace() {
     if (params.mixed_check_every > 0 || mixed_checks > 0 || next_check <= mixed_steps + 1) return;
     std::vector<double> d;
@@ -341,7 +353,8 @@ class ITPSystem {
       const double rate = std::fabs(tr - last_trace) / std::fabs(tr) / (mixed_steps - trace_step);
       if (rate < best_trace_rate) { best_trace_rate = rate; trace_stall = 0; }
       else trace_stall++;
-      if (rate < 1e-6 || (trace_stall >= 4 && mixed_steps >= 8)) start = true;
+      if (mixed_steps >= 16 && (rate < 1e-5 || trace_stall >= 4)) start = true;
+      if (verbosity >= 3) std::fprintf(out, "\tCholesky log-determinant %.12e, change per step %.3e\n", tr, rate);
     }
     last_trace = tr;
     trace_step = mixed_steps;
@@ -349,25 +362,44 @@ class ITPSystem {
     if (start) next_check = mixed_steps + 1;
   }
 
-  // Steps until the next check: fixed (mixed_check_every), or predicted from
-  // the geometric decay of the per-step energy change between the last two
-  // checks (the step count at which it falls below mixed_tol), within 2..8.
-  void schedule_next_check(double worst) {
-    const int interval = mixed_steps - last_check;
+  // Per-step energy change at the end of the interval since the last check,
+  // and the step to check next. The largest change over an interval of K
+  // steps is modelled as a geometric sum, r(t) = A rho^t per step; two
+  // consecutive intervals (W1 over K1 steps, then W2 over K2) fix rho through
+  // W2 / W1 = rho^K1 (1 - rho^K2) / (1 - rho^K1), and the rate at the end is
+  // W2 (1 - rho) rho^(K2 - 1) / (1 - rho^K2). The next check comes when that
+  // rate is predicted to fall below mixed_tol (2..16 steps; 8 while rho is
+  // unknown), or after mixed_check_every steps if that is set.
+  double end_rate(double worst, int interval) {
+    double rate = worst / interval;
+    rho_est = -1;
+    if (prev_worst > 0 && worst > 0 && prev_interval > 0) {
+      const int K1 = prev_interval, K2 = interval;
+      const double target = worst / prev_worst;
+      auto f = [&](double r) { return std::pow(r, K1) * (1 - std::pow(r, K2)) / (1 - std::pow(r, K1)); };
+      if (target < double(K2) / K1 * 0.999) {
+        double lo = 1e-6, hi = 1 - 1e-9;
+        for (int it = 0; it < 60; it++) {
+          const double mid = 0.5 * (lo + hi);
+          (f(mid) < target ? lo : hi) = mid;
+        }
+        rho_est = 0.5 * (lo + hi);
+        rate = worst * (1 - rho_est) * std::pow(rho_est, K2 - 1) / (1 - std::pow(rho_est, K2));
+      }
+    }
+    prev_worst = worst;
+    prev_interval = interval;
+    return rate;
+  }
+
+  void schedule_next_check(double rate) {
     last_check = mixed_steps;
-    int next = 4;
+    int next = 8;
     if (params.mixed_check_every > 0) {
       next = params.mixed_check_every;
-    } else if (prev_rate > 0 && worst > 0 && interval > 0) {
-      const double rate = worst / interval;  // per-step change
-      const double rho = std::pow(rate / prev_rate, 1.0 / interval);
-      if (rho < 0.999 && rho > 0) {
-        const double n = std::log(params.mixed_tol / rate) / std::log(rho);
-        next = (int)std::ceil(std::max(2.0, std::min(8.0, n)));
-      }
-      prev_rate = rate;
-    } else if (worst > 0 && interval > 0) {
-      prev_rate = worst / interval;
+    } else if (rho_est > 0 && rate > 0) {
+      const double n = std::log(params.mixed_tol / rate) / std::log(rho_est);
+      next = (int)std::ceil(std::max(2.0, std::min(16.0, n)));
     }
     next_check = mixed_steps + next;
   }
@@ -403,14 +435,15 @@ class ITPSystem {
     // (states entering the span, levels reordering) is no floor
     constexpr int kStall = 3;
     constexpr double kFloorZone = 1e-9;
-    const double rate = worst / std::max(1, mixed_steps - last_check);
+    const double rate = end_rate(worst, std::max(1, mixed_steps - last_check));
     if (rate < best_change) { best_change = rate; stall = 0; stall_any = 0; }
     else {
       stall_any++;  // eight checks without progress anywhere: stuck, let FP64 finish
       if (best_change < kFloorZone) stall++;
     }
-    schedule_next_check(worst);
-    if (!(worst < params.mixed_tol) && stall < kStall && stall_any < 8) return;
+    schedule_next_check(rate);
+    // the reference's test is on the change per step; checks are spaced
+    if (!(rate < params.mixed_tol) && stall < kStall && stall_any < 8) return;
     // FP32 phase done: the FP64 copy becomes the state set (the FP32 one is
     // released, so the two precisions' sets are resident together only
     // during the checks)
