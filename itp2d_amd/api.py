@@ -505,6 +505,12 @@ class StateSet(_Owned):
         self.grid.ctx.check(lib().itp_stateset_init_noise(self.h, C.byref(st), C.c_double(dx)), "init_noise")
         return st.value
 
+    def cholesky_condition(self):
+        """max R_ii^2 / min R_ii^2 of the last Cholesky orthonormalization's factor (0: none)."""
+        k = C.c_double(0)
+        self.grid.ctx.check(lib().itp_stateset_cholesky_condition(self.h, C.byref(k)), "cholesky_condition")
+        return k.value
+
     def overlap_eigenvalues(self):
         lam = np.zeros(self.N)
         self.grid.ctx.check(lib().itp_stateset_overlap_eigenvalues(self.h, _p(lam)), "overlap_eigenvalues")

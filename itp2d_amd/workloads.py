@@ -133,6 +133,25 @@ def config(name):
                    eps_values=[0.1], eps_divisor=5.0, max_steps=2000, min_time_step=1e-9,
                    seed=SEED_BASE, timestep_test=("relchange", 1e-9), final_test=("relchange", 1e-9))
         return cfg, harmonic(sx, sy, L / sx)
+    if name == "c3_ho128":
+        # the C3 physics (oscillator, periodic, 2nd-order split, eps 0.1) at a
+        # size the CPU oracle runs to convergence in minutes: 128x128 on
+        # [-8,8]^2, N = 200 (160 wanted)
+        sx = sy = 128
+        L = 16.0
+        cfg = dict(sx=sx, sy=sy, lenx=L, boundary=0, N=200, n_wanted=160, halforder=1, B=0.0,
+                   eps_values=[0.1], eps_divisor=5.0, max_steps=2000, min_time_step=1e-9,
+                   seed=SEED_BASE, timestep_test=("relchange", 1e-9), final_test=("relchange", 1e-9))
+        return cfg, harmonic(sx, sy, L / sx)
+    if name == "c5_stadium256":
+        # the C5 physics (hard-wall stadium, Dirichlet, B = 2, 6th-order split,
+        # eps 0.02, recovery on) at 256x256 on [-2.5,2.5]^2 with N = 100 (80 wanted)
+        sx = sy = 256
+        L = 5.0
+        cfg = dict(sx=sx, sy=sy, lenx=L, boundary=1, N=100, n_wanted=80, halforder=3, B=2.0, recover=1,
+                   eps_values=[0.02], eps_divisor=5.0, max_steps=3000, min_time_step=1e-9,
+                   seed=SEED_BASE, timestep_test=("relchange", 1e-9), final_test=("relchange", 1e-9))
+        return cfg, hard_stadium(sx, sy, L / sx)
     if name == "c5_stadium1024":
         # 1024x1024 Dirichlet grid on [-2.5,2.5]^2, N=4000 (3200 wanted), hard
         # stadium (V = 0 inside radius 1 / straight length 2, 1e4 outside),
