@@ -205,6 +205,10 @@ def sweep_roofline(I, cfg, V, calls=20):
             "shape": f"{cfg['sx']}x{cfg['sy']}, N={N}, h={h}, complex FP32"}
 
 
+def largest_instance(insts):
+    return max(insts, key=lambda x: x[0]["N"] * x[0]["sx"] * x[0]["sy"])
+
+
 def c4_cpu_baseline(insts, index=0):
     """The FP64 C++ oracle's full solve of one instance (the smallest) on the host cores."""
     from oracle import pyoracle as O
@@ -288,7 +292,8 @@ def c4_headline(I, W, args):
     total = sum(walls)
     value = sum(goods) / total
     v64 = conv64 / wall64
-    roof = sweep_roofline(I, *max(insts, key=lambda x: x[0]["N"] * x[0]["sx"] * x[0]["sy"]))
+    roof = sweep_roofline(I, *largest_instance(insts))
+    roof["traffic"] = measured_traffic("c4_largest", "fp32")
     log(f"roofline: {roof}")
     line = {
         "metric": METRIC,
@@ -338,6 +343,9 @@ def main():
     ap.add_argument("--iterations", type=int, default=0,
                     help="profiling mode: run this many ITP iterations of one solve and print the phase times "
                          "instead of the benchmark line")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="profiling mode: only the sweep roofline measurement of the batched configuration "
+                         "(20 propagations at its largest instance's shape) and exit")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="profiling mode: run 3 plain device copies of this many MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against a known byte count) and exit")
@@ -356,6 +364,10 @@ def main():
     import itp2d_amd as I
     from itp2d_amd import workloads as W
 
+    if args.workload == "c4_batch64" and args.roofline_only:
+        insts = W.c4_instances(64)
+        print(json.dumps(sweep_roofline(I, *largest_instance(insts))), flush=True)
+        return
     if args.workload == "c4_batch64":
         c4_headline(I, W, args)
         return
