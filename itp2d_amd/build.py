@@ -1,6 +1,8 @@
 """Builds the gfx950 library itp2d_amd/lib/libitp2d_amd.so with hipcc.
 
-Sources: csrc/kernels.hip, csrc/mixed.hip (kernels), csrc/capi.hip (C ABI of include/itp2d.h),
+Sources: csrc/kernels.hip, csrc/mixed.hip, csrc/chol.hip (kernels), csrc/reg_sweep_kernels.hip (the
+register engine's sweep kernels, compiled once per precision and line length), csrc/capi.hip (C ABI of
+include/itp2d.h),
 host/itpsystem.cpp (ITP driver, C ABI of include/itp2d_system.h), host/batch.cpp (batches of
 instances on worker threads, C ABI of include/itp2d_batch.h).
 
@@ -19,14 +21,19 @@ LIB = os.path.join(LIB_DIR, "libitp2d_amd.so")
 DATAFILE_LIB = os.path.join(LIB_DIR, "libitp2d_datafile.so")
 HDF5_DIR = os.environ.get("HDF5_DIR", "/opt/conda")
 
-SOURCES = [
+SWEEP_LENGTHS = (32, 64, 128, 192, 256, 512, 1024)
+
+# (source, extra flags, object name)
+SOURCES = [(os.path.join(HERE, "csrc", "reg_sweep_kernels.hip"), [f"-DITP_SWEEP_REAL={r}", f"-DITP_SWEEP_N={n}"],
+            f"reg_sweep_{r}_{n}.o") for r in ("double", "float") for n in SWEEP_LENGTHS] + [
+    (src, [], os.path.basename(src) + ".o") for src in [
     os.path.join(HERE, "csrc", "kernels.hip"),
     os.path.join(HERE, "csrc", "mixed.hip"),
     os.path.join(HERE, "csrc", "chol.hip"),
     os.path.join(HERE, "csrc", "capi.hip"),
     os.path.join(HERE, "host", "itpsystem.cpp"),
     os.path.join(HERE, "host", "batch.cpp"),
-]
+]]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -45,16 +52,24 @@ FLAGS = [
 
 def build(verbose=False):
     os.makedirs(LIB_DIR, exist_ok=True)
-    objs, procs = [], []
-    # the translation units compile in parallel (one process each)
-    for src in SOURCES:
-        obj = os.path.join(LIB_DIR, os.path.basename(src) + ".o")
-        cmd = [HIPCC] + FLAGS + ["-I", os.path.join(ROOT, "include"), "-c", src, "-o", obj]
+    objs, failed = [], []
+    # the translation units compile in parallel, at most `jobs` processes at a time
+    jobs = max(1, min(int(os.environ.get("MAX_JOBS", os.cpu_count() or 4)), 16))
+    running = []
+    for src, extra, oname in SOURCES:
+        obj = os.path.join(LIB_DIR, oname)
+        cmd = [HIPCC] + FLAGS + extra + ["-I", os.path.join(ROOT, "include"), "-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
-        procs.append((src, subprocess.Popen(cmd)))
+        while len(running) >= jobs:
+            s0, p0 = running.pop(0)
+            if p0.wait() != 0:
+                failed.append(s0)
+        running.append((f"{src} {' '.join(extra)}", subprocess.Popen(cmd)))
         objs.append(obj)
-    failed = [src for src, p in procs if p.wait() != 0]
+    for s0, p0 in running:
+        if p0.wait() != 0:
+            failed.append(s0)
     if failed:
         raise subprocess.CalledProcessError(1, "hipcc " + " ".join(failed))
     tmp = LIB + ".tmp"

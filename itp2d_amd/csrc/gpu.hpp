@@ -8,6 +8,10 @@ namespace itp {
 
 template <class R>
 hipError_t launch_sweep(const SweepArgs<R>& a, int nblk, int nfields, hipStream_t st);
+// one register-engine sweep of line length N (reg_sweep_kernels.hip, one translation unit
+// per precision and length); the caller has checked the geometry
+template <class R, int N>
+hipError_t launch_reg_sweep_n(const SweepArgs<R>& a, int nblk, int nfields, hipStream_t st);
 hipError_t launch_energy_reduce(const cplx<double>* p1, const cplx<double>* p2, int nblk, int N, double dx,
                                 double* E, double* sigma, cplx<double>* mean, cplx<double>* msq, hipStream_t st);
 // V (sx x sy, device) = the potential kind q on the cell-centred grid

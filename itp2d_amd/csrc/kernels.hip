@@ -44,54 +44,26 @@ __global__ __launch_bounds__(256) void sweep_kernel(SweepArgs<R> a) {
   sweep_program<R>(ex, a, s, (int)blockIdx.y, (int)blockIdx.x);
 }
 
-// Per-thread state of the register engine lives in the exec object, so the
-// same pass code runs per thread on the GPU and per emulated thread on the CPU.
-// The thread index is passed through opaque() in every pass: addresses and
-// table loads derived from it then stay inside their pass instead of being
-// hoisted out of the op loop, which would pin registers for the whole kernel.
-template <class S>
-struct GpuRegExec {
-  S st;
-  template <class F>
-  __device__ void run(F&& f) {
-    f(opaque((int)threadIdx.x), st);
-    __syncthreads();
-  }
-  template <class F>
-  __device__ void local(F&& f) {
-    f(opaque((int)threadIdx.x), st);
-  }
-};
-
-// At least two waves per SIMD: the line exchanges are latency-bound, and the
-// LDS of two workgroups per CU allows it.
-template <class R, int N, int ORIENT, bool XS>
-__global__ __launch_bounds__(256, 2) void reg_sweep_kernel(SweepArgs<R> a) {
-  extern __shared__ __align__(16) char lds_raw[];
-  GpuRegExec<RegState<R, RegShape<N>::Q>> ex;
-  reg_sweep_program<R, N, ORIENT, XS>(ex, a, reinterpret_cast<cplx<R>*>(lds_raw), (int)blockIdx.y, (int)blockIdx.x);
-}
-
 template <class R>
 hipError_t launch_reg_sweep(const SweepArgs<R>& a, int nblk, int nfields, hipStream_t st) {
   const int n = a.plan.n;
+  if (!reg_supported(n)) return hipErrorInvalidValue;
   const size_t lds = reg_lds_bytes<R>(n);
   if (lds > 160 * 1024 || a.L != reg_lines_per_wg(n)) return hipErrorInvalidValue;
   const int lines = a.orient == 0 ? a.sy : a.sx;
   if ((a.orient == 0 ? a.sx : a.sy) != n || nblk * a.L != lines) return hipErrorInvalidValue;
-  hipError_t err = hipSuccess;
-  const bool xs = ops_have_xsplit(a.ops, a.nops);
-  const bool ok = reg_dispatch(n, a.orient, xs, [&](auto nc, auto oc, auto xc) {
-    constexpr int N = decltype(nc)::value, O = decltype(oc)::value;
-    constexpr bool X = decltype(xc)::value;
-    // once per instantiation (thread-safe static initialisation)
-    static const hipError_t attr_set = hipFuncSetAttribute((const void*)reg_sweep_kernel<R, N, O, X>,
-                                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    (void)attr_set;
-    hipLaunchKernelGGL((reg_sweep_kernel<R, N, O, X>), dim3(nblk, nfields), dim3(kRegThreads), lds, st, a);
-    err = hipGetLastError();
-  });
-  return ok ? err : hipErrorInvalidValue;
+  // the magnetic Dirichlet split runs along rows only
+  if (a.orient != 0 && ops_have_xsplit(a.ops, a.nops)) return hipErrorInvalidValue;
+  switch (n) {
+    case 32: return launch_reg_sweep_n<R, 32>(a, nblk, nfields, st);
+    case 64: return launch_reg_sweep_n<R, 64>(a, nblk, nfields, st);
+    case 128: return launch_reg_sweep_n<R, 128>(a, nblk, nfields, st);
+    case 192: return launch_reg_sweep_n<R, 192>(a, nblk, nfields, st);
+    case 256: return launch_reg_sweep_n<R, 256>(a, nblk, nfields, st);
+    case 512: return launch_reg_sweep_n<R, 512>(a, nblk, nfields, st);
+    case 1024: return launch_reg_sweep_n<R, 1024>(a, nblk, nfields, st);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 template <class R>

@@ -49,7 +49,7 @@ ITP_HD bool reg_supported(int n) {
   return n == 32 || n == 64 || n == 128 || n == 192 || n == 256 || n == 512 || n == 1024;
 }
 // Lines per workgroup for a line length (kRegThreads / P).
-ITP_HD int reg_lines_per_wg(int n) {
+ITP_HD constexpr int reg_lines_per_wg(int n) {
   switch (n) {
     case 32: return 64;
     case 64: return 32;
@@ -63,12 +63,30 @@ ITP_HD int reg_lines_per_wg(int n) {
 }
 // LDS line stride: one pad element per 16 (stage writes of stride 16 fall in
 // distinct banks) plus one, so consecutive lines start in distinct banks.
-ITP_HD int reg_ld(int n) { return n + n / 16 + 1; }
+ITP_HD constexpr int reg_ld(int n) { return n + n / 16 + 1; }
+// LDS of a sweep: the line buffers (or the dot-product reduction area, if
+// larger), then, when that still leaves room for two workgroups per CU, the
+// line's twiddle and Makhoul tables in the working precision (otherwise they
+// are read from global memory).
 template <class R>
-ITP_HD size_t reg_lds_bytes(int n) {
+ITP_HD constexpr size_t reg_lds_lines_bytes(int n) {
   const size_t lines = (size_t)reg_lines_per_wg(n) * reg_ld(n) * sizeof(cplx<R>);
   const size_t red = (size_t)(kRegThreads + 16) * sizeof(cplx<double>);
   return lines > red ? lines : red;
+}
+template <class R>
+ITP_HD constexpr bool reg_tabs_in_lds(int n) {
+  return reg_lds_lines_bytes<R>(n) + (size_t)2 * n * sizeof(cplx<R>) <= 80 * 1024;
+}
+template <class R>
+ITP_HD constexpr size_t reg_lds_bytes(int n) {
+  return reg_lds_lines_bytes<R>(n) + (reg_tabs_in_lds<R>(n) ? (size_t)2 * n * sizeof(cplx<R>) : 0);
+}
+// A twiddle or Makhoul factor in the working precision, from either table kind.
+template <class R, class T>
+ITP_HD cplx<R> tab_val(const cplx<T>* t, int i) {
+  const cplx<T> v = t[i];
+  return mk<R>(R(v.x), R(v.y));
 }
 
 ITP_HD int lpad(int i) { return i + (i >> 4); }
@@ -204,8 +222,8 @@ ITP_HD void dft_reg(cplx<R>* v) {
 // twiddles by w_n^{r (j mod NS) n/(NS RAD)}, and writes positions
 // (j/NS) NS RAD + (j mod NS) + r NS: to LDS, or, in the last stage
 // (NS RAD = n), back to registers b + r Q/RAD in natural order.
-template <class R, int N, int RAD, int NS, bool INV, bool LAST>
-ITP_HD void reg_stage(cplx<R>* A, int t, cplx<R>* line, const cplx<double>* tw) {
+template <class R, int N, int RAD, int NS, bool INV, bool LAST, class TW>
+ITP_HD void reg_stage(cplx<R>* A, int t, cplx<R>* line, const TW* tw) {
   constexpr int P = RegShape<N>::P, Q = RegShape<N>::Q;
   constexpr int NB = Q / RAD;
   constexpr int TWS = N / (NS * RAD);
@@ -219,8 +237,8 @@ ITP_HD void reg_stage(cplx<R>* A, int t, cplx<R>* line, const cplx<double>* tw) 
     if constexpr (NS > 1) {
 #pragma unroll
       for (int r = 1; r < RAD; r++) {
-        const cplx<double> w = tw[r * k * TWS];
-        x[r] = x[r] * mk<R>(R(w.x), INV ? R(-w.y) : R(w.y));
+        const cplx<R> w = tab_val<R>(tw, r * k * TWS);
+        x[r] = x[r] * (INV ? conj(w) : w);
       }
     }
     dft_reg<R, RAD, INV>(x);
@@ -251,8 +269,8 @@ ITP_HD void reg_read_natural(Exec& ex, cplx<R>* lds) {
 }
 
 // Complex DFT of every line (forward e^{-}, or inverse e^{+}), unnormalised.
-template <class G, bool INV, int WHICH, class R, class Exec>
-ITP_HD void reg_fft(Exec& ex, cplx<R>* lds, const cplx<double>* tw) {
+template <class G, bool INV, int WHICH, class R, class Exec, class TW>
+ITP_HD void reg_fft(Exec& ex, cplx<R>* lds, const TW* tw) {
   using S = RegShape<G::n>;
   constexpr int N = G::n;
   if constexpr (S::NST == 2) {
@@ -332,16 +350,15 @@ ITP_HD R elem_factor(const LineOp& op, long long g, int pos) {
 
 // P(p) for a pending kind known at compile time. fg, fp: the pending factor's
 // table strides in (global offset, position); ft its table.
-template <class TAG, class R, int N>
-ITP_HD cplx<R> pend_value(const cplx<R>* line, int p, long long g, const cplx<double>* mkt, const double* ft,
+template <class TAG, class R, int N, class MK>
+ITP_HD cplx<R> pend_value(const cplx<R>* line, int p, long long g, const MK* mkt, const double* ft,
                           long long fsg, long long fsp) {
   constexpr int n = N;
   constexpr int K = TAG::kind;
   cplx<R> v;
   if constexpr (K == PK_R2F_DCT || K == PK_R2F_DST) {
     const int k = K == PK_R2F_DST ? n - 1 - p : p;
-    const cplx<double> wd = mkt[opaque(k)];
-    const cplx<R> w = mk<R>(R(wd.x), R(wd.y));
+    const cplx<R> w = tab_val<R>(mkt, opaque(k));
     v = w * line[lpad(k)] + conj(w) * line[lpad(k == 0 ? 0 : n - k)];
     if constexpr (TAG::elem) v = v * R(ft[g * fsg + p * fsp]);
   } else if constexpr (K == PK_PERM_DCT || K == PK_PERM_DST) {
@@ -404,7 +421,27 @@ ITP_HD void reg_sweep_program(Exec& ex, const SweepArgs<R>& a, cplx<R>* lds, int
   const int line0 = blk * G::LPW;
   const long long fo = (long long)field * a.fstride;
   const LineOp* ops = a.ops;
-  const cplx<double>* mkt = a.mk;
+  // twiddles and Makhoul factors, staged in LDS where they fit (every later
+  // read of them follows at least one workgroup barrier)
+  constexpr bool TL = reg_tabs_in_lds<R>(N);
+  using TabT = std::conditional_t<TL, cplx<R>, cplx<double>>;
+  const TabT* ltw;
+  const TabT* mkt;
+  if constexpr (TL) {
+    cplx<R>* t = lds + reg_lds_lines_bytes<R>(N) / sizeof(cplx<R>);
+    cplx<R>* m = t + N;
+    ex.local([&](int tid, auto&) {
+      for (int i = tid; i < N; i += kRegThreads) {
+        t[i] = tab_val<R>(a.tw, i);
+        m[i] = tab_val<R>(a.mk, i);
+      }
+    });
+    ltw = t;
+    mkt = m;
+  } else {
+    ltw = a.tw;
+    mkt = a.mk;
+  }
   auto goff = [&](int l, int pos) -> long long {
     return ORIENT == 0 ? (long long)(line0 + l) * sx + pos : (long long)pos * sx + (line0 + l);
   };
@@ -450,7 +487,7 @@ ITP_HD void reg_sweep_program(Exec& ex, const SweepArgs<R>& a, cplx<R>* lds, int
   RegPending pd = {PK_NONE, -1};
   int o = lead;
   if (fold_in) {
-    reg_fft<G, false, 0>(ex, lds, a.tw);
+    reg_fft<G, false, 0>(ex, lds, ltw);
     pd.kind = in_dst ? PK_R2F_DST : PK_R2F_DCT;
     o = lead + 1;
   }
@@ -501,8 +538,8 @@ ITP_HD void reg_sweep_program(Exec& ex, const SweepArgs<R>& a, cplx<R>* lds, int
     }
     if (code == LOP_FFT_FWD || code == LOP_FFT_INV) {
       materialize();
-      if (code == LOP_FFT_FWD) reg_fft<G, false, 0>(ex, lds, a.tw);
-      else reg_fft<G, true, 0>(ex, lds, a.tw);
+      if (code == LOP_FFT_FWD) reg_fft<G, false, 0>(ex, lds, ltw);
+      else reg_fft<G, true, 0>(ex, lds, ltw);
       continue;
     }
     const RegPending pc = pd;
@@ -519,7 +556,7 @@ ITP_HD void reg_sweep_program(Exec& ex, const SweepArgs<R>& a, cplx<R>* lds, int
           return (dst && (j & 1)) ? -v : v;
         });
       });
-      reg_fft<G, false, 0>(ex, lds, a.tw);
+      reg_fft<G, false, 0>(ex, lds, ltw);
       pd.kind = dst ? PK_R2F_DST : PK_R2F_DCT;
     } else if (lop_r2i(code)) {
       // DCT-III / DST-III: U_0 = X_0, U_k = w_k (X_k + i X_{n-k}); DST-III reads X reversed
@@ -531,12 +568,11 @@ ITP_HD void reg_sweep_program(Exec& ex, const SweepArgs<R>& a, cplx<R>* lds, int
           const int pb = k == 0 ? pa : (dst ? k - 1 : n - k);
           const cplx<R> xa = pend_value<T, R, N>(line, pa, goff(l, pa), mkt, ft, fsg, fsp);
           const cplx<R> xb = pend_value<T, R, N>(line, pb, goff(l, pb), mkt, ft, fsg, fsp);
-          const cplx<double> wd = mkt[opaque(k)];
-          const cplx<R> u = mk<R>(R(wd.x), R(wd.y)) * (xa + mul_i(xb));
+          const cplx<R> u = tab_val<R>(mkt, opaque(k)) * (xa + mul_i(xb));
           return k == 0 ? xa : u;
         });
       });
-      reg_fft<G, false, 0>(ex, lds, a.tw);
+      reg_fft<G, false, 0>(ex, lds, ltw);
       pd.kind = dst ? PK_PERM_DST : PK_PERM_DCT;
     } else if (code == LOP_XSPLIT) {
       if constexpr (XS) {
@@ -568,8 +604,7 @@ ITP_HD void reg_sweep_program(Exec& ex, const SweepArgs<R>& a, cplx<R>* lds, int
             const cplx<R> bpa = pend_value<T, R, N>(line, pa, goff(l, pa), mkt, ft, fsg, fsp);
             const cplx<R> bb = pend_value<T, R, N>(line, pb, goff(l, pb), mkt, ft, fsg, fsp);
             const cplx<R> ba = bpa * R(X[row + pa]);
-            const cplx<double> wd = mkt[opaque(k)];
-            const cplx<R> w = mk<R>(R(wd.x), R(wd.y));
+            const cplx<R> w = tab_val<R>(mkt, opaque(k));
             const cplx<R> uv = w * (ba + mul_i(bb * R(X[row + pb])));
             const cplx<R> uw = w * (mul_i(bb * R(X2[row + pb])) - bpa * R(X2[row + pa]));
             st.v[q] = k == 0 ? ba : uv;
@@ -577,8 +612,8 @@ ITP_HD void reg_sweep_program(Exec& ex, const SweepArgs<R>& a, cplx<R>* lds, int
           }
         });
       });
-      reg_fft<G, false, 0>(ex, lds, a.tw);
-      reg_fft<G, false, 1>(ex, lds, a.tw);
+      reg_fft<G, false, 0>(ex, lds, ltw);
+      reg_fft<G, false, 1>(ex, lds, ltw);
       // combine before the shared output permutation: DST-III's (-1)^j sign
       // falls on the FFT outputs m >= n/2
       ex.local([&](int tid, auto& st) {

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <ctime>
@@ -544,15 +545,22 @@ class ITPSystem {
 
   void check_timestep_convergence() {  // itpsystem.cpp:291-332
     // The reference's flags are sticky: a state that passed once stays
-    // passed until the time step changes. After the mixed path's hand-off the
-    // flags are recomputed every step instead: its FP64 steps start from
-    // states whose remaining error is rounding noise spread over nearly
-    // degenerate levels, whose energies wander around a tolerance far below
-    // FP32 reach, and sticky flags would then collect passes from different
-    // steps and trigger a time-step change the reference run never makes.
+    // passed until the time step changes, and when the last one passes, the
+    // final test decides between finishing and a smaller time step. Whether
+    // every state passes again at that step is a property of the trajectory
+    // (in nearly degenerate levels the sorted energies drift for many
+    // steps), and the mixed path's FP64 steps follow the same trajectory when
+    // the hand-off tolerance is within FP32 reach (mixed_tol >= 1e-10), so
+    // they keep sticky flags. Below that the hand-off comes at the FP32
+    // floor: the FP64 steps then start by washing out rounding noise spread
+    // over nearly degenerate levels, whose energies wander around the
+    // tolerance, and sticky flags would collect passes from different steps
+    // and trigger a time-step change the reference run never makes, so the
+    // flags are recomputed every step instead.
+    const bool sticky = !after_handoff || params.mixed_tol >= 1e-10;
     for (size_t n = 0; n < params.N; n++) {
       const bool pass = test(params.timestep_test, n);
-      if (pass || after_handoff) set_ts(std::get<2>(esn[n]), pass);
+      if (pass || !sticky) set_ts(std::get<2>(esn[n]), pass);
     }
     bool flag = true;
     for (size_t n = params.ignore_lowest; n < params.ignore_lowest + params.n_wanted; n++)
