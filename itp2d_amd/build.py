@@ -50,14 +50,30 @@ FLAGS = [
 ]
 
 
-def build(verbose=False):
+def _newest_input(src):
+    # the source and every header it can include
+    heads = [os.path.join(d, f) for d in (os.path.join(HERE, "csrc"), os.path.join(HERE, "host"),
+                                          os.path.join(ROOT, "include")) for f in os.listdir(d)
+             if f.endswith((".hpp", ".h"))]
+    return max(os.path.getmtime(f) for f in [src] + heads)
+
+
+def build(verbose=False, incremental=False):
+    """Compiles every translation unit and links the library. incremental=True
+    keeps the objects (under lib/obj) and recompiles only those older than
+    their source or a header."""
     os.makedirs(LIB_DIR, exist_ok=True)
+    obj_dir = os.path.join(LIB_DIR, "obj") if incremental else LIB_DIR
+    os.makedirs(obj_dir, exist_ok=True)
     objs, failed = [], []
     # the translation units compile in parallel, at most `jobs` processes at a time
     jobs = max(1, min(int(os.environ.get("MAX_JOBS", os.cpu_count() or 4)), 16))
     running = []
     for src, extra, oname in SOURCES:
-        obj = os.path.join(LIB_DIR, oname)
+        obj = os.path.join(obj_dir, oname)
+        if incremental and os.path.exists(obj) and os.path.getmtime(obj) > _newest_input(src):
+            objs.append(obj)
+            continue
         cmd = [HIPCC] + FLAGS + extra + ["-I", os.path.join(ROOT, "include"), "-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
@@ -78,8 +94,9 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
-    for o in objs:
-        os.remove(o)
+    if not incremental:
+        for o in objs:
+            os.remove(o)
     build_datafile(verbose)
     return LIB
 

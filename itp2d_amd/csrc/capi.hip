@@ -118,7 +118,8 @@ struct itp_stateset {
 struct MemberTabs {
   ExpKinTabs host;
   std::vector<double> vh, vf, vp;
-  double* dev = nullptr;  // one allocation for all tables of the member
+  double* dev = nullptr;   // one allocation for all tables of the member
+  float* dev32 = nullptr;  // and one for their FP32 copies (FP32 sweep programs)
   MemberDev d;
 };
 
@@ -301,8 +302,44 @@ int upload_tables(itp_ctx* c, const std::vector<const std::vector<double>*>& tab
   return ITP_OK;
 }
 
+// FP32 copies of host tables in one device allocation; ptrs[i] is null for an empty table
+int upload_tables32(itp_ctx* c, std::initializer_list<const std::vector<double>*> tabs, float** dev,
+                    std::vector<const float*>& ptrs) {
+  size_t total = 0;
+  for (auto t : tabs) total += t->size();
+  int rc = dalloc(c, (void**)dev, std::max<size_t>(total, 1) * sizeof(float));
+  if (rc) return rc;
+  std::vector<float> h(total);
+  size_t off = 0;
+  ptrs.clear();
+  for (auto t : tabs) {
+    ptrs.push_back(t->empty() ? nullptr : *dev + off);
+    for (size_t i = 0; i < t->size(); i++) h[off + i] = float((*t)[i]);
+    off += t->size();
+  }
+  if (total) HIPCK(c, hipMemcpy(*dev, h.data(), total * sizeof(float), hipMemcpyHostToDevice));
+  return ITP_OK;
+}
+
 void free_member(MemberTabs* m) {
   if (m && m->dev) { (void)hipFree(m->dev); m->dev = nullptr; }
+  if (m && m->dev32) { (void)hipFree(m->dev32); m->dev32 = nullptr; }
+}
+
+// FP32 copies of a member's tables (for FP32 sweep programs)
+int upload_member32(itp_ctx* c, MemberTabs* m) {
+  std::vector<const float*> q;
+  int rc = upload_tables32(c, {&m->host.mult, &m->host.xm, &m->host.xm2, &m->host.ym, &m->vh, &m->vf, &m->vp},
+                           &m->dev32, q);
+  if (rc) return rc;
+  m->d.kin.mult32 = q[0];
+  m->d.kin.xm32 = q[1];
+  m->d.kin.xm2_32 = q[2];
+  m->d.kin.ym32 = q[3];
+  m->d.vhalf32 = q[4];
+  m->d.vfull32 = q[5];
+  m->d.vpref32 = q[6];
+  return ITP_OK;
 }
 
 // (Re)build the device tables of every MultiProductSplit member for time step eps
@@ -334,6 +371,8 @@ int build_split_tables(itp_split* sp) {
     m->d.vhalf = p[4];
     m->d.vfull = p[5];
     m->d.vpref = p[6];
+    rc = upload_member32(c, m.get());
+    if (rc) return rc;
     sp->members.push_back(std::move(m));
   }
   return ITP_OK;

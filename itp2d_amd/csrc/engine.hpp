@@ -212,7 +212,11 @@ inline std::vector<double> mps_coefficients(int h) {
 }
 
 // Device-resident (or emulator-resident) copy of one ExpKinetic's tables.
-struct KinDev { const double* mult = nullptr; const double* xm = nullptr; const double* xm2 = nullptr; const double* ym = nullptr; };
+// (the *32 pointers: FP32 copies of the same tables, or null)
+struct KinDev {
+  const double* mult = nullptr; const double* xm = nullptr; const double* xm2 = nullptr; const double* ym = nullptr;
+  const float* mult32 = nullptr; const float* xm32 = nullptr; const float* xm2_32 = nullptr; const float* ym32 = nullptr;
+};
 
 // The x-direction halves of one kinetic application, per branch.
 //   forward part (first row sweep of an application)
@@ -223,11 +227,11 @@ inline void fwd_x_ops(Branch br, const KinDev& k, std::vector<LineOp>& ops) {
     case BR_B0D: ops.push_back({LOP_DST2, nullptr, nullptr, 0}); break;
     case BR_BP:
       ops.push_back({LOP_FFT_FWD, nullptr, nullptr, 0});
-      ops.push_back({LOP_MUL_TAB, k.xm, nullptr, 0});
+      ops.push_back({LOP_MUL_TAB, k.xm, nullptr, 0, k.xm32, nullptr});
       break;
     case BR_BD:
       ops.push_back({LOP_DST2, nullptr, nullptr, 0});
-      ops.push_back({LOP_XSPLIT, k.xm, k.xm2, 0});
+      ops.push_back({LOP_XSPLIT, k.xm, k.xm2, 0, k.xm32, k.xm2_32});
       ops.push_back({LOP_DST2, nullptr, nullptr, 0});
       break;
   }
@@ -237,18 +241,18 @@ inline void inv_x_ops(Branch br, const KinDev& k, std::vector<LineOp>& ops) {
     case BR_B0P: ops.push_back({LOP_FFT_INV, nullptr, nullptr, 0}); break;
     case BR_B0D: ops.push_back({LOP_DST3, nullptr, nullptr, 0}); break;
     case BR_BP:
-      ops.push_back({LOP_MUL_TAB, k.xm, nullptr, 0});
+      ops.push_back({LOP_MUL_TAB, k.xm, nullptr, 0, k.xm32, nullptr});
       ops.push_back({LOP_FFT_INV, nullptr, nullptr, 0});
       break;
-    case BR_BD: ops.push_back({LOP_XSPLIT, k.xm, k.xm2, 0}); break;
+    case BR_BD: ops.push_back({LOP_XSPLIT, k.xm, k.xm2, 0, k.xm32, k.xm2_32}); break;
   }
 }
 // column sweep: y transform, multiplier, inverse y transform
 inline void col_ops(Branch br, const KinDev& k, std::vector<LineOp>& ops) {
   const bool per = (br == BR_B0P || br == BR_BP);
   ops.push_back({per ? LOP_FFT_FWD : LOP_DST2, nullptr, nullptr, 0});
-  if (br == BR_B0P || br == BR_B0D) ops.push_back({LOP_MUL_TAB, k.mult, nullptr, 0});
-  else ops.push_back({LOP_MUL_POS, k.ym, nullptr, 0});
+  if (br == BR_B0P || br == BR_B0D) ops.push_back({LOP_MUL_TAB, k.mult, nullptr, 0, k.mult32, nullptr});
+  else ops.push_back({LOP_MUL_POS, k.ym, nullptr, 0, k.ym32, nullptr});
   ops.push_back({per ? LOP_FFT_INV : LOP_DST3, nullptr, nullptr, 0});
 }
 
@@ -326,6 +330,9 @@ struct MemberDev {
   const double* vhalf = nullptr;  // exp(-0.5 h V)                (potential_part)
   const double* vfull = nullptr;  // exp(-h V)                    (potential_part_square)
   const double* vpref = nullptr;  // c_t exp(-0.5 h V)            (potential_with_prefactor)
+  const float* vhalf32 = nullptr;  // FP32 copies (or null)
+  const float* vfull32 = nullptr;
+  const float* vpref32 = nullptr;
 };
 
 // Sweep sequence of one SecondOrderSplit member with exponent t applied to
@@ -341,7 +348,7 @@ void member_program(Branch br, const MemberDev& m, int t, bool has_v, const cplx
   {
     Sweep<R> s;
     s.orient = 0;
-    if (has_v) s.ops.push_back({LOP_MUL_TAB, m.vhalf, nullptr, 0});
+    if (has_v) s.ops.push_back({LOP_MUL_TAB, m.vhalf, nullptr, 0, m.vhalf32, nullptr});
     fwd_x_ops(br, m.kin, s.ops);
     s.src = in; s.dst = work;
     prog.push_back(s);
@@ -356,11 +363,11 @@ void member_program(Branch br, const MemberDev& m, int t, bool has_v, const cplx
     r.orient = 0;
     inv_x_ops(br, m.kin, r.ops);
     if (k < t) {
-      r.ops.push_back({LOP_MUL_TAB, m.vfull, nullptr, 0});
+      r.ops.push_back({LOP_MUL_TAB, m.vfull, nullptr, 0, m.vfull32, nullptr});
       fwd_x_ops(br, m.kin, r.ops);
       r.src = work; r.dst = work;
     } else {
-      if (has_v) r.ops.push_back({LOP_MUL_TAB, m.vpref, nullptr, 0});
+      if (has_v) r.ops.push_back({LOP_MUL_TAB, m.vpref, nullptr, 0, m.vpref32, nullptr});
       r.src = work; r.dst = out;
       if (accumulate) r.add1 = out;
     }

@@ -112,6 +112,11 @@ struct LineOp {
   const double* t0;
   const double* t1;
   double s;
+  // FP32 copies of t0 and t1, read by compile-time FP32 sweep programs
+  // (reg_sweep.hpp); null where no copy exists (aggregate initialisation
+  // without them leaves them null)
+  const float* f0;
+  const float* f1;
 };
 
 constexpr int kMaxOps = 8;

@@ -25,6 +25,7 @@
 #pragma once
 
 #include <type_traits>
+#include <utility>
 
 #include "line_ops.hpp"
 
@@ -339,19 +340,54 @@ struct RegPending {
 
 ITP_HD int perm_src(int n, int p) { return (p & 1) ? (n - 1 - (p >> 1)) : (p >> 1); }
 
-// Factor of an elementwise op at global offset g and line position pos
-// (MUL_TAB: t0[g], MUL_POS: t0[pos], SCALE: s).
-template <class R>
-ITP_HD R elem_factor(const LineOp& op, long long g, int pos) {
-  if (op.code == LOP_MUL_TAB) return R(op.t0[g]);
-  if (op.code == LOP_MUL_POS) return R(op.t0[pos]);
-  return R(op.s);
+// Factor of an elementwise op with code `code`, table t and scale s at
+// global offset g and line position pos (MUL_TAB: t[g], MUL_POS: t[pos],
+// SCALE: s).
+template <class R, class TT>
+ITP_HD R elem_factor(int code, const TT* t, double s, long long g, int pos) {
+  if (code == LOP_MUL_TAB) return R(t[g]);
+  if (code == LOP_MUL_POS) return R(t[pos]);
+  return R(s);
+}
+
+// A sweep program known at compile time: PG::n op codes in PG::c. The
+// runtime op list (SweepArgs::ops) then only supplies the tables; the
+// kernel's op loop unrolls into straight-line code, and FP32 programs read
+// the FP32 copies of the tables (LineOp::f0, f1). RtProg: the op list is
+// read at run time.
+struct RtProg {
+  static constexpr int n = -1;
+  static constexpr int c[kMaxOps] = {};
+};
+template <int... C>
+struct CtProg {
+  static constexpr int n = sizeof...(C);
+  static constexpr int c[kMaxOps] = {C...};
+};
+// Index of the first op that is not elementwise.
+template <class PG>
+ITP_HD constexpr int ct_lead() {
+  int i = 0;
+  while (i < PG::n && (PG::c[i] == LOP_MUL_TAB || PG::c[i] == LOP_MUL_POS || PG::c[i] == LOP_SCALE)) i++;
+  return i;
+}
+// Does the compile-time program match the runtime op list?
+template <class PG>
+ITP_HD bool ct_matches(const LineOp* ops, int nops) {
+  if (PG::n != nops) return false;
+  for (int i = 0; i < nops; i++)
+    if (ops[i].code != PG::c[i]) return false;
+  return true;
+}
+template <class F, int... I>
+ITP_HD void static_for(F&& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
 }
 
 // P(p) for a pending kind known at compile time. fg, fp: the pending factor's
 // table strides in (global offset, position); ft its table.
-template <class TAG, class R, int N, class MK>
-ITP_HD cplx<R> pend_value(const cplx<R>* line, int p, long long g, const MK* mkt, const double* ft,
+template <class TAG, class R, int N, class MK, class TT>
+ITP_HD cplx<R> pend_value(const cplx<R>* line, int p, long long g, const MK* mkt, const TT* ft,
                           long long fsg, long long fsp) {
   constexpr int n = N;
   constexpr int K = TAG::kind;
@@ -412,11 +448,15 @@ ITP_HD void reg_exchange(Exec& ex, cplx<R>* lds, F&& f) {
 // The whole sweep for line block blk of field `field`. XS: the program may
 // contain the magnetic Dirichlet split (rows only); without it the code path
 // and its second register set are compiled out.
-template <class R, int N, int ORIENT, bool XS, class Exec>
+template <class R, int N, int ORIENT, bool XS, class PG = RtProg, class Exec>
 ITP_HD void reg_sweep_program(Exec& ex, const SweepArgs<R>& a, cplx<R>* lds, int field, int blk) {
   using G = RegGeom<N, ORIENT>;
   constexpr int P = G::P, Q = G::Q, n = N;
-  const int nops = a.nops;
+  constexpr bool CT = PG::n >= 0;
+  // tables of elementwise ops: FP32 copies for compile-time FP32 programs
+  constexpr bool F32T = CT && sizeof(R) == 4;
+  using TT = std::conditional_t<F32T, float, double>;
+  const int nops = CT ? PG::n : a.nops;
   const int sx = a.sx;
   const int line0 = blk * G::LPW;
   const long long fo = (long long)field * a.fstride;
@@ -445,12 +485,26 @@ ITP_HD void reg_sweep_program(Exec& ex, const SweepArgs<R>& a, cplx<R>* lds, int
   auto goff = [&](int l, int pos) -> long long {
     return ORIENT == 0 ? (long long)(line0 + l) * sx + pos : (long long)pos * sx + (line0 + l);
   };
+  auto code_at = [&](int i) -> int {
+    if constexpr (CT) return PG::c[i];
+    else return ops[i].code;
+  };
+  auto tab0 = [&](const LineOp& op) -> const TT* {
+    if constexpr (F32T) return op.f0;
+    else return op.t0;
+  };
+  auto tab1 = [&](const LineOp& op) -> const TT* {
+    if constexpr (F32T) return op.f1;
+    else return op.t1;
+  };
   // leading elementwise ops, then possibly a forward r2r whose input
   // permutation is taken from global memory
   int lead = 0;
-  while (lead < nops && lop_elementwise(ops[lead].code)) lead++;
-  const bool fold_in = lead < nops && lop_r2f(ops[lead].code);
-  const bool in_dst = fold_in && ops[lead].code == LOP_DST2;
+  if constexpr (CT) lead = ct_lead<PG>();
+  else
+    while (lead < nops && lop_elementwise(code_at(lead))) lead++;
+  const bool fold_in = lead < nops && lop_r2f(code_at(lead));
+  const bool in_dst = fold_in && code_at(lead) == LOP_DST2;
 
   // load
   {
@@ -469,14 +523,16 @@ ITP_HD void reg_sweep_program(Exec& ex, const SweepArgs<R>& a, cplx<R>* lds, int
         }
       });
       for (int i = 0; i < lead; i++) {
-        const LineOp op = ops[i];
+        const LineOp& op = ops[i];
+        const int code = code_at(i);
+        const TT* t0 = tab0(op);
         ex.local([&](int tid, auto& st) {
           const int t = G::t_of(tid), l = G::l_of(tid);
 #pragma unroll
           for (int q = 0; q < Q; q++) {
             const int m = t + P * q;
             const int pos = FI ? r2f_src(n, m) : m;
-            st.v[q] = st.v[q] * elem_factor<R>(op, goff(l, pos), pos);
+            st.v[q] = st.v[q] * elem_factor<R>(code, t0, op.s, goff(l, pos), pos);
           }
         });
       }
@@ -485,27 +541,26 @@ ITP_HD void reg_sweep_program(Exec& ex, const SweepArgs<R>& a, cplx<R>* lds, int
     else load(std::integral_constant<bool, false>{});
   }
   RegPending pd = {PK_NONE, -1};
-  int o = lead;
+  const int start = fold_in ? lead + 1 : lead;
   if (fold_in) {
     reg_fft<G, false, 0>(ex, lds, ltw);
     pd.kind = in_dst ? PK_R2F_DST : PK_R2F_DCT;
-    o = lead + 1;
   }
   // the pending factor's table and strides
-  auto ftab = [&](const RegPending& pc, const double*& ft, long long& fsg, long long& fsp) {
+  auto ftab = [&](const RegPending& pc, const TT*& ft, long long& fsg, long long& fsp) {
     ft = nullptr; fsg = 0; fsp = 0;
     if (pc.el >= 0) {
-      const LineOp& e = ops[pc.el];
-      ft = e.t0;
-      fsg = e.code == LOP_MUL_TAB ? 1 : 0;
-      fsp = e.code == LOP_MUL_POS ? 1 : 0;
+      const int c = code_at(pc.el);
+      ft = tab0(ops[pc.el]);
+      fsg = c == LOP_MUL_TAB ? 1 : 0;
+      fsp = c == LOP_MUL_POS ? 1 : 0;
     }
   };
   // bring registers to natural order (one exchange if a post-step is pending)
   auto materialize = [&]() {
     if (pd.kind == PK_NONE) return;
     const RegPending pc = pd;
-    const double* ft; long long fsg, fsp;
+    const TT* ft; long long fsg, fsp;
     ftab(pc, ft, fsg, fsp);
     with_pend(pc, [&](auto tag) {
       reg_exchange<G, 0>(ex, lds, [&](int, int l, int p, const cplx<R>* line) {
@@ -515,35 +570,38 @@ ITP_HD void reg_sweep_program(Exec& ex, const SweepArgs<R>& a, cplx<R>* lds, int
     pd.kind = PK_NONE;
     pd.el = -1;
   };
-  for (; o < nops; o++) {
-    const LineOp op = ops[o];
-    const int code = op.code;
+  // one op of the program (for compile-time programs o is a constant after
+  // inlining, so every branch on the op code and the pending state folds)
+  auto do_op = [&](int o) __attribute__((always_inline)) {
+    const LineOp& op = ops[o];
+    const int code = code_at(o);
     if (lop_elementwise(code)) {
-      if ((pd.kind == PK_R2F_DCT || pd.kind == PK_R2F_DST) && op.code != LOP_SCALE) {
+      if ((pd.kind == PK_R2F_DCT || pd.kind == PK_R2F_DST) && code != LOP_SCALE) {
         if (pd.el >= 0) materialize();  // at most one pending factor
-        else { pd.el = o; continue; }
+        else { pd.el = o; return; }
       }
       // nothing or a permutation pending: apply at the logical position now
       const bool perm = pd.kind == PK_PERM_DCT || pd.kind == PK_PERM_DST;
+      const TT* t0 = tab0(op);
       ex.local([&](int tid, auto& st) {
         const int t = G::t_of(tid), l = G::l_of(tid);
 #pragma unroll
         for (int q = 0; q < Q; q++) {
           const int m = t + P * q;
           const int pos = perm ? r2i_dst(n, m) : m;
-          st.v[q] = st.v[q] * elem_factor<R>(op, goff(l, pos), pos);
+          st.v[q] = st.v[q] * elem_factor<R>(code, t0, op.s, goff(l, pos), pos);
         }
       });
-      continue;
+      return;
     }
     if (code == LOP_FFT_FWD || code == LOP_FFT_INV) {
       materialize();
       if (code == LOP_FFT_FWD) reg_fft<G, false, 0>(ex, lds, ltw);
       else reg_fft<G, true, 0>(ex, lds, ltw);
-      continue;
+      return;
     }
     const RegPending pc = pd;
-    const double* ft; long long fsg, fsp;
+    const TT* ft; long long fsg, fsp;
     ftab(pc, ft, fsg, fsp);
     if (lop_r2f(code)) {
       // DCT-II / DST-II: FFT input m holds +-P(j(m)) (Makhoul's reordering)
@@ -581,8 +639,8 @@ ITP_HD void reg_sweep_program(Exec& ex, const SweepArgs<R>& a, cplx<R>* lds, int
       // kinetic.cpp:128-134). Both pre-steps need only b_{n-1-k} and b_{k-1}:
       //   U^v_k = w_k (v_{n-1-k} + i v_{k-1}),  U^v_0 = v_{n-1}
       //   U^w_k = w_k (w_k + i w_{n-k}),        U^w_0 = 0
-      const double* X = op.t0;
-      const double* X2 = op.t1;
+      const TT* X = tab0(op);
+      const TT* X2 = tab1(op);
       with_pend(pc, [&](auto tag) {
         using T = decltype(tag);
         ex.run([&](int tid, auto& st) {
@@ -628,6 +686,14 @@ ITP_HD void reg_sweep_program(Exec& ex, const SweepArgs<R>& a, cplx<R>* lds, int
       }
     }
     pd.el = -1;
+  };
+  if constexpr (CT) {
+    static_for([&](auto oc) __attribute__((always_inline)) {
+      constexpr int o = decltype(oc)::value;
+      if (o >= start) do_op(o);
+    }, std::make_integer_sequence<int, (PG::n > 0 ? PG::n : 0)>{});
+  } else {
+    for (int o = start; o < nops; o++) do_op(o);
   }
   // a pending pair combination needs one more exchange; a pending
   // permutation is applied by the store itself

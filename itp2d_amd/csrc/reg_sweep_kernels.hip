@@ -3,6 +3,12 @@
 // (float or double) and ITP_SWEEP_N. The build compiles this file once per
 // pair, so the instantiations compile in parallel; launch_reg_sweep in
 // kernels.hip picks the launcher for a sweep's line length.
+//
+// Besides the kernel that reads its op list at run time, the sweeps of the
+// split-operator members (engine.hpp member_program: head, column, inner and
+// last sweeps of each kinetic branch, with a potential) have kernels compiled
+// for their op lists: their op loop is straight-line code, and in FP32 they
+// read the FP32 copies of the tables.
 #include <hip/hip_runtime.h>
 
 #include "gpu.hpp"
@@ -35,27 +41,85 @@ struct GpuRegExec {
 
 // At least two waves per SIMD: the line exchanges are latency-bound, and the
 // LDS of two workgroups per CU allows it.
-template <class R, int N, int ORIENT, bool XS>
+template <class R, int N, int ORIENT, bool XS, class PG>
 __global__ __launch_bounds__(256, 2) void reg_sweep_kernel(SweepArgs<R> a) {
   extern __shared__ __align__(16) char lds_raw[];
   GpuRegExec<RegState<R, RegShape<N>::Q>> ex;
-  reg_sweep_program<R, N, ORIENT, XS>(ex, a, reinterpret_cast<cplx<R>*>(lds_raw), (int)blockIdx.y, (int)blockIdx.x);
+  reg_sweep_program<R, N, ORIENT, XS, PG>(ex, a, reinterpret_cast<cplx<R>*>(lds_raw), (int)blockIdx.y,
+                                           (int)blockIdx.x);
 }
+
+namespace {
+
+constexpr int T_ = LOP_MUL_TAB, P_ = LOP_MUL_POS, F_ = LOP_FFT_FWD, I_ = LOP_FFT_INV, D2 = LOP_DST2,
+              D3 = LOP_DST3, X_ = LOP_XSPLIT;
+
+// member sweeps with a potential (engine.hpp fwd_x_ops, inv_x_ops, col_ops,
+// member_program), per kinetic branch: B = 0 periodic, B != 0 periodic,
+// B = 0 Dirichlet, B != 0 Dirichlet
+using RowPrograms = std::tuple<
+    CtProg<T_, F_>, CtProg<I_, T_, F_>, CtProg<I_, T_>,                      // B = 0 periodic
+    CtProg<T_, F_, T_>, CtProg<T_, I_, T_, F_, T_>, CtProg<T_, I_, T_>,       // B != 0 periodic
+    CtProg<T_, D2>, CtProg<D3, T_, D2>, CtProg<D3, T_>>;                      // B = 0 Dirichlet
+using XsRowPrograms = std::tuple<CtProg<T_, D2, X_, D2>, CtProg<X_, T_, D2, X_, D2>, CtProg<X_, T_>>;
+using ColPrograms = std::tuple<CtProg<F_, T_, I_>, CtProg<F_, P_, I_>, CtProg<D2, T_, D3>, CtProg<D2, P_, D3>>;
+
+template <class R, int N, int ORIENT, bool XS, class PG>
+hipError_t launch_one(const SweepArgs<R>& a, int nblk, int nfields, hipStream_t st) {
+  // once per kernel (thread-safe static initialisation)
+  static const hipError_t attr_set = hipFuncSetAttribute((const void*)reg_sweep_kernel<R, N, ORIENT, XS, PG>,
+                                                         hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  (void)attr_set;
+  hipLaunchKernelGGL((reg_sweep_kernel<R, N, ORIENT, XS, PG>), dim3(nblk, nfields), dim3(kRegThreads),
+                     reg_lds_bytes<R>(N), st, a);
+  return hipGetLastError();
+}
+
+// Launches the first compile-time program of the tuple that matches the op
+// list; false if none does.
+template <class R, int N, int ORIENT, bool XS, class... PGs>
+bool launch_ct(std::tuple<PGs...>*, const SweepArgs<R>& a, int nblk, int nfields, hipStream_t st, hipError_t& err) {
+  bool done = false;
+  auto one = [&](auto* pg) {
+    using PG = std::remove_pointer_t<decltype(pg)>;
+    if (done || !ct_matches<PG>(a.ops, a.nops)) return;
+    err = launch_one<R, N, ORIENT, XS, PG>(a, nblk, nfields, st);
+    done = true;
+  };
+  (one((PGs*)nullptr), ...);
+  return done;
+}
+
+// Compile-time FP32 programs read the FP32 table copies: every table an op
+// uses must have one.
+bool have_f32_tables(const LineOp* ops, int nops) {
+  for (int i = 0; i < nops; i++) {
+    const LineOp& o = ops[i];
+    if ((o.code == LOP_MUL_TAB || o.code == LOP_MUL_POS) && !o.f0) return false;
+    if (o.code == LOP_XSPLIT && (!o.f0 || !o.f1)) return false;
+  }
+  return true;
+}
+
+}  // namespace
 
 template <class R, int N>
 hipError_t launch_reg_sweep_n(const SweepArgs<R>& a, int nblk, int nfields, hipStream_t st) {
-  const size_t lds = reg_lds_bytes<R>(N);
-  auto go = [&](auto kern) {
-    // once per instantiation (thread-safe static initialisation)
-    static const hipError_t attr_set =
-        hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    (void)attr_set;
-    hipLaunchKernelGGL(kern, dim3(nblk, nfields), dim3(kRegThreads), lds, st, a);
-    return hipGetLastError();
-  };
-  if (a.orient != 0) return go(reg_sweep_kernel<R, N, 1, false>);
-  if (ops_have_xsplit(a.ops, a.nops)) return go(reg_sweep_kernel<R, N, 0, true>);
-  return go(reg_sweep_kernel<R, N, 0, false>);
+  const bool xs = ops_have_xsplit(a.ops, a.nops);
+  constexpr bool kCt = N >= 64;  // shorter lines appear only in tests
+  if (kCt && (sizeof(R) == 8 || have_f32_tables(a.ops, a.nops))) {
+    hipError_t err = hipSuccess;
+    if (a.orient != 0) {
+      if (launch_ct<R, N, 1, false>((ColPrograms*)nullptr, a, nblk, nfields, st, err)) return err;
+    } else if (xs) {
+      if (launch_ct<R, N, 0, true>((XsRowPrograms*)nullptr, a, nblk, nfields, st, err)) return err;
+    } else {
+      if (launch_ct<R, N, 0, false>((RowPrograms*)nullptr, a, nblk, nfields, st, err)) return err;
+    }
+  }
+  if (a.orient != 0) return launch_one<R, N, 1, false, RtProg>(a, nblk, nfields, st);
+  if (xs) return launch_one<R, N, 0, true, RtProg>(a, nblk, nfields, st);
+  return launch_one<R, N, 0, false, RtProg>(a, nblk, nfields, st);
 }
 
 template hipError_t launch_reg_sweep_n<ITP_SWEEP_REAL, ITP_SWEEP_N>(const SweepArgs<ITP_SWEEP_REAL>&, int, int,
