@@ -119,6 +119,20 @@ int itps_mixed_stats(const itps_system* s, int* out);
  * itp2d.cpp:35-60): when *flag is non-zero the run finishes with the error
  * flag set. NULL removes it. The pointer must stay valid while steps run. */
 int itps_set_abort_flag(itps_system* s, const volatile int* flag);
+/* Save flag checked after propagation, orthonormalization and the energies of
+ * every step and at its end (the reference's save_flagptr and
+ * check_save_flag, itpsystem.cpp:235-241, 437-469, raised by its SIGUSR1
+ * handler, itp2d.cpp:57-60): when *flag is non-zero, fn(user) runs (the
+ * caller saves the states, e.g. with itps_states and itps_sorted, as the
+ * reference's save_states writes them sorted by the last energies,
+ * itpsystem.cpp:374-388) and *flag is cleared. NULL flag removes it. */
+typedef void (*itps_save_fn)(void* user);
+int itps_set_save_flag(itps_system* s, volatile int* flag, itps_save_fn fn, void* user);
+/* The reference program's signal handlers (itp2d.cpp:40-66): SIGINT raises
+ * *abort_flag (a second SIGINT ends the process at once) and SIGUSR1 raises
+ * *save_flag, each with the reference's note on stderr. For command-line
+ * drivers; the flags must stay valid for the life of the process. */
+int itps_install_signal_flags(volatile int* abort_flag, volatile int* save_flag);
 /* Progress printout on stdout/stderr as the reference's verbosity levels
  * (0 quiet, the default; 1: start, epsilon changes and final message;
  * 2: per-step lines, phase ratios and final energies; 3: energies every

@@ -50,6 +50,8 @@ _lib = None
 _dp = C.POINTER(C.c_double)
 _sz = C.c_size_t
 _vp = C.c_void_p
+# void (*itps_save_fn)(void* user)
+_SaveFn = C.CFUNCTYPE(None, C.c_void_p)
 
 
 class _Test(C.Structure):
@@ -158,6 +160,8 @@ def lib():
             "itp_stateset_cholesky_condition": [_vp, _dp],
             "itp_stateset_cholesky_diagonal": [_vp, _dp],
             "itps_set_abort_flag": [_vp, C.POINTER(C.c_int)],
+            "itps_set_save_flag": [_vp, C.POINTER(C.c_int), _SaveFn, _vp],
+            "itps_install_signal_flags": [C.POINTER(C.c_int), C.POINTER(C.c_int)],
             "itps_set_verbosity": [_vp, C.c_int],
             "itp_potential_create_type": [_vp, C.c_char_p, _dp, _sz, C.POINTER(_vp)],
             "itp_potential_type_params": [C.c_char_p, _dp, _sz, C.POINTER(C.c_int), _dp],
@@ -598,6 +602,23 @@ class Hamiltonian(_Owned):
 
 
 
+_signal_flags = None
+
+
+def install_signal_flags():
+    """The reference program's SIGINT and SIGUSR1 handlers (itp2d.cpp:40-66),
+    installed in the library: returns (abort_flag, save_flag) as ctypes.c_int
+    for ITPSystem.set_abort_flag and set_save_flag. Installed once per process."""
+    global _signal_flags
+    if _signal_flags is None:
+        flags = (C.c_int(0), C.c_int(0))
+        rc = lib().itps_install_signal_flags(C.byref(flags[0]), C.byref(flags[1]))
+        if rc:
+            raise ITPError(rc, "itps_install_signal_flags")
+        _signal_flags = flags
+    return _signal_flags
+
+
 def noalloc_max_bytes():
     """Size of the fixed device scratch array of propagate_noalloc."""
     return lib().itp_propagate_noalloc_max_bytes()
@@ -710,6 +731,18 @@ class ITPSystem(_Owned):
         (the reference's SIGINT abort flag). None removes it."""
         self._abort = flag
         self.ctx.check(lib().itps_set_abort_flag(self.h, None if flag is None else C.byref(flag)), "set_abort_flag")
+
+    def set_save_flag(self, flag, callback):
+        """Check a ctypes.c_int after propagation, orthonormalization and the energies of every step and at
+        its end; when non-zero, callback(self) runs and the flag is cleared (the reference's SIGUSR1 save
+        flag, itpsystem.cpp:235-241). None removes it."""
+        if flag is None:
+            self._save = None
+            self.ctx.check(lib().itps_set_save_flag(self.h, None, _SaveFn(), None), "set_save_flag")
+            return
+        fn = _SaveFn(lambda user: callback(self))
+        self._save = (flag, fn)  # kept alive while the driver may call it
+        self.ctx.check(lib().itps_set_save_flag(self.h, C.byref(flag), fn, None), "set_save_flag")
 
     def set_verbosity(self, level):
         """Progress printout levels 0-3 as the reference's -v."""

@@ -223,15 +223,22 @@ def run(cfg, opts, device=0):
             d = os.path.dirname(opts["datafile"])
             if d and not os.path.isdir(d):
                 raise OSError(f"directory '{d}' does not exist")
+        # SIGINT finishes the run at the next step with the error flag, SIGUSR1 saves the states at the
+        # next check point (itp2d.cpp:40-66, itpsystem.cpp:235-241, 407-416)
+        abort_flag, save_flag = I.install_signal_flags()
+        s.set_abort_flag(abort_flag)
         if opts["save"] == "everything":
-            datafile.run_saving_every_step(s, opts["datafile"], V=V, clobber=opts["clobber"], attributes=attrs)
-        else:
-            if opts["save"] != "nothing" and os.path.exists(opts["datafile"]) and not opts["clobber"]:
+            datafile.run_saving_every_step(s, opts["datafile"], V=V, clobber=opts["clobber"], attributes=attrs,
+                                           save_flag=save_flag)
+        elif opts["save"] != "nothing":
+            if os.path.exists(opts["datafile"]) and not opts["clobber"]:
                 raise OSError(f"datafile '{opts['datafile']}' exists (use --force to overwrite)")
+            datafile.run_and_save(s, opts["datafile"], V=V, clobber=opts["clobber"], attributes=attrs,
+                                  save="energies" if opts["save"] == "energies" else "final_states",
+                                  save_flag=save_flag)
+        else:
+            s.set_save_flag(save_flag, lambda sys_: None)  # no result file: the flag is only cleared
             s.run()
-            if opts["save"] != "nothing":
-                datafile.save_run(s, opts["datafile"], V=V, clobber=opts["clobber"], attributes=attrs,
-                                  save="energies" if opts["save"] == "energies" else "final_states")
     except Exception as e:  # the run failed (itp2d.cpp:113-121)
         print(f"Error during the run: {e}", file=sys.stderr)
         ctx.close()

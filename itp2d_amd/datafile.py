@@ -197,21 +197,61 @@ def save_run(system, path, V=None, save="final_states", clobber=False, attribute
     return path
 
 
-def run_saving_every_step(system, path, V=None, clobber=False, attributes=None, noise_data=None):
+def _save_on_flag(system, f, save_flag):
+    """SIGUSR1-style saves (itpsystem.cpp:235-241, 374-388): when save_flag (a
+    ctypes.c_int) is raised during the run, the current states are written at
+    the current step, sorted by the last computed energies (unsorted before
+    the first energies)."""
+    if save_flag is None:
+        return
+
+    def save(sys_):
+        order = sys_.sorted()[2] if sys_.history()[0].shape[0] > 0 else None
+        f.write_states(sys_.total_steps, sys_.states(), order=order)
+
+    system.set_save_flag(save_flag, save)
+
+
+def run_and_save(system, path, V=None, save="final_states", clobber=False, attributes=None, noise_data=None,
+                 save_flag=None):
+    """Runs `system` to the end with the result file open, as the reference's
+    ITPSystem does: the run description first, states written whenever
+    save_flag (a ctypes.c_int, e.g. raised by SIGUSR1) is raised, then the
+    results as save_run writes them. Returns the driver's status."""
+    if save not in ("final_states", "energies"):
+        raise ValueError("save must be 'final_states' or 'energies'")
+    cfg = system.cfg
+    with Datafile(path, int(cfg["sx"]), int(cfg["sy"]), float(cfg["lenx"]) / int(cfg["sx"]), clobber) as f:
+        write_run_attributes(f, system, V, attributes, noise_data)
+        _save_on_flag(system, f, save_flag)
+        try:
+            rc = system.run()
+        finally:
+            if save_flag is not None:
+                system.set_save_flag(None, None)
+        write_run_results(f, system, final_states=save == "final_states")
+    return rc
+
+
+def run_saving_every_step(system, path, V=None, clobber=False, attributes=None, noise_data=None, save_flag=None):
     """Runs `system` to the end, saving as the reference's Everything mode does:
     the initial states (unsorted, step 0, itpsystem.cpp:132-133), the sorted
     states after every step (itpsystem.cpp:451-453), the histories and the
-    final energies. Returns the driver's status."""
+    final energies; and, when save_flag is raised, the states at that point
+    as well. Returns the driver's status."""
     cfg = system.cfg
     rc = 0
     with Datafile(path, int(cfg["sx"]), int(cfg["sy"]), float(cfg["lenx"]) / int(cfg["sx"]), clobber) as f:
         write_run_attributes(f, system, V, attributes, noise_data)
+        _save_on_flag(system, f, save_flag)
         f.write_states(0, system.states())
         while not system.finished:
             r = system.step()
             rc = rc or r
             if system.total_steps > 0 and not (system.finished and system.error_flag):
                 f.write_states(system.total_steps, system.states(), order=system.sorted()[2])
+        if save_flag is not None:
+            system.set_save_flag(None, None)
         write_run_results(f, system, final_states=False)
     return rc
 

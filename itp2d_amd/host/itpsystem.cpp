@@ -2,6 +2,8 @@
 // the reference) driving the gfx950 engine through the C++ facade, which in
 // turn only uses the C ABI. Convergence tests follow convergence.cpp:75-200.
 #include <algorithm>
+#include <csignal>
+#include <unistd.h>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -80,10 +82,17 @@ class ITPSystem {
       std::fprintf(out, "Step %d (step %d with eps = %e) starting at %s\n", total_step_counter, step_counter, eps,
                    timestring().c_str());
     T_->operate(*states_);          // propagate (itpsystem.cpp:243-256)
-    if (mixed) { mixed_step(); return; }
+    check_save_flag();
+    if (mixed) {
+      mixed_step();
+      if (!finished) check_save_flag();
+      return;
+    }
     orthonormalize();               // itpsystem.cpp:258-289
     if (finished) return;
+    check_save_flag();
     calculate_energies();           // itpsystem.cpp:473-511
+    check_save_flag();
     if (verbosity >= 3) print_energies();
     check_timestep_convergence();
     if (all_ts) {
@@ -98,6 +107,17 @@ class ITPSystem {
       change_time_step();
     }
     if (verbosity > 0) std::fflush(out);
+    check_save_flag();
+  }
+
+  // ITPSystem::check_save_flag (itpsystem.cpp:235-241): the caller's save
+  // callback (the reference writes the states sorted by the last energies)
+  // runs when the flag is raised, then the flag is cleared
+  void check_save_flag() {
+    if (save_flag && *save_flag) {
+      if (save_fn) save_fn(save_user);
+      *save_flag = 0;
+    }
   }
 
   void finish() {
@@ -202,6 +222,11 @@ class ITPSystem {
   std::vector<std::pair<int, double>> ts_hist;
   bool record_steps = true;
   const volatile int* abort_flag = nullptr;
+  // save flag checked after each phase of a step (the reference's
+  // save_flagptr, raised by its SIGUSR1 handler, itp2d.cpp:57-60)
+  volatile int* save_flag = nullptr;
+  itps_save_fn save_fn = nullptr;
+  void* save_user = nullptr;
   int verbosity = 0;  // 0 quiet; 1-3 as the reference's -v levels
   bool pot_null = false;
   FILE* out = stdout;
@@ -707,6 +732,43 @@ int itps_rayleigh_ritz(itps_system* s) {
 int itps_set_abort_flag(itps_system* s, const volatile int* flag) {
   if (!s) return ITP_E_INVALID;
   s->sys->abort_flag = flag;
+  return ITP_OK;
+}
+
+int itps_set_save_flag(itps_system* s, volatile int* flag, itps_save_fn fn, void* user) {
+  if (!s) return ITP_E_INVALID;
+  s->sys->save_flag = flag;
+  s->sys->save_fn = fn;
+  s->sys->save_user = user;
+  return ITP_OK;
+}
+
+namespace {
+volatile int* g_sig_abort = nullptr;
+volatile int* g_sig_save = nullptr;
+const char kAbortNote[] =
+    "\nCaught SIGINT. Saving data and quitting at next convenient spot.\nPress Ctrl-C again to signal immediate "
+    "stop.\n";
+const char kSaveNote[] = "\nCaught SIGUSR1. Saving states at next convenient stop and continuing.\n";
+// itp2d.cpp:49-60: a second SIGINT ends the program at once
+void on_sigint(int) {
+  if (*g_sig_abort) std::_Exit(1);
+  ssize_t r = write(STDERR_FILENO, kAbortNote, sizeof(kAbortNote) - 1);
+  (void)r;
+  *g_sig_abort = 1;
+}
+void on_sigusr1(int) {
+  ssize_t r = write(STDERR_FILENO, kSaveNote, sizeof(kSaveNote) - 1);
+  (void)r;
+  *g_sig_save = 1;
+}
+}  // namespace
+
+int itps_install_signal_flags(volatile int* abort_flag, volatile int* save_flag) {
+  if (!abort_flag || !save_flag) return ITP_E_INVALID;
+  g_sig_abort = abort_flag;
+  g_sig_save = save_flag;
+  if (std::signal(SIGINT, on_sigint) == SIG_ERR || std::signal(SIGUSR1, on_sigusr1) == SIG_ERR) return ITP_E_INVALID;
   return ITP_OK;
 }
 

@@ -104,3 +104,20 @@ def test_cli_run_matches_api(ctx, tmp_path):
     assert cli.main(argv + ["-o", p, "-f"]) == 0
     assert cli.main(argv + ["--save-nothing", "-o", str(tmp_path / "none.h5")]) == 0
     assert not (tmp_path / "none.h5").exists()
+
+
+def test_signal_flags_installed_like_the_reference():
+    """SIGUSR1 raises the save flag and SIGINT the abort flag, each with the
+    reference's note on stderr (itp2d.cpp:40-66); run in a child process so
+    the handlers stay out of the test runner."""
+    import subprocess
+    import sys
+    code = ("import os, signal, time; from itp2d_amd import api as I; a, s = I.install_signal_flags(); "
+            "os.kill(os.getpid(), signal.SIGUSR1); time.sleep(0.2); print('save', s.value, 'abort', a.value); "
+            "os.kill(os.getpid(), signal.SIGINT); time.sleep(0.2); print('save', s.value, 'abort', a.value)")
+    root = __import__("os").path.dirname(__import__("os").path.dirname(__import__("os").path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.split("\n")[:2] == ["save 1 abort 0", "save 1 abort 1"]
+    assert "Caught SIGUSR1. Saving states at next convenient stop and continuing." in r.stderr
+    assert "Caught SIGINT. Saving data and quitting at next convenient spot." in r.stderr

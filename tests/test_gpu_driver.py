@@ -70,3 +70,35 @@ def test_driver_semantics_match_oracle(ctx, case):
         assert g.error_flag
     if case == "eps_list_exhausted":
         assert [e for _, e in g.time_step_history()][:4] == [0.3, 0.2, 0.1, 0.05]
+
+
+def test_save_flag_checked_and_cleared(ctx):
+    """The save flag (the reference's SIGUSR1 flag, itpsystem.cpp:235-241) is
+    checked after propagation, orthonormalization, the energies and at the end
+    of every step: a flag raised before the run calls the save callback once,
+    right after the first propagation, and is cleared; raising it again from
+    the callback's own step calls it at the next check point. The run itself
+    is unchanged (same steps and energies as without the flag)."""
+    say("driver save flag")
+    cfg, V = _base()
+    ref = I.ITPSystem(ctx, cfg, V)
+    ref.run()
+    flag = __import__("ctypes").c_int(1)
+    calls = []
+
+    def save(sys_):
+        st = sys_.states()
+        assert st.shape == (cfg["N"], cfg["sy"], cfg["sx"])
+        calls.append(sys_.total_steps)
+
+    g = I.ITPSystem(ctx, cfg, V)
+    g.set_save_flag(flag, save)
+    g.step()
+    assert calls == [1] and flag.value == 0
+    flag.value = 1
+    g.step()
+    assert calls == [1, 2] and flag.value == 0
+    g.run()
+    assert calls == [1, 2]
+    assert g.total_steps == ref.total_steps and g.num_converged == ref.num_converged
+    assert np.array_equal(g.sorted()[0], ref.sorted()[0])
