@@ -244,6 +244,10 @@ int itp_apply_hamiltonian_host(itp_hamiltonian* h, double* fields, size_t n);
  * column-major (upper triangle read); on return A holds eigenvectors in
  * columns and w the eigenvalues in ascending order, like zheev('V','U'). */
 int itp_eigh_host(itp_ctx* ctx, size_t n, double* A, double* w);
+/* Eigenvalues only (ascending, w[n]) of the same input, by Householder
+ * tridiagonalization and bisection on the GPU: the route of the eigenvalue
+ * check (itp_ctx_set_eig_check_min). Test hook; A is not changed. */
+int itp_eigvalsh_host(itp_ctx* ctx, size_t n, const double* A, double* w);
 
 /* Seeded initial states: re, im ~ U[-1,1] from a splitmix64 stream, each state
  * normalised (StateSet::init_to_gaussian_noise, stateset.cpp:124-131, with
@@ -296,6 +300,16 @@ int itp_ctx_oa_stats(itp_ctx* ctx, long long* refined, long long* fallbacks);
  * the cluster start off (default 0.1). No counterpart in the reference, which
  * calls zheev (eigensolver.hpp:80-84): same eigenpairs, another route. */
 int itp_ctx_set_eig_cluster(itp_ctx* ctx, double tau);
+/* Overlap matrices of at least n states (default 1024) that a Cholesky
+ * factorization does not show positive definite (a pivot below 1e-12 of its
+ * diagonal) get their eigenvalues first, by Householder tridiagonalization and
+ * bisection, and the reference's checks (stateset.cpp:164-173) run on those: a
+ * rejected matrix returns ITP_E_NONPOSITIVE_EIG / ITP_E_NONNORMAL_EIG without
+ * the eigenvectors being computed (the eigenvalues reported are the check's).
+ * No counterpart in the reference, whose zheev computes both. */
+int itp_ctx_set_eig_check_min(itp_ctx* ctx, int n);
+/* Eigenvalue checks run and matrices they rejected. */
+int itp_ctx_eig_check_stats(itp_ctx* ctx, long long* checks, long long* rejected);
 /* out[4]: calls finished by the cluster start, clusters solved, largest
  * cluster, calls that fell back to the block-Jacobi route. */
 int itp_ctx_cluster_stats(itp_ctx* ctx, long long* out);

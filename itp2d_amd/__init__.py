@@ -13,5 +13,5 @@ from .api import (  # noqa: F401
     TRANSFORMS, PERIODIC, DIRICHLET, FP64, FP32, TESTS,
     ITPError, NonPositiveEigenvalue, NonNormalEigenvalue, EigensolverError,
     Context, Grid, Potential, StateSet, MultiProductSplit, Hamiltonian,
-    ITPSystem, Batch, init_states, eigh, noalloc_max_bytes, lib, library_path,
+    ITPSystem, Batch, init_states, eigh, eigvalsh, install_signal_flags, noalloc_max_bytes, lib, library_path,
 )

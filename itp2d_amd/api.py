@@ -133,6 +133,9 @@ def lib():
             "itp_apply_secondorder_host": [_vp, C.c_double, C.c_double, C.c_double, C.c_int, _dp, _sz],
             "itp_apply_hamiltonian_host": [_vp, _dp, _sz],
             "itp_eigh_host": [_vp, _sz, _dp, _dp],
+            "itp_eigvalsh_host": [_vp, _sz, _dp, _dp],
+            "itp_ctx_set_eig_check_min": [_vp, C.c_int],
+            "itp_ctx_eig_check_stats": [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
             "itp_init_states": [C.POINTER(C.c_uint64), _sz, _sz, C.c_double, _sz, _dp],
             "itps_create": [_vp, C.POINTER(_Params), _dp, _dp, C.POINTER(_vp)],
             "itps_destroy": [_vp],
@@ -317,6 +320,17 @@ class Context:
     def set_fp32_rotation(self, kind):
         """Rotation of FP32 sets: 1 split-BF16x3 (default), 0 FP32 MFMA."""
         self.check(lib().itp_ctx_set_fp32_rotation(self.h, int(kind)), "set_fp32_rotation")
+
+    def set_eig_check_min(self, n):
+        """Overlap matrices of at least n states get the eigenvalue check (tridiagonalization and
+        bisection) before the eigendecomposition (default 1024)."""
+        self.check(lib().itp_ctx_set_eig_check_min(self.h, int(n)), "set_eig_check_min")
+
+    def eig_check_stats(self):
+        """(eigenvalue checks run, matrices they rejected)"""
+        a, b = C.c_longlong(0), C.c_longlong(0)
+        lib().itp_ctx_eig_check_stats(self.h, C.byref(a), C.byref(b))
+        return a.value, b.value
 
     def set_eig_cluster(self, tau):
         """Cluster start threshold on |a_ij| / |a_ii - a_jj| (0: off)."""
@@ -632,6 +646,16 @@ def eigh(ctx, A):
     w = np.zeros(n)
     ctx.check(lib().itp_eigh_host(ctx.h, n, _p(a.view(np.float64)), _p(w)), "eigh")
     return w, np.ascontiguousarray(a.reshape(n, n).T)
+
+
+def eigvalsh(ctx, A):
+    """Ascending eigenvalues of the Hermitian A by tridiagonalization and bisection on the GPU."""
+    A = np.asarray(A, dtype=np.complex128)
+    n = A.shape[0]
+    a = np.ascontiguousarray(A.T).ravel().copy()  # column-major
+    w = np.zeros(n)
+    ctx.check(lib().itp_eigvalsh_host(ctx.h, n, _p(a.view(np.float64)), _p(w)), "eigvalsh")
+    return w
 
 
 def init_states(seed, N, sx, sy, dx):

@@ -1,6 +1,6 @@
 """Builds the gfx950 library itp2d_amd/lib/libitp2d_amd.so with hipcc.
 
-Sources: csrc/kernels.hip, csrc/mixed.hip, csrc/chol.hip (kernels), csrc/reg_sweep_kernels.hip (the
+Sources: csrc/kernels.hip, csrc/mixed.hip, csrc/chol.hip, csrc/tridiag.hip (kernels), csrc/reg_sweep_kernels.hip (the
 register engine's sweep kernels, compiled once per precision and line length), csrc/capi.hip (C ABI of
 include/itp2d.h),
 host/itpsystem.cpp (ITP driver, C ABI of include/itp2d_system.h), host/batch.cpp (batches of
@@ -30,6 +30,7 @@ SOURCES = [(os.path.join(HERE, "csrc", "reg_sweep_kernels.hip"), [f"-DITP_SWEEP_
     os.path.join(HERE, "csrc", "kernels.hip"),
     os.path.join(HERE, "csrc", "mixed.hip"),
     os.path.join(HERE, "csrc", "chol.hip"),
+    os.path.join(HERE, "csrc", "tridiag.hip"),
     os.path.join(HERE, "csrc", "capi.hip"),
     os.path.join(HERE, "host", "itpsystem.cpp"),
     os.path.join(HERE, "host", "batch.cpp"),

@@ -46,6 +46,10 @@ struct itp_ctx {
   int fp32_rotation = 1;      // FP32 sets' rotation: 0 FP32 MFMA, 1 split-BF16x3 MFMA
   unsigned eig_spin_limit = 1u << 24;  // polls of a block-Jacobi grid barrier before it reports a timeout
   double chol_tol = 0;        // Cholesky pivot threshold relative to the diagonal (0: 1e-5 FP32 sets, 1e-12 FP64)
+  // eigenvalue check before the eigendecomposition (tridiagonalization and
+  // bisection) from this matrix size on; calls and rejections
+  int td_min_n = 1024;
+  long long td_checks = 0, td_rejects = 0;
   long long oa_done = 0, oa_fallback = 0;
   long long cl_calls = 0, cl_clusters = 0, cl_largest = 0, cl_fallback = 0;
 };
@@ -113,6 +117,7 @@ struct itp_stateset {
   cplx<double>* epart = nullptr;  // [2][N][max(sx,sy)]
   double* dE = nullptr;           // E[N], sigma[N]
   cplx<double>* dmean = nullptr;  // mean[N], msq[N]
+  void* td_scratch = nullptr;     // eigenvalue check (tridiagonalization), allocated on first use
 };
 
 struct MemberTabs {
@@ -602,6 +607,46 @@ int cluster_route(itp_stateset* s, cplx<double>* Acur, const cplx<double>* A0, b
   return 1;
 }
 
+// Ascending eigenvalues of the N x N Hermitian matrix A (np x np, device) by
+// tridiagonalization and bisection (tridiag.hip), A left unchanged.
+int hermitian_eigenvalues(itp_stateset* s, const cplx<double>* A, std::vector<double>& lam) {
+  itp_ctx* c = s->grid->ctx;
+  const int N = (int)s->N;
+  if (!s->td_scratch) {
+    int rc = dalloc(c, &s->td_scratch, hermitian_eigenvalues_scratch(s->np, N));
+    if (rc) return rc;
+  }
+  HIPCK(c, launch_hermitian_eigenvalues(A, s->np, N, s->td_scratch, s->lam, c->eig_spin_limit, c->stream));
+  lam.resize(N);
+  unsigned bar[2] = {0, 0};
+  HIPCK(c, hipMemcpyAsync(lam.data(), s->lam, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipMemcpyAsync(bar, (const char*)s->td_scratch + hermitian_eigenvalues_barrier_offset(s->np, N), sizeof(bar),
+                          hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  if (bar[1]) return set_err(c, ITP_E_EIGENSOLVER, "tridiagonalization: a grid barrier timed out");
+  return ITP_OK;
+}
+
+// The reference's checks of the ascending overlap eigenvalues
+// (stateset.cpp:164-173): the first that is not positive, or not normal.
+int check_eigenvalues(itp_ctx* c, const std::vector<double>& lam, size_t* bad_n, double* bad_lambda) {
+  for (size_t n = 0; n < lam.size(); n++) {
+    const double ev = lam[n];
+    if (ev <= 0) {
+      if (bad_n) *bad_n = n;
+      if (bad_lambda) *bad_lambda = ev;
+      return set_err(c, ITP_E_NONPOSITIVE_EIG, "non-positive overlap eigenvalue " + std::to_string(ev) +
+                                                   " at n = " + std::to_string(n));
+    }
+    if (std::fpclassify(ev) != FP_NORMAL) {
+      if (bad_n) *bad_n = n;
+      if (bad_lambda) *bad_lambda = ev;
+      return set_err(c, ITP_E_NONNORMAL_EIG, "non-normal overlap eigenvalue at n = " + std::to_string(n));
+    }
+  }
+  return ITP_OK;
+}
+
 int eigen_stage(itp_stateset* s, size_t* bad_n, double* bad_lambda, bool scale, bool check_positive = true) {
   itp_ctx* c = s->grid->ctx;
   const int N = (int)s->N;
@@ -626,6 +671,32 @@ int eigen_stage(itp_stateset* s, size_t* bad_n, double* bad_lambda, bool scale, 
   if (rd.bad) {
     c->eig_calls++;
     return set_err(c, ITP_E_EIGENSOLVER, "overlap matrix has non-finite entries");
+  }
+  // Large matrices: when the reference's checks would reject the eigenvalues,
+  // it raises before any use of the eigenvectors, so those are not computed
+  // (the costly case: overlaps of states from noise with a time step too
+  // large, which the caller's recovery handles). A Cholesky factorization with
+  // every pivot above 1e-12 of its diagonal shows the matrix positive definite
+  // far above rounding (every check passes); otherwise the eigenvalues come
+  // first, by tridiagonalization and bisection.
+  if (check_positive && N >= c->td_min_n) {
+    int fail = 0;
+    HIPCK(c, launch_cholinv(s->A[0], s->np, N, s->A[1], s->diag0, s->C, s->nc, s->chol_fail, 1e-12, c->stream));
+    HIPCK(c, hipMemcpyAsync(&fail, s->chol_fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    if (hybrid) HIPCK(c, hipMemcpyAsync(s->A[1], s->A[0], np2, hipMemcpyDeviceToDevice, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    if (fail) {
+      std::vector<double> ev;
+      int rc = hermitian_eigenvalues(s, s->A[0], ev);
+      if (rc) return rc;
+      c->td_checks++;
+      if ((rc = check_eigenvalues(c, ev, bad_n, bad_lambda))) {
+        c->td_rejects++;
+        c->eig_calls++;
+        s->lam_host = ev;
+        return rc;
+      }
+    }
   }
   bool done = false, try_clusters = clusters, handoff = hybrid;
   int sweeps = 0, rc = ITP_OK;
@@ -691,25 +762,12 @@ int eigen_stage(itp_stateset* s, size_t* bad_n, double* bad_lambda, bool scale, 
   if (!done)
     return set_err(c, ITP_E_EIGENSOLVER, "Jacobi eigensolver did not converge in " + std::to_string(sweeps) + " sweeps");
   s->lam_host = lam;
-  for (int n = 0; n < N; n++) {
-    const double ev = lam[n];
-    if (!check_positive) {
-      if (!std::isfinite(ev)) return set_err(c, ITP_E_EIGENSOLVER, "non-finite eigenvalue");
-      continue;
-    }
-    if (ev <= 0) {
-      if (bad_n) *bad_n = n;
-      if (bad_lambda) *bad_lambda = ev;
-      return set_err(c, ITP_E_NONPOSITIVE_EIG, "non-positive overlap eigenvalue " + std::to_string(ev) +
-                                                   " at n = " + std::to_string(n));
-    }
-    if (std::fpclassify(ev) != FP_NORMAL) {
-      if (bad_n) *bad_n = n;
-      if (bad_lambda) *bad_lambda = ev;
-      return set_err(c, ITP_E_NONNORMAL_EIG, "non-normal overlap eigenvalue at n = " + std::to_string(n));
-    }
+  if (!check_positive) {
+    for (int n = 0; n < N; n++)
+      if (!std::isfinite(lam[n])) return set_err(c, ITP_E_EIGENSOLVER, "non-finite eigenvalue");
+    return ITP_OK;
   }
-  return ITP_OK;
+  return check_eigenvalues(c, lam, bad_n, bad_lambda);
 }
 
 template <class R>
@@ -968,6 +1026,19 @@ int itp_ctx_phase_times(itp_ctx* c, double* ms, long long* counts) {
   }
   return ITP_OK;
 }
+int itp_ctx_set_eig_check_min(itp_ctx* c, int n) {
+  if (!c || n < 1) return ITP_E_INVALID;
+  c->td_min_n = n;
+  return ITP_OK;
+}
+
+int itp_ctx_eig_check_stats(itp_ctx* c, long long* checks, long long* rejected) {
+  if (!c) return ITP_E_INVALID;
+  if (checks) *checks = c->td_checks;
+  if (rejected) *rejected = c->td_rejects;
+  return ITP_OK;
+}
+
 int itp_ctx_eig_stats(itp_ctx* c, long long* calls, long long* sweeps) {
   if (!c) return ITP_E_INVALID;
   if (calls) *calls = c->eig_calls;
@@ -1356,7 +1427,8 @@ void itp_stateset_destroy(itp_stateset* s) {
   void* ptrs[] = {s->buf[0], s->buf[1], s->part, s->A[0], s->A[1], s->V[0], s->V[1], s->Vinit, s->bjQ, s->bjD,
                   s->bjslots, s->bjident, s->oa_work, s->oa_lam, s->oa_slots, s->oa_state, s->cl_hdr, s->cl_pairs,
                   s->cl_members, s->cl_offsets, s->Vsave, s->oa_part, s->C, s->C32,
-                  s->lam, s->perm, s->info, s->bad, s->epart, s->dE, s->dmean, s->diag0, s->chol_fail, s->chol_mm, s->norms};
+                  s->lam, s->perm, s->info, s->bad, s->epart, s->dE, s->dmean, s->diag0, s->chol_fail, s->chol_mm, s->norms,
+                  s->td_scratch};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete s;
@@ -1756,6 +1828,31 @@ int itp_eigh_host(itp_ctx* c, size_t n, double* A, double* w) {
     for (size_t r = 0; r < n; r++)
       for (size_t cc = 0; cc < n; cc++) o[r * n + cc] = C[cc * s->nc + r];  // column r = eigenvector r
     c->err.clear();
+    return ITP_OK;
+  });
+}
+
+int itp_eigvalsh_host(itp_ctx* c, size_t n, const double* A, double* w) {
+  if (!c || !A || !w || n < 1 || n > 4096) return ITP_E_INVALID;
+  return guarded(c, [&]() -> int {
+    // a throwaway state set of the right size carries the workspace
+    itp_grid* g = nullptr;
+    int rc = itp_grid_create(c, 8, 8, 1.0, ITP_PERIODIC, &g);
+    if (rc) return rc;
+    std::unique_ptr<itp_grid, void (*)(itp_grid*)> hg(g, itp_grid_destroy);
+    itp_stateset* s = nullptr;
+    if ((rc = itp_stateset_create(g, n, ITP_FP64, &s))) return rc;
+    std::unique_ptr<itp_stateset, void (*)(itp_stateset*)> hs(s, itp_stateset_destroy);
+    const cplx<double>* a = (const cplx<double>*)A;
+    std::vector<cplx<double>> full((size_t)s->np * s->np, mk<double>(0.0, 0.0));
+    for (size_t i = 0; i < n; i++)
+      for (size_t j = 0; j < n; j++)
+        full[i * s->np + j] = (i <= j) ? a[j * n + i] : conj(a[i * n + j]);  // upper triangle, column-major
+    for (size_t i = 0; i < n; i++) full[i * s->np + i].y = 0;
+    HIPCK(c, hipMemcpy(s->A[0], full.data(), full.size() * 16, hipMemcpyHostToDevice));
+    std::vector<double> ev;
+    if ((rc = hermitian_eigenvalues(s, s->A[0], ev))) return rc;
+    std::copy(ev.begin(), ev.end(), w);
     return ITP_OK;
   });
 }

@@ -10,6 +10,15 @@ template <class R>
 hipError_t launch_sweep(const SweepArgs<R>& a, int nblk, int nfields, hipStream_t st);
 // one register-engine sweep of line length N (reg_sweep_kernels.hip, one translation unit
 // per precision and length); the caller has checked the geometry
+// Ascending eigenvalues lam[N] of the Hermitian np x np row-major A (device,
+// both triangles, left unchanged) by Householder tridiagonalization and
+// bisection (tridiag.hip), in scratch of hermitian_eigenvalues_scratch bytes;
+// the uint32 at hermitian_eigenvalues_barrier_offset + 4 is non-zero after
+// the launch if a grid barrier gave up (spin_limit polls).
+size_t hermitian_eigenvalues_scratch(int np, int N);
+size_t hermitian_eigenvalues_barrier_offset(int np, int N);
+hipError_t launch_hermitian_eigenvalues(const cplx<double>* A, int np, int N, void* scratch, double* lam,
+                                        unsigned spin_limit, hipStream_t st);
 template <class R, int N>
 hipError_t launch_reg_sweep_n(const SweepArgs<R>& a, int nblk, int nfields, hipStream_t st);
 hipError_t launch_energy_reduce(const cplx<double>* p1, const cplx<double>* p2, int nblk, int N, double dx,

@@ -19,6 +19,7 @@
 #include "cluster_eig.hpp"
 #include "dense_ops.hpp"
 #include "gpu.hpp"
+#include "grid_sync.hpp"
 #include "line_ops.hpp"
 #include "potentials.hpp"
 #include "reg_sweep.hpp"
@@ -377,38 +378,6 @@ template hipError_t launch_gram<float>(const cplx<float>*, const cplx<float>*, l
 // sweep's own slot, which every workgroup reads after the sweep's last
 // barrier, so all workgroups take the same decision.
 // ---------------------------------------------------------------------------
-// Grid-wide barrier of a cooperative launch: an arrival counter (zeroed
-// before the launch) with agent-scope release/acquire fences. Barrier number
-// k of every workgroup waits for k * gridDim.x arrivals. A workgroup that is
-// not the last to arrive polls the counter at most `limit` times; past that
-// it raises the error flag instead of waiting on. Returns true when the flag
-// is raised (by this or any other workgroup): the caller then leaves the
-// kernel, and every workgroup still waiting on a later barrier gives up after
-// its own bounded wait, so the launch always drains.
-__device__ __forceinline__ bool grid_barrier(unsigned* cnt, unsigned& epoch, unsigned* err, unsigned limit) {
-  __shared__ int failed;
-  __syncthreads();
-  const unsigned target = ++epoch * gridDim.x;
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned arrived = atomicAdd(cnt, 1u) + 1u;
-    if (arrived < target) {
-      for (unsigned spins = 0;; spins++) {
-        if (spins >= limit) {
-          atomicOr(err, 1u);
-          break;
-        }
-        if (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
-        __builtin_amdgcn_s_sleep(1);
-      }
-    }
-    __threadfence();
-    failed = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
-  }
-  __syncthreads();
-  return failed != 0;
-}
-
 __global__ __launch_bounds__(256) void block_jacobi_kernel(BlockJacobiArgs a) {
   unsigned* bar = reinterpret_cast<unsigned*>(a.slots + 2 * (a.max_sweeps + 1));
   unsigned* bar_err = bar + 1;

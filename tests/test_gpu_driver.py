@@ -41,12 +41,19 @@ CASES = {
 }
 
 
-@pytest.mark.parametrize("case", list(CASES) + ["recover"])
+@pytest.mark.parametrize("case", list(CASES) + ["recover", "recover_eigenvalue_check"])
 def test_driver_semantics_match_oracle(ctx, case):
+    """recover_eigenvalue_check: the recovery case with the eigenvalue check
+    (Cholesky test, then tridiagonalization and bisection before the
+    eigendecomposition) on for every matrix size: the rejected first overlap
+    skips the eigenvectors and the run is the same."""
     say(f"driver semantics {case}")
     cfg, V = _base()
     init = None
-    if case == "recover":
+    if case == "recover_eigenvalue_check":
+        ctx.set_eig_check_min(1)
+        checks0 = ctx.eig_check_stats()
+    if case.startswith("recover"):
         cfg.update(recover=1, eps_values=[0.1], max_steps=1500)
         init = I.init_states(cfg["seed"] + 7, cfg["N"], cfg["sx"], cfg["sy"], cfg["lenx"] / cfg["sx"])
         init[3] = 0  # an exactly zero overlap eigenvalue: the first orthonormalization fails
@@ -63,9 +70,14 @@ def test_driver_semantics_match_oracle(ctx, case):
     Eg, Eo = g.sorted()[0], o.sorted()[0]
     nw = cfg["n_wanted"]
     assert np.abs(Eg[:nw] - Eo[:nw]).max() / np.abs(Eo[:nw]).max() < 1e-10
-    if case == "recover":
+    if case.startswith("recover"):
         assert g.recoveries == 1
         assert g.eps == pytest.approx(0.1 / 5 ** (len(g.time_step_history()) - 1))
+    if case == "recover_eigenvalue_check":
+        checks, rejected = (a - b for a, b in zip(ctx.eig_check_stats(), checks0))
+        ctx.set_eig_check_min(1024)
+        # only the first overlap (with a zero state) fails the Cholesky test and gets the check
+        assert rejected == 1 and checks == 1
     if case.endswith("bailout"):
         assert g.error_flag
     if case == "eps_list_exhausted":

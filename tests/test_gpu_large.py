@@ -107,3 +107,40 @@ def test_eigh_large(ctx, n):
     assert np.abs(w - wl).max() < 1e-13 * np.abs(wl).max()
     r = A @ V[:, :64] - V[:, :64] * w[:64]
     assert np.abs(r).max() < 1e-12 * np.abs(A).max()
+
+
+def _hermitian(kind, n, seed):
+    rng = np.random.default_rng(seed)
+    if kind == "overlap":
+        return _overlap_like(n, seed)
+    if kind == "diag":  # already diagonal: every reflector is the identity
+        return np.diag(rng.uniform(-1, 1, n)).astype(np.complex128)
+    if kind == "phase":  # tridiagonal with complex couplings: zero x, complex alpha
+        A = np.diag(rng.uniform(-1, 1, n)).astype(np.complex128)
+        for k in range(n - 1):
+            A[k + 1, k] = np.exp(1j * rng.uniform(0, 2 * np.pi)) * rng.uniform(0.1, 1)
+            A[k, k + 1] = np.conj(A[k + 1, k])
+        return A
+    if kind == "block":  # two uncoupled blocks and a repeated eigenvalue
+        X = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        A = X + X.conj().T
+        h = n // 2
+        A[:h, h:] = 0
+        A[h:, :h] = 0
+        A[h:, h:] = A[:n - h, :n - h] if n % 2 == 0 else A[h:, h:]
+        return A
+    X = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    return 0.5 * (X + X.conj().T)
+
+
+@pytest.mark.parametrize("n,kind", [(1, "rand"), (2, "rand"), (3, "phase"), (17, "rand"), (64, "diag"), (130, "block"),
+                                    (300, "phase"), (500, "overlap"), (2000, "overlap"), (4000, "rand")])
+def test_eigvalsh_tridiagonal(ctx, n, kind):
+    """Eigenvalues by Householder tridiagonalization and bisection (the
+    eigenvalue check of large overlaps) against LAPACK's eigvalsh."""
+    say(f"eigvalsh n={n} {kind}")
+    A = _hermitian(kind, n, n + 7)
+    w = I.eigvalsh(ctx, A)
+    wl = np.linalg.eigvalsh(A)
+    assert np.all(np.diff(w) >= 0)
+    assert np.abs(w - wl).max() < 1e-13 * np.abs(wl).max()
