@@ -300,6 +300,14 @@ int itp_ctx_oa_stats(itp_ctx* ctx, long long* refined, long long* fallbacks);
  * the cluster start off (default 0.1). No counterpart in the reference, which
  * calls zheev (eigensolver.hpp:80-84): same eigenpairs, another route. */
 int itp_ctx_set_eig_cluster(itp_ctx* ctx, double tau);
+/* 1: the fixed launch sequences of a propagation (every sweep of the split
+ * members), of an overlap (Gram product and reduction) and of a Cholesky
+ * factorization are captured as HIP graphs on their second call with the same
+ * objects and buffers and replayed afterwards; 0 (default): launched one by one
+ * (the graphs kept so far are released). Results are the same. While any
+ * context has graphs on, allocations, frees and device-wide synchronisations
+ * of the library wait for captures in progress in other threads. */
+int itp_ctx_set_graphs(itp_ctx* ctx, int on);
 /* Overlap matrices of at least n states (default 1024) that a Cholesky
  * factorization does not show positive definite (a pivot below 1e-12 of its
  * diagonal) get their eigenvalues first, by Householder tridiagonalization and

@@ -135,6 +135,7 @@ def lib():
             "itp_eigh_host": [_vp, _sz, _dp, _dp],
             "itp_eigvalsh_host": [_vp, _sz, _dp, _dp],
             "itp_ctx_set_eig_check_min": [_vp, C.c_int],
+            "itp_ctx_set_graphs": [_vp, C.c_int],
             "itp_ctx_eig_check_stats": [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
             "itp_init_states": [C.POINTER(C.c_uint64), _sz, _sz, C.c_double, _sz, _dp],
             "itps_create": [_vp, C.POINTER(_Params), _dp, _dp, C.POINTER(_vp)],
@@ -320,6 +321,11 @@ class Context:
     def set_fp32_rotation(self, kind):
         """Rotation of FP32 sets: 1 split-BF16x3 (default), 0 FP32 MFMA."""
         self.check(lib().itp_ctx_set_fp32_rotation(self.h, int(kind)), "set_fp32_rotation")
+
+    def set_graphs(self, on):
+        """1: propagation, overlap and Cholesky factor launch sequences replayed as HIP graphs; 0 (default):
+        launched one by one."""
+        self.check(lib().itp_ctx_set_graphs(self.h, int(on)), "set_graphs")
 
     def set_eig_check_min(self, n):
         """Overlap matrices of at least n states get the eigenvalue check (tridiagonalization and

@@ -4,9 +4,12 @@
 // Table construction (host, FP64, the reference's formulas) and the sweep
 // sequences come from engine.hpp; kernels from kernels.hip.
 #include <mutex>
+#include <shared_mutex>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -20,6 +23,101 @@
 #include "gpu.hpp"
 
 using namespace itp;
+
+// Device-wide operations (allocation, freeing, device synchronisation,
+// synchronous copies) must not run while another thread captures a graph on
+// its stream: they would join or invalidate the capture. While any context
+// has graphs on, they take this lock exclusively and captures take it
+// shared; otherwise (the default) they take no lock.
+std::shared_mutex& device_op_mutex() {
+  static std::shared_mutex m;
+  return m;
+}
+std::atomic<int>& graph_contexts() {
+  static std::atomic<int> n{0};
+  return n;
+}
+template <class F>
+hipError_t device_op(F&& f) {
+  if (graph_contexts().load() == 0) return f();
+  std::unique_lock<std::shared_mutex> lk(device_op_mutex());
+  return f();
+}
+hipError_t dev_malloc(void** p, size_t bytes) { return device_op([&] { return hipMalloc(p, bytes); }); }
+hipError_t dev_free(void* p) { return device_op([&] { return hipFree(p); }); }
+hipError_t dev_sync() { return device_op([&] { return hipDeviceSynchronize(); }); }
+hipError_t dev_memcpy(void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  return device_op([&] { return hipMemcpy(dst, src, bytes, kind); });
+}
+hipError_t dev_memset(void* dst, int v, size_t bytes) { return device_op([&] { return hipMemset(dst, v, bytes); }); }
+
+// HIP graphs of fixed launch sequences on the context's stream, keyed by what
+// determines their launches and arguments. The first call with a key runs the
+// launches directly (kernel attributes and lazily allocated buffers are set
+// up then); the second captures them and instantiates a graph, which later
+// calls replay. At most kGraphCap graphs are kept (least recently used out).
+struct GraphCache {
+  using Key = std::array<uint64_t, 4>;
+  struct Entry {
+    Key key;
+    hipGraphExec_t exec = nullptr;  // null: seen once, not captured yet
+    uint64_t last = 0;
+  };
+  static constexpr size_t kGraphCap = 64;
+  std::vector<Entry> entries;
+  uint64_t clock = 0;
+  void clear() {
+    for (auto& e : entries)
+      if (e.exec) (void)hipGraphExecDestroy(e.exec);
+    entries.clear();
+  }
+  // Runs fn (launches on st only, nothing that synchronises or copies to the
+  // host), directly or as a graph.
+  template <class F>
+  hipError_t run(hipStream_t st, const Key& key, F&& fn) {
+    Entry* hit = nullptr;
+    for (auto& e : entries)
+      if (e.key == key) hit = &e;
+    clock++;
+    if (hit && hit->exec) {
+      hit->last = clock;
+      return hipGraphLaunch(hit->exec, st);
+    }
+    if (!hit) {
+      if (entries.size() >= kGraphCap) {
+        auto old = std::min_element(entries.begin(), entries.end(),
+                                    [](const Entry& a, const Entry& b) { return a.last < b.last; });
+        if (old->exec) (void)hipGraphExecDestroy(old->exec);
+        entries.erase(old);
+      }
+      entries.push_back(Entry{key, nullptr, clock});
+      return fn();
+    }
+    hit->last = clock;
+    std::shared_lock<std::shared_mutex> lk(device_op_mutex());
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    const hipError_t ef = fn();
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(st, &graph);
+    if (ef != hipSuccess || e != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return ef != hipSuccess ? ef : e;
+    }
+    e = hipGraphInstantiate(&hit->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      hit->exec = nullptr;
+      return e;
+    }
+    return hipGraphLaunch(hit->exec, st);
+  }
+};
+
+uint64_t next_object_id() {
+  static std::atomic<uint64_t> id{1};
+  return id++;
+}
 
 struct itp_ctx {
   int device = 0;
@@ -50,6 +148,9 @@ struct itp_ctx {
   // bisection) from this matrix size on; calls and rejections
   int td_min_n = 1024;
   long long td_checks = 0, td_rejects = 0;
+  // fixed launch sequences (propagation, overlap, Cholesky factor) replayed as HIP graphs
+  bool graphs = false;
+  GraphCache gcache;
   long long oa_done = 0, oa_fallback = 0;
   long long cl_calls = 0, cl_clusters = 0, cl_largest = 0, cl_fallback = 0;
 };
@@ -69,6 +170,7 @@ struct itp_potential {
 };
 
 struct itp_stateset {
+  uint64_t id = next_object_id();  // graph keys
   itp_grid* grid = nullptr;
   size_t N = 0;
   int prec = ITP_FP64;
@@ -129,6 +231,8 @@ struct MemberTabs {
 };
 
 struct itp_split {
+  uint64_t id = next_object_id();  // graph keys: id, and gen (bumped when the tables are rebuilt)
+  uint64_t gen = 0;
   itp_potential* pot = nullptr;
   int halforder = 1;
   double B = 0, eps = 0;
@@ -201,7 +305,7 @@ int guarded(itp_ctx* c, F f) {
 
 int dalloc(itp_ctx* c, void** p, size_t bytes) {
   if (bytes == 0) bytes = 16;
-  hipError_t e = hipMalloc(p, bytes);
+  hipError_t e = dev_malloc(p, bytes);
   if (e != hipSuccess) {
     *p = nullptr;
     (void)hipGetLastError();
@@ -214,7 +318,7 @@ int ensure_ws(itp_ctx* c, size_t bytes) {
   if (bytes <= c->ws_bytes) return ITP_OK;
   if (c->ws) {
     HIPCK(c, hipStreamSynchronize(c->stream));
-    HIPCK(c, hipFree(c->ws));
+    HIPCK(c, dev_free(c->ws));
     c->ws = nullptr;
     c->ws_bytes = 0;
   }
@@ -301,7 +405,7 @@ int upload_tables(itp_ctx* c, const std::vector<const std::vector<double>*>& tab
   ptrs.clear();
   for (auto t : tabs) {
     ptrs.push_back(t->empty() ? nullptr : *dev + off);
-    if (!t->empty()) HIPCK(c, hipMemcpy(*dev + off, t->data(), t->size() * sizeof(double), hipMemcpyHostToDevice));
+    if (!t->empty()) HIPCK(c, dev_memcpy(*dev + off, t->data(), t->size() * sizeof(double), hipMemcpyHostToDevice));
     off += t->size();
   }
   return ITP_OK;
@@ -322,13 +426,13 @@ int upload_tables32(itp_ctx* c, std::initializer_list<const std::vector<double>*
     for (size_t i = 0; i < t->size(); i++) h[off + i] = float((*t)[i]);
     off += t->size();
   }
-  if (total) HIPCK(c, hipMemcpy(*dev, h.data(), total * sizeof(float), hipMemcpyHostToDevice));
+  if (total) HIPCK(c, dev_memcpy(*dev, h.data(), total * sizeof(float), hipMemcpyHostToDevice));
   return ITP_OK;
 }
 
 void free_member(MemberTabs* m) {
-  if (m && m->dev) { (void)hipFree(m->dev); m->dev = nullptr; }
-  if (m && m->dev32) { (void)hipFree(m->dev32); m->dev32 = nullptr; }
+  if (m && m->dev) { (void)dev_free(m->dev); m->dev = nullptr; }
+  if (m && m->dev32) { (void)dev_free(m->dev32); m->dev32 = nullptr; }
 }
 
 // FP32 copies of a member's tables (for FP32 sweep programs)
@@ -356,6 +460,7 @@ int build_split_tables(itp_split* sp) {
   HIPCK(c, hipStreamSynchronize(c->stream));
   for (auto& m : sp->members) free_member(m.get());
   sp->members.clear();
+  sp->gen = next_object_id();
   for (int t = 1; t <= sp->halforder; t++) {
     std::unique_ptr<MemberTabs> m(new MemberTabs);
     const double h = sp->eps / t;
@@ -413,10 +518,19 @@ int propagate_impl(itp_split* sp, itp_stateset* s, bool noalloc = false) {
   for (int t = 1; t <= sp->halforder; t++)
     member_program<R>(br, sp->members[t - 1]->d, has_v ? t : 1, has_v, in, work, out, t > 1, prog);
   Phase ph(c, ITP_PHASE_PROPAGATE);
-  GpuBackend be;
-  be.st = c->stream;
-  run_program<R>(be, g, gr->lg[sizeof(R) == 4], prog, (int)s->N);
-  if (be.err != hipSuccess) return set_err(c, ITP_E_HIP, std::string("propagate: ") + hipGetErrorString(be.err));
+  auto launches = [&]() -> hipError_t {
+    GpuBackend be;
+    be.st = c->stream;
+    run_program<R>(be, g, gr->lg[sizeof(R) == 4], prog, (int)s->N);
+    return be.err;
+  };
+  // (the entry point that allocates nothing launches directly: instantiating a graph allocates)
+  const hipError_t err =
+      c->graphs && !noalloc
+          ? c->gcache.run(c->stream, {sp->id ^ (sp->gen << 20), s->id, (uint64_t)s->cur, (uint64_t)(uintptr_t)work},
+                          launches)
+          : launches();
+  if (err != hipSuccess) return set_err(c, ITP_E_HIP, std::string("propagate: ") + hipGetErrorString(err));
   ph.stop();
   s->cur ^= 1;
   return ITP_OK;
@@ -775,17 +889,19 @@ int gram_impl(itp_stateset* s) {
   itp_ctx* c = s->grid->ctx;
   const GridInfo& g = s->grid->g;
   Phase po(c, ITP_PHASE_OVERLAP);
-  HIPCK(c, hipMemsetAsync(s->bad, 0, sizeof(int), c->stream));
-  if (sizeof(R) == 4) {
-    const cplx<float>* psi = (const cplx<float>*)s->buf[s->cur];
-    HIPCK(c, launch_gram_bf16x3(psi, psi, g.M, (int)s->N, s->nc, s->part, s->nchunk32, s->kchunk32, g.dx, s->A[0],
-                                s->np, s->bad, c->stream));
-    po.stop();
-    return ITP_OK;
-  }
-  HIPCK(c, launch_gram<R>((const cplx<R>*)s->buf[s->cur], (const cplx<R>*)s->buf[s->cur], g.M, (int)s->N, s->nc,
-                          s->part, s->nchunk, s->kchunk, g.dx,
-                          s->A[0], s->np, s->bad, c->stream));
+  auto launches = [&]() -> hipError_t {
+    // the non-finite flag is cleared by a kernel (graph-safe)
+    hipError_t e = launch_zero_ints(s->bad, 1, c->stream);
+    if (e != hipSuccess) return e;
+    if (sizeof(R) == 4) {
+      const cplx<float>* psi = (const cplx<float>*)s->buf[s->cur];
+      return launch_gram_bf16x3(psi, psi, g.M, (int)s->N, s->nc, s->part, s->nchunk32, s->kchunk32, g.dx, s->A[0],
+                                s->np, s->bad, c->stream);
+    }
+    return launch_gram<R>((const cplx<R>*)s->buf[s->cur], (const cplx<R>*)s->buf[s->cur], g.M, (int)s->N, s->nc,
+                          s->part, s->nchunk, s->kchunk, g.dx, s->A[0], s->np, s->bad, c->stream);
+  };
+  HIPCK(c, c->graphs ? c->gcache.run(c->stream, {s->id, (uint64_t)s->cur, 1, 0}, launches) : launches());
   po.stop();
   return ITP_OK;
 }
@@ -826,8 +942,15 @@ int cholqr_impl(itp_stateset* s, int* used_eigen, size_t* bad_n, double* bad_lam
     // FP32 fields give an overlap with ~1e-7 relative errors: pivots below
     // 1e-5 of their diagonal would be dominated by them
     const double tol = c->chol_tol > 0 ? c->chol_tol : (sizeof(R) == 4 ? 1e-5 : 1e-12);
-    HIPCK(c, launch_cholinv(s->A[0], s->np, (int)s->N, s->A[1], s->diag0, s->C, s->nc, s->chol_fail, tol, c->stream));
-    HIPCK(c, launch_chol_diag_range(s->A[1], s->np, (int)s->N, s->chol_mm, c->stream));
+    auto launches = [&]() -> hipError_t {
+      hipError_t e = launch_cholinv(s->A[0], s->np, (int)s->N, s->A[1], s->diag0, s->C, s->nc, s->chol_fail, tol,
+                                    c->stream);
+      if (e != hipSuccess) return e;
+      return launch_chol_diag_range(s->A[1], s->np, (int)s->N, s->chol_mm, c->stream);
+    };
+    uint64_t tbits;
+    std::memcpy(&tbits, &tol, sizeof(tbits));
+    HIPCK(c, c->graphs ? c->gcache.run(c->stream, {s->id, tbits, 2, 0}, launches) : launches());
     std::vector<double> mm(2 + s->N, 0.0);
     HIPCK(c, hipMemcpyAsync(&fail[0], s->chol_fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCK(c, hipMemcpyAsync(&fail[1], s->bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -946,8 +1069,8 @@ struct DevFields {
   cplx<double>* p = nullptr;
   cplx<double>* w = nullptr;
   ~DevFields() {
-    if (p) (void)hipFree(p);
-    if (w) (void)hipFree(w);
+    if (p) (void)dev_free(p);
+    if (w) (void)dev_free(w);
   }
 };
 
@@ -959,14 +1082,14 @@ int hook_begin(itp_ctx* c, const double* host, size_t nelem, DevFields& df, bool
     rc = dalloc(c, (void**)&df.w, nelem * 16);
     if (rc) return rc;
   }
-  HIPCK(c, hipMemcpy(df.p, host, nelem * 16, hipMemcpyHostToDevice));
+  HIPCK(c, dev_memcpy(df.p, host, nelem * 16, hipMemcpyHostToDevice));
   return ITP_OK;
 }
 
 int hook_end(itp_ctx* c, double* host, size_t nelem, DevFields& df, hipError_t err) {
   if (err != hipSuccess) return set_err(c, ITP_E_HIP, hipGetErrorString(err));
   HIPCK(c, hipStreamSynchronize(c->stream));
-  HIPCK(c, hipMemcpy(host, df.p, nelem * 16, hipMemcpyDeviceToHost));
+  HIPCK(c, dev_memcpy(host, df.p, nelem * 16, hipMemcpyDeviceToHost));
   return ITP_OK;
 }
 
@@ -981,7 +1104,10 @@ int itp_ctx_create(int device, itp_ctx** out) {
   c->device = device;
   hipError_t e = hipSetDevice(device);
   if (e != hipSuccess) return ITP_E_HIP;
-  if (hipStreamCreate(&c->stream) != hipSuccess) return ITP_E_HIP;
+  // non-blocking: synchronous copies of other threads' contexts (on the null
+  // stream) do not join this stream, so a graph capture on it is not broken
+  // by them; every host read of this stream's results synchronises it first
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return ITP_E_HIP;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) return ITP_E_HIP;
   c->num_cus = prop.multiProcessorCount;
@@ -996,7 +1122,9 @@ int itp_ctx_create(int device, itp_ctx** out) {
 void itp_ctx_destroy(itp_ctx* c) {
   if (!c) return;
   (void)hipStreamSynchronize(c->stream);
-  if (c->ws) (void)hipFree(c->ws);
+  c->gcache.clear();
+  if (c->graphs) graph_contexts()--;
+  if (c->ws) (void)dev_free(c->ws);
   for (auto& e : c->evpool)
     if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(c->stream);
@@ -1026,6 +1154,19 @@ int itp_ctx_phase_times(itp_ctx* c, double* ms, long long* counts) {
   }
   return ITP_OK;
 }
+int itp_ctx_set_graphs(itp_ctx* c, int on) {
+  if (!c) return ITP_E_INVALID;
+  const bool was = c->graphs;
+  c->graphs = on != 0;
+  if (c->graphs && !was) graph_contexts()++;
+  if (!c->graphs && was) {
+    (void)hipStreamSynchronize(c->stream);
+    c->gcache.clear();
+    graph_contexts()--;
+  }
+  return ITP_OK;
+}
+
 int itp_ctx_set_eig_check_min(itp_ctx* c, int n) {
   if (!c || n < 1) return ITP_E_INVALID;
   c->td_min_n = n;
@@ -1140,7 +1281,7 @@ int itp_grid_create(itp_ctx* c, size_t sx, size_t sy, double dx, int boundary, i
     all.insert(all.end(), mky.begin(), mky.end());
     int rc = dalloc(c, (void**)&gr->d_tw, all.size() * 16);
     if (rc) return rc;
-    HIPCK(c, hipMemcpy(gr->d_tw, all.data(), all.size() * 16, hipMemcpyHostToDevice));
+    HIPCK(c, dev_memcpy(gr->d_tw, all.data(), all.size() * 16, hipMemcpyHostToDevice));
     gr->lg[0] = make_geom(gr->g, gr->d_tw, 16);
     gr->lg[1] = make_geom(gr->g, gr->d_tw, 8);
     *out = gr.release();
@@ -1150,8 +1291,8 @@ int itp_grid_create(itp_ctx* c, size_t sx, size_t sy, double dx, int boundary, i
 
 void itp_grid_destroy(itp_grid* g) {
   if (!g) return;
-  (void)hipDeviceSynchronize();  // no parent access: parents may already be gone
-  if (g->d_tw) (void)hipFree(g->d_tw);
+  (void)dev_sync();  // no parent access: parents may already be gone
+  if (g->d_tw) (void)dev_free(g->d_tw);
   delete g;
 }
 
@@ -1222,7 +1363,7 @@ int itp_potential_create(itp_grid* g, const double* V, itp_potential** out) {
       p->V.assign(V, V + g->g.M);
       int rc = dalloc(c, (void**)&p->dV, g->g.M * sizeof(double));
       if (rc) return rc;
-      HIPCK(c, hipMemcpy(p->dV, V, g->g.M * sizeof(double), hipMemcpyHostToDevice));
+      HIPCK(c, dev_memcpy(p->dV, V, g->g.M * sizeof(double), hipMemcpyHostToDevice));
     }
     *out = p.release();
     return ITP_OK;
@@ -1272,7 +1413,7 @@ int itp_potential_add_impurities(itp_potential* p, int kind, const double* data,
       int rc = dalloc(c, (void**)&p->dV, gi.M * sizeof(double));
       if (rc) return rc;
       p->V.assign(gi.M, 0.0);
-      HIPCK(c, hipMemcpy(p->dV, p->V.data(), gi.M * sizeof(double), hipMemcpyHostToDevice));
+      HIPCK(c, dev_memcpy(p->dV, p->V.data(), gi.M * sizeof(double), hipMemcpyHostToDevice));
     }
     if (kind == kImpDelta) {
       // one grid point each (DeltaImpurities::add_noise, noise.cpp:80-88): on the host copy
@@ -1287,7 +1428,7 @@ int itp_potential_add_impurities(itp_potential* p, int kind, const double* data,
         at[n] = y * (long long)gi.sx + x;
       }
       for (size_t n = 0; n < count; n++) p->V[at[n]] += data[3 * n + 2];
-      HIPCK(c, hipMemcpy(p->dV, p->V.data(), gi.M * sizeof(double), hipMemcpyHostToDevice));
+      HIPCK(c, dev_memcpy(p->dV, p->V.data(), gi.M * sizeof(double), hipMemcpyHostToDevice));
       if (count) p->null = false;
       return ITP_OK;
     }
@@ -1298,13 +1439,13 @@ int itp_potential_add_impurities(itp_potential* p, int kind, const double* data,
     double* dd = nullptr;
     int rc = dalloc(c, (void**)&dd, std::max<size_t>(1, count) * stride * sizeof(double));
     if (rc) return rc;
-    hipError_t e = hipMemcpy(dd, data, count * stride * sizeof(double), hipMemcpyHostToDevice);
+    hipError_t e = dev_memcpy(dd, data, count * stride * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = launch_impurities(kind, dd, stride, (long long)count, g0, g1, gi.sx, gi.sy, gi.dx, p->dV,
                                                c->stream);
     if (e == hipSuccess)
       e = hipMemcpyAsync(p->V.data(), p->dV, gi.M * sizeof(double), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    (void)hipFree(dd);
+    (void)dev_free(dd);
     if (e != hipSuccess) return set_err(c, ITP_E_HIP, std::string("impurities: ") + hipGetErrorString(e));
     if (count) p->null = false;
     return ITP_OK;
@@ -1320,8 +1461,8 @@ int itp_potential_values(const itp_potential* p, double* V) {
 
 void itp_potential_destroy(itp_potential* p) {
   if (!p) return;
-  (void)hipDeviceSynchronize();  // no parent access: parents may already be gone
-  if (p->dV) (void)hipFree(p->dV);
+  (void)dev_sync();  // no parent access: parents may already be gone
+  if (p->dV) (void)dev_free(p->dV);
   delete p;
 }
 
@@ -1342,7 +1483,7 @@ int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out
     int rc;
     for (int b = 0; b < 2; b++) {
       if ((rc = dalloc(c, &s->buf[b], N * M * s->esz))) return rc;
-      HIPCK(c, hipMemset(s->buf[b], 0, N * M * s->esz));
+      HIPCK(c, dev_memset(s->buf[b], 0, N * M * s->esz));
     }
     s->nc = (int)((N + 63) / 64 * 64);
     s->np = s->nc;  // the eigensolver works on the zero-padded nc x nc matrix
@@ -1376,7 +1517,7 @@ int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out
     for (int b = 0; b < 2; b++) {
       if ((rc = dalloc(c, (void**)&s->A[b], np2))) return rc;
       if ((rc = dalloc(c, (void**)&s->V[b], np2))) return rc;
-      HIPCK(c, hipMemset(s->A[b], 0, np2));
+      HIPCK(c, dev_memset(s->A[b], 0, np2));
     }
     if ((rc = dalloc(c, (void**)&s->Vinit, np2))) return rc;
     {
@@ -1399,11 +1540,11 @@ int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out
     {
       std::vector<cplx<double>> I((size_t)s->np * s->np, mk<double>(0.0, 0.0));
       for (int i = 0; i < s->np; i++) I[(size_t)i * s->np + i] = mk<double>(1.0, 0.0);
-      HIPCK(c, hipMemcpy(s->Vinit, I.data(), np2, hipMemcpyHostToDevice));
+      HIPCK(c, dev_memcpy(s->Vinit, I.data(), np2, hipMemcpyHostToDevice));
     }
     if ((rc = dalloc(c, (void**)&s->C, (size_t)s->nc * s->nc * 16))) return rc;
     if (precision == ITP_FP32 && (rc = dalloc(c, (void**)&s->C32, (size_t)s->nc * s->nc * 8))) return rc;
-    HIPCK(c, hipMemset(s->C, 0, (size_t)s->nc * s->nc * 16));
+    HIPCK(c, dev_memset(s->C, 0, (size_t)s->nc * s->nc * 16));
     if ((rc = dalloc(c, (void**)&s->lam, N * sizeof(double)))) return rc;
     if ((rc = dalloc(c, (void**)&s->diag0, (size_t)s->np * sizeof(double)))) return rc;
     if ((rc = dalloc(c, (void**)&s->chol_fail, sizeof(int)))) return rc;
@@ -1423,14 +1564,14 @@ int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out
 
 void itp_stateset_destroy(itp_stateset* s) {
   if (!s) return;
-  (void)hipDeviceSynchronize();  // no parent access: parents may already be gone
+  (void)dev_sync();  // no parent access: parents may already be gone
   void* ptrs[] = {s->buf[0], s->buf[1], s->part, s->A[0], s->A[1], s->V[0], s->V[1], s->Vinit, s->bjQ, s->bjD,
                   s->bjslots, s->bjident, s->oa_work, s->oa_lam, s->oa_slots, s->oa_state, s->cl_hdr, s->cl_pairs,
                   s->cl_members, s->cl_offsets, s->Vsave, s->oa_part, s->C, s->C32,
                   s->lam, s->perm, s->info, s->bad, s->epart, s->dE, s->dmean, s->diag0, s->chol_fail, s->chol_mm, s->norms,
                   s->td_scratch};
   for (void* p : ptrs)
-    if (p) (void)hipFree(p);
+    if (p) (void)dev_free(p);
   delete s;
 }
 
@@ -1494,7 +1635,7 @@ int itp_split_create(itp_potential* pot, int halforder, double eps, double B, it
 
 void itp_split_destroy(itp_split* sp) {
   if (!sp) return;
-  (void)hipDeviceSynchronize();  // no parent access: parents may already be gone
+  (void)dev_sync();  // no parent access: parents may already be gone
   for (auto& m : sp->members) free_member(m.get());
   delete sp;
 }
@@ -1571,7 +1712,7 @@ int itp_rayleigh_quotients(itp_hamiltonian* h, const itp_stateset* src, itp_stat
     int rc = energies_impl<double>(h, dst, E, nullptr, nullptr, nullptr, true);
     if (rc) return rc;
     std::vector<double> nrm(src->N);
-    HIPCK(c, hipMemcpy(nrm.data(), dst->norms, src->N * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCK(c, dev_memcpy(nrm.data(), dst->norms, src->N * sizeof(double), hipMemcpyDeviceToHost));
     const double dx2 = g.dx * g.dx;
     for (size_t k = 0; k < src->N; k++) E[k] /= nrm[k] * dx2;
     return ITP_OK;
@@ -1663,8 +1804,8 @@ int itp_hamiltonian_create(itp_potential* pot, double B, itp_hamiltonian** out) 
 
 void itp_hamiltonian_destroy(itp_hamiltonian* h) {
   if (!h) return;
-  (void)hipDeviceSynchronize();  // no parent access: parents may already be gone
-  if (h->dev) (void)hipFree(h->dev);
+  (void)dev_sync();  // no parent access: parents may already be gone
+  if (h->dev) (void)dev_free(h->dev);
   delete h;
 }
 
@@ -1697,7 +1838,7 @@ int itp_apply_expkinetic_host(itp_grid* g, double B, double e, double coeff, dou
     std::vector<const double*> p;
     int rc = upload_tables(c, {&t.mult, &t.xm, &t.xm2, &t.ym}, &dev, p);
     if (rc) return rc;
-    std::unique_ptr<double, void (*)(double*)> hold(dev, [](double* q) { (void)hipFree(q); });
+    std::unique_ptr<double, void (*)(double*)> hold(dev, [](double* q) { (void)dev_free(q); });
     DevFields df;
     const size_t nel = n * (size_t)g->g.M;
     if ((rc = hook_begin(c, fields, nel, df, true))) return rc;
@@ -1722,7 +1863,7 @@ int itp_apply_exppotential_host(itp_potential* pot, double e, double coeff, doub
     if (!pot->null) exppot_values(g->g, pot->V.data(), e, coeff, prefactor, vals);
     int rc = upload_tables(c, {&vals}, &dev, p);
     if (rc) return rc;
-    std::unique_ptr<double, void (*)(double*)> hold(dev, [](double* q) { (void)hipFree(q); });
+    std::unique_ptr<double, void (*)(double*)> hold(dev, [](double* q) { (void)dev_free(q); });
     DevFields df;
     const size_t nel = n * (size_t)g->g.M;
     if ((rc = hook_begin(c, fields, nel, df, false))) return rc;
@@ -1754,7 +1895,7 @@ int itp_apply_secondorder_host(itp_potential* pot, double B, double e, double pr
     std::vector<const double*> p;
     int rc = upload_tables(c, {&m.host.mult, &m.host.xm, &m.host.xm2, &m.host.ym, &m.vh, &m.vf, &m.vp}, &m.dev, p);
     if (rc) return rc;
-    std::unique_ptr<double, void (*)(double*)> hold(m.dev, [](double* q) { (void)hipFree(q); });
+    std::unique_ptr<double, void (*)(double*)> hold(m.dev, [](double* q) { (void)dev_free(q); });
     m.d.kin = KinDev{p[0], p[1], p[2], p[3]};
     m.d.vhalf = p[4];
     m.d.vfull = p[5];
@@ -1764,7 +1905,7 @@ int itp_apply_secondorder_host(itp_potential* pot, double B, double e, double pr
     if ((rc = hook_begin(c, fields, nel, df, true))) return rc;
     cplx<double>* out = nullptr;
     if ((rc = dalloc(c, (void**)&out, nel * 16))) return rc;
-    std::unique_ptr<cplx<double>, void (*)(cplx<double>*)> hold2(out, [](cplx<double>* q) { (void)hipFree(q); });
+    std::unique_ptr<cplx<double>, void (*)(cplx<double>*)> hold2(out, [](cplx<double>* q) { (void)dev_free(q); });
     std::vector<Sweep<double>> prog;
     member_program<double>(branch_of(B, g->g.bt), m.d, has_v ? exponent : 1, has_v, df.p, df.w, out, false, prog);
     GpuBackend be;
@@ -1787,7 +1928,7 @@ int itp_apply_hamiltonian_host(itp_hamiltonian* h, double* fields, size_t n) {
     if (rc) return rc;
     cplx<double>* out = nullptr;
     if ((rc = dalloc(c, (void**)&out, nel * 16))) return rc;
-    std::unique_ptr<cplx<double>, void (*)(cplx<double>*)> hold(out, [](cplx<double>* q) { (void)hipFree(q); });
+    std::unique_ptr<cplx<double>, void (*)(cplx<double>*)> hold(out, [](cplx<double>* q) { (void)dev_free(q); });
     std::vector<Sweep<double>> prog;
     hamiltonian_program<double>(g->g, g->lg[0], branch_of(h->B, g->g.bt), h->d, h->pot->null ? nullptr : h->pot->dV,
                                 df.p, df.w, out, nullptr, nullptr, prog);
@@ -1817,13 +1958,13 @@ int itp_eigh_host(itp_ctx* c, size_t n, double* A, double* w) {
       for (size_t j = 0; j < n; j++)
         full[i * s->np + j] = (i <= j) ? a[j * n + i] : conj(a[i * n + j]);  // upper triangle, column-major
     for (size_t i = 0; i < n; i++) full[i * s->np + i].y = 0;
-    HIPCK(c, hipMemcpy(s->A[0], full.data(), full.size() * 16, hipMemcpyHostToDevice));
-    HIPCK(c, hipMemset(s->bad, 0, sizeof(int)));
+    HIPCK(c, dev_memcpy(s->A[0], full.data(), full.size() * 16, hipMemcpyHostToDevice));
+    HIPCK(c, dev_memset(s->bad, 0, sizeof(int)));
     rc = eigen_stage(s, nullptr, nullptr, false);
     if (rc && rc != ITP_E_NONPOSITIVE_EIG && rc != ITP_E_NONNORMAL_EIG) return rc;
     std::vector<cplx<double>> C((size_t)s->nc * s->nc);
-    HIPCK(c, hipMemcpy(C.data(), s->C, C.size() * 16, hipMemcpyDeviceToHost));
-    HIPCK(c, hipMemcpy(w, s->lam, n * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCK(c, dev_memcpy(C.data(), s->C, C.size() * 16, hipMemcpyDeviceToHost));
+    HIPCK(c, dev_memcpy(w, s->lam, n * sizeof(double), hipMemcpyDeviceToHost));
     cplx<double>* o = (cplx<double>*)A;
     for (size_t r = 0; r < n; r++)
       for (size_t cc = 0; cc < n; cc++) o[r * n + cc] = C[cc * s->nc + r];  // column r = eigenvector r
@@ -1849,7 +1990,7 @@ int itp_eigvalsh_host(itp_ctx* c, size_t n, const double* A, double* w) {
       for (size_t j = 0; j < n; j++)
         full[i * s->np + j] = (i <= j) ? a[j * n + i] : conj(a[i * n + j]);  // upper triangle, column-major
     for (size_t i = 0; i < n; i++) full[i * s->np + i].y = 0;
-    HIPCK(c, hipMemcpy(s->A[0], full.data(), full.size() * 16, hipMemcpyHostToDevice));
+    HIPCK(c, dev_memcpy(s->A[0], full.data(), full.size() * 16, hipMemcpyHostToDevice));
     std::vector<double> ev;
     if ((rc = hermitian_eigenvalues(s, s->A[0], ev))) return rc;
     std::copy(ev.begin(), ev.end(), w);

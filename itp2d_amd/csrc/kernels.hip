@@ -67,6 +67,14 @@ hipError_t launch_reg_sweep(const SweepArgs<R>& a, int nblk, int nfields, hipStr
   }
 }
 
+__global__ void zero_ints_kernel(int* p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+hipError_t launch_zero_ints(int* p, int n, hipStream_t st) {
+  hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(64), 0, st, p, n);
+  return hipGetLastError();
+}
+
 template <class R>
 hipError_t launch_sweep(const SweepArgs<R>& a, int nblk, int nfields, hipStream_t st) {
   if (nblk <= 0 || nfields <= 0) return hipSuccess;
