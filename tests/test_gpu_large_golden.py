@@ -1,11 +1,13 @@
-"""The reduced forms of the large configurations against oracle goldens
+"""The reduced forms of the large configurations, and the 256x256 Dirichlet
+configuration at its full size, against oracle goldens
 (tests/golden/itp_<name>.npz, made by tests/golden/make_itp_golden.py):
-c3_ho128 is the 512x512 oscillator's physics at 128x128 with N = 200, and
+c3_ho128 is the 512x512 oscillator's physics at 128x128 with N = 200,
 c5_stadium256 the magnetic hard-wall stadium (B = 2, 6th-order split,
-eps 0.02, recovery) at 256x256 with N = 100. The FP64 path reproduces each
-golden run (steps, converged count, recoveries' outcome, eigenvalues to
-1e-10); the mixed path converges every state the golden converges, within
-1e-8."""
+eps 0.02, recovery) at 256x256 with N = 100, and c2_dirichlet256 the full
+256x256, N = 500, B = 1 workload (the oracle's 55 steps took about an hour on
+8 cores). The FP64 path reproduces each golden run (steps, converged count,
+recoveries' outcome, eigenvalues to 1e-10); the mixed path converges every
+state the golden converges, within 1e-8."""
 import os
 
 import numpy as np
@@ -16,7 +18,7 @@ from itp2d_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
-NAMES = ["c3_ho128", "c5_stadium256"]
+NAMES = ["c3_ho128", "c5_stadium256", "c2_dirichlet256"]
 
 
 def say(msg):
