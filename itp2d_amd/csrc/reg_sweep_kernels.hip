@@ -61,7 +61,11 @@ using RowPrograms = std::tuple<
     CtProg<T_, F_>, CtProg<I_, T_, F_>, CtProg<I_, T_>,                      // B = 0 periodic
     CtProg<T_, F_, T_>, CtProg<T_, I_, T_, F_, T_>, CtProg<T_, I_, T_>,       // B != 0 periodic
     CtProg<T_, D2>, CtProg<D3, T_, D2>, CtProg<D3, T_>>;                      // B = 0 Dirichlet
-using XsRowPrograms = std::tuple<CtProg<T_, D2, X_, D2>, CtProg<X_, T_, D2, X_, D2>, CtProg<X_, T_>>;
+// (the FP64 last row sweep, split then table, spills registers when compiled
+// for its op list: the runtime kernel runs it)
+using XsRowPrograms = std::conditional_t<
+    sizeof(ITP_SWEEP_REAL) == 8, std::tuple<CtProg<T_, D2, X_, D2>, CtProg<X_, T_, D2, X_, D2>>,
+    std::tuple<CtProg<T_, D2, X_, D2>, CtProg<X_, T_, D2, X_, D2>, CtProg<X_, T_>>>;
 using ColPrograms = std::tuple<CtProg<F_, T_, I_>, CtProg<F_, P_, I_>, CtProg<D2, T_, D3>, CtProg<D2, P_, D3>>;
 
 template <class R, int N, int ORIENT, bool XS, class PG>
