@@ -13,8 +13,10 @@ wall time counts).
 
 value: converged states per second of the mixed-precision path, counting a
 state only if the FP64 path converges it too and the two energies agree to
-1e-8 relative. The FP64 path runs the same batch once (timed, its converged
-states/s in "fp64"); speedup_vs_fp64 is the ratio. The 256x256 Dirichlet
+1e-8 relative. The FP64 path runs the same batch after the same W warm-up
+solves, once (timed, its converged states/s in "fp64"); speedup_vs_fp64 is
+the ratio. Both paths tune the overlap's launch shape (--autotune 1, the
+default) in their warm-up solves. The 256x256 Dirichlet
 single-instance workload (N = 500, B = 1) is reported beside it ("c2").
 
 Prints one JSON line. Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
@@ -135,9 +137,10 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes": n * 4, "dispatches": 3}), flush=True)
 
 
-def batch_solve(I, insts, prec, workers):
+def batch_solve(I, insts, prec, workers, autotune=False):
     """One batched solve of all instances on one path: (rc, wall s, per-instance results)."""
     b = I.Batch(0, workers=workers)
+    b.set_autotune(int(autotune))
     for c, V in insts:
         b.add(dict(c, precision=prec), V)
     rc = b.run()
@@ -263,18 +266,23 @@ def c2_summary(I, W):
 def c4_headline(I, W, args):
     insts = W.c4_instances(64)
     states = sum(c["N"] for c, _ in insts)
-    log(f"c4: {len(insts)} instances, {states} states, workers {args.batch_workers}")
-    # the FP64 path once: timed, and the reference energies of the mixed path
-    rc64, wall64, res64 = batch_solve(I, insts, I.FP64, args.batch_workers)
+    log(f"c4: {len(insts)} instances, {states} states, workers {args.batch_workers}, autotune {args.autotune}")
+    tune = bool(args.autotune)
+    # the FP64 path: warm-up solves as for the mixed path (kernel loading, launch-shape tuning), then
+    # one timed solve, which also gives the reference energies of the mixed path
+    for w in range(args.warmup):
+        rc, wall, _ = batch_solve(I, insts, I.FP64, args.batch_workers, tune)
+        log(f"fp64 warm-up {w}: rc={rc} {wall:.2f}s")
+    rc64, wall64, res64 = batch_solve(I, insts, I.FP64, args.batch_workers, tune)
     conv64 = sum(int(r["conv"].sum()) for r in res64)
     it64 = sum(r["steps"] for r in res64)
     log(f"fp64 batch rc={rc64}: {wall64:.2f}s, {conv64} converged, {it64} ITP iterations")
     for w in range(args.warmup):
-        rc, wall, _ = batch_solve(I, insts, I.FP32, args.batch_workers)
+        rc, wall, _ = batch_solve(I, insts, I.FP32, args.batch_workers, tune)
         log(f"mixed warm-up {w}: rc={rc} {wall:.2f}s")
     walls, goods, fars, extras, worst, its, fp32, checks, skips, fb = [], [], [], [], 0.0, 0, 0, 0, 0, 0
     for k in range(args.steps):
-        rc, wall, res = batch_solve(I, insts, I.FP32, args.batch_workers)
+        rc, wall, res = batch_solve(I, insts, I.FP32, args.batch_workers, tune)
         good, far, extra, w_ = agreement(res, res64)
         walls.append(wall)
         goods.append(good)
@@ -311,7 +319,7 @@ def c4_headline(I, W, args):
         "data": "synthetic (64 seeded instances: grids 128^2/192^2/256^2, 50-400 states, B in [0,2), "
                 "oscillator plus 40 Gaussian impurities, seeded noise initial states)",
         "config": {"workload": "c4_batch64", "instances": len(insts), "states": states,
-                   "workers": args.batch_workers, "path": "mixed",
+                   "workers": args.batch_workers, "path": "mixed", "autotune": tune,
                    "converged_within_1e-8_of_fp64": goods[-1], "converged_farther_than_1e-8": fars[-1],
                    "converged_not_by_fp64": extras[-1], "max_rel_err_vs_fp64": worst,
                    "itp_iterations": its, "fp32_iterations": fp32, "fp64_checks": checks,
@@ -352,6 +360,8 @@ def main():
     ap.add_argument("--batch-workers", type=int, default=8,
                     help="batched configuration (--workload c4_batch64): worker threads, one stream each")
     ap.add_argument("--no-c2", action="store_true", help="batched configuration: skip the c2 summary")
+    ap.add_argument("--autotune", type=int, default=1, choices=[0, 1],
+                    help="batched configuration: launch-shape tuning of the overlap (done in the warm-up solves)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "

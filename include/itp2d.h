@@ -300,6 +300,25 @@ int itp_ctx_oa_stats(itp_ctx* ctx, long long* refined, long long* fallbacks);
  * the cluster start off (default 0.1). No counterpart in the reference, which
  * calls zheev (eigensolver.hpp:80-84): same eigenpairs, another route. */
 int itp_ctx_set_eig_cluster(itp_ctx* ctx, double tau);
+/* Launch-shape autotuning (default off): with it on, the first overlap of each
+ * shape (precision, grid points, states padded to 64) times the split-K chunk
+ * counts of the Gram product (the formula's, which fills the last wave, and
+ * 3/4, 1/2, 1/4 of it) and every later set of that shape uses the fastest.
+ * The tuned shapes are the process's "wisdom" (shared by all its contexts and
+ * used by those with autotuning on), exported and imported as text lines
+ * "gram <bytes per real: 4|8> <grid points> <padded states> <chunks>" (the
+ * counterpart of the reference's FFTW wisdom file, itp2d.cpp:85-93, 124-128).
+ * Only the launch shape changes: a chunk count changes the order of the Gram
+ * product's partial sums, so results agree to rounding, not bit for bit. */
+int itp_ctx_set_autotune(itp_ctx* ctx, int on);
+/* Writes the wisdom text into out (cap bytes, NUL-terminated, may be NULL) and
+ * its length into *len. */
+int itp_ctx_wisdom_export(const itp_ctx* ctx, char* out, size_t cap, size_t* len);
+/* Adds the shapes of a wisdom text (as exported) to the process's wisdom; a
+ * malformed line is rejected with ITP_E_INVALID and nothing is added. */
+int itp_ctx_wisdom_import(itp_ctx* ctx, const char* text);
+/* Shapes tuned by measurement in this context. */
+int itp_ctx_tunings(const itp_ctx* ctx);
 /* 1: the fixed launch sequences of a propagation (every sweep of the split
  * members), of an overlap (Gram product and reduction) and of a Cholesky
  * factorization are captured as HIP graphs on their second call with the same

@@ -31,6 +31,9 @@ typedef struct itp_batch itp_batch;
  * own context and stream. */
 int itp_batch_create(int device, int workers, itp_batch** out);
 void itp_batch_destroy(itp_batch* b);
+/* Launch-shape autotuning (itp_ctx_set_autotune) in every worker context of
+ * the following runs (default off). */
+int itp_batch_set_autotune(itp_batch* b, int on);
 /* Adds an instance: p (its eps_values included) and V ([sy][sx], NULL for the
  * null potential) are copied. *index receives the instance's position. */
 int itp_batch_add(itp_batch* b, const itps_params* p, const double* V, size_t* index);

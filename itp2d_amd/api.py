@@ -136,6 +136,10 @@ def lib():
             "itp_eigvalsh_host": [_vp, _sz, _dp, _dp],
             "itp_ctx_set_eig_check_min": [_vp, C.c_int],
             "itp_ctx_set_graphs": [_vp, C.c_int],
+            "itp_ctx_set_autotune": [_vp, C.c_int],
+            "itp_ctx_wisdom_export": [_vp, C.c_char_p, _sz, C.POINTER(_sz)],
+            "itp_ctx_wisdom_import": [_vp, C.c_char_p],
+            "itp_ctx_tunings": [_vp],
             "itp_ctx_eig_check_stats": [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
             "itp_init_states": [C.POINTER(C.c_uint64), _sz, _sz, C.c_double, _sz, _dp],
             "itps_create": [_vp, C.POINTER(_Params), _dp, _dp, C.POINTER(_vp)],
@@ -173,6 +177,7 @@ def lib():
             "itp_potential_values": [_vp, _dp],
             "itp_batch_create": [C.c_int, C.c_int, C.POINTER(_vp)],
             "itp_batch_destroy": [_vp],
+            "itp_batch_set_autotune": [_vp, C.c_int],
             "itp_batch_add": [_vp, C.POINTER(_Params), _dp, C.POINTER(_sz)],
             "itp_batch_size": [_vp],
             "itp_batch_run": [_vp],
@@ -326,6 +331,29 @@ class Context:
         """1: propagation, overlap and Cholesky factor launch sequences replayed as HIP graphs; 0 (default):
         launched one by one."""
         self.check(lib().itp_ctx_set_graphs(self.h, int(on)), "set_graphs")
+
+    def set_autotune(self, on):
+        """1: the first overlap of each shape (precision, grid points, padded states) times the Gram
+        product's split-K chunk counts and later sets of that shape use the fastest; 0 (default): the
+        formula's count."""
+        self.check(lib().itp_ctx_set_autotune(self.h, int(on)), "set_autotune")
+
+    def tunings(self):
+        """Shapes tuned by measurement in this context."""
+        return lib().itp_ctx_tunings(self.h)
+
+    def wisdom_export(self):
+        """The process's tuned shapes as text, one "gram <bytes per real> <grid points> <padded
+        states> <chunks>" line each."""
+        n = _sz(0)
+        self.check(lib().itp_ctx_wisdom_export(self.h, None, 0, C.byref(n)), "wisdom_export")
+        buf = C.create_string_buffer(n.value + 1)
+        self.check(lib().itp_ctx_wisdom_export(self.h, buf, n.value + 1, C.byref(n)), "wisdom_export")
+        return buf.value.decode()
+
+    def wisdom_import(self, text):
+        """Adds the shapes of a wisdom text; ITPError (nothing added) on a malformed line."""
+        self.check(lib().itp_ctx_wisdom_import(self.h, text.encode()), "wisdom_import")
 
     def set_eig_check_min(self, n):
         """Overlap matrices of at least n states get the eigenvalue check (tridiagonalization and
@@ -830,6 +858,12 @@ class Batch:
         if rc:
             raise ITPError(rc, "itp_batch_create")
         self._cfgs = []
+
+    def set_autotune(self, on):
+        """Autotuning (Context.set_autotune) in every worker context of later runs."""
+        rc = lib().itp_batch_set_autotune(self.h, int(on))
+        if rc:
+            raise ITPError(rc, "itp_batch_set_autotune")
 
     def add(self, cfg, V=None):
         p, eps = _params(cfg)

@@ -10,7 +10,13 @@ run failed.
 
 Options with no effect on this engine, accepted for compatibility:
 --highmem-orthonormalization (the rotation is always one out-of-place
-product), --wisdomfile (no FFTW plans) and --threads (one GPU stream).
+product) and --threads (one GPU stream).
+--wisdomfile names the file of measured launch shapes, the counterpart of the
+reference's FFTW wisdom (itp2d.cpp:85-93, 124-128; plans are measured there
+by default, FFTW_PATIENT, itp2d_common.hpp:56): runs tune the overlap's
+launch shape on first use of a shape, read the file first if it exists (a
+file in another format is ignored with a note, as fftw_import_wisdom_from_file
+fails on it) and write the tuned shapes back after the run.
 Not supported (an argument error): --noise impurities (the reference draws
 impurity realisations with its own generator; pass realisations through
 itp_potential_add_impurities instead) and --copy-states.
@@ -85,7 +91,8 @@ def argument_parser():
                                 "on an MI355X GPU (the reference itp2d's options).")
     add = a.add_argument
     add("--highmem-orthonormalization", action="store_true", help="accepted; no effect here")
-    add("--wisdomfile", default="fftw_wisdom", help="accepted; no effect here")
+    add("--wisdomfile", default="fftw_wisdom", help="file of measured launch shapes (read if present, "
+        "written after the run)")
     add("--noise", default=DEFAULTS["noise"])
     add("--impurity-type", default=None)
     add("--impurity-distribution", default=None)
@@ -192,7 +199,7 @@ def parse(argv):
         raise CLIError(f"invalid potential '{ns.potential}': {e}") from None
     opts = dict(save=save, datafile=ns.datafile, clobber=ns.force, verbosity=1 + ns.verbose - ns.quiet,
                 potential=(pname, pparams), potential_string=ns.potential, mixed=ns.mixed,
-                impurity_constraint=ns.impurity_constraint, threads=ns.threads)
+                impurity_constraint=ns.impurity_constraint, threads=ns.threads, wisdomfile=ns.wisdomfile)
     return cfg, opts
 
 
@@ -203,6 +210,8 @@ def run(cfg, opts, device=0):
     ctx = None
     try:
         ctx = I.Context(device)
+        ctx.set_autotune(1)
+        load_wisdom(ctx, opts.get("wisdomfile"), opts["verbosity"])
         dx = cfg["lenx"] / cfg["sx"]
         g = I.Grid(ctx, cfg["sx"], cfg["sy"], dx, cfg["boundary"])
         name, params = opts["potential"]
@@ -244,8 +253,34 @@ def run(cfg, opts, device=0):
         ctx.close()
         return 4
     status = 1 if s.error_flag else 0
+    save_wisdom(ctx, opts.get("wisdomfile"))
     ctx.close()  # frees the run's device objects before the interpreter exits
     return status
+
+
+def load_wisdom(ctx, path, verbosity=1):
+    """Adds the measured launch shapes of `path` if it exists (itp2d.cpp:86-92);
+    a file this engine did not write is left unused, with a note."""
+    if not path or not os.path.isfile(path):
+        return
+    from .api import ITPError
+    try:
+        with open(path) as f:
+            ctx.wisdom_import(f.read())
+    except (ITPError, OSError, UnicodeDecodeError) as e:
+        if verbosity > 0:
+            print(f"itp2d: wisdom file '{path}' not used ({e})", file=sys.stderr)
+
+
+def save_wisdom(ctx, path):
+    """Writes the process's measured launch shapes to `path` (itp2d.cpp:124-126)."""
+    if not path:
+        return
+    try:
+        with open(path, "w") as f:
+            f.write(ctx.wisdom_export())
+    except OSError as e:
+        print(f"itp2d: cannot write wisdom file '{path}': {e}", file=sys.stderr)
 
 
 def main(argv=None):

@@ -11,7 +11,10 @@
 #include <array>
 #include <atomic>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -114,6 +117,17 @@ struct GraphCache {
   }
 };
 
+// Tuned split-K chunk counts of the overlap per (precision bytes, grid points,
+// padded states), shared by every context of the process.
+struct GramWisdom {
+  std::mutex m;
+  std::map<std::array<long long, 3>, int> w;
+};
+GramWisdom& gram_wisdom() {
+  static GramWisdom g;
+  return g;
+}
+
 uint64_t next_object_id() {
   static std::atomic<uint64_t> id{1};
   return id++;
@@ -151,6 +165,10 @@ struct itp_ctx {
   // fixed launch sequences (propagation, overlap, Cholesky factor) replayed as HIP graphs
   bool graphs = false;
   GraphCache gcache;
+  // launch-shape tuning on first use of a shape (the tuned shapes are shared by
+  // the process's contexts: gram_wisdom())
+  bool autotune = false;
+  long long tunings = 0;
   long long oa_done = 0, oa_fallback = 0;
   long long cl_calls = 0, cl_clusters = 0, cl_largest = 0, cl_fallback = 0;
 };
@@ -182,6 +200,8 @@ struct itp_stateset {
   long long kchunk = 0;
   int nchunk32 = 1;             // split-K of the bf16x3 Gram (FP32 sets)
   long long kchunk32 = 0;
+  int nchunk_max = 1;           // chunks the partial-product buffer holds
+  bool gram_tuned = false;      // chunk count taken from the context's tuning (autotune)
   cplx<float>* C32 = nullptr;   // FP32 copy of the coefficients (FP32 sets)
   std::vector<double> lam_host; // ascending overlap eigenvalues of the last orthonormalization
   cplx<double>* part = nullptr;
@@ -884,6 +904,74 @@ int eigen_stage(itp_stateset* s, size_t* bad_n, double* bad_lambda, bool scale, 
   return check_eigenvalues(c, lam, bad_n, bad_lambda);
 }
 
+// Split-K of the overlap products for a requested chunk count (at most the
+// buffer's): FP64 chunks of whole 16-element groups, bf16x3 chunks of whole
+// 512-element groups.
+void set_gram_chunks(itp_stateset* s, long long nchunk) {
+  const long long M = s->grid->g.M;
+  nchunk = std::max<long long>(1, std::min<long long>(nchunk, s->nchunk_max));
+  s->kchunk = ((M + nchunk - 1) / nchunk + 15) / 16 * 16;
+  s->nchunk = (int)((M + s->kchunk - 1) / s->kchunk);
+  s->kchunk32 = ((M + nchunk - 1) / nchunk + 511) / 512 * 512;
+  s->nchunk32 = (int)((M + s->kchunk32 - 1) / s->kchunk32);
+}
+
+// The autotuner of the overlap's split-K: on the first overlap of a shape
+// (precision, grid points, padded states) it times the formula's chunk count
+// (sized for full last waves) and 3/4, 1/2 and 1/4 of it, two launches each,
+// and keeps the fastest for every later set of that shape. launches() runs one
+// overlap of the set's current buffer; it always runs last with the chosen
+// count (also when the shape was tuned before), so the overlap it leaves is
+// the one the caller asked for.
+template <class F>
+int tune_gram(itp_stateset* s, int esz, F&& launches) {
+  itp_ctx* c = s->grid->ctx;
+  s->gram_tuned = true;
+  const std::array<long long, 3> key = {esz, s->grid->g.M, s->nc};
+  {
+    std::lock_guard<std::mutex> lk(gram_wisdom().m);
+    auto it = gram_wisdom().w.find(key);
+    if (it != gram_wisdom().w.end()) {
+      set_gram_chunks(s, it->second);
+      HIPCK(c, launches());
+      return ITP_OK;
+    }
+  }
+  const long long n0 = s->nchunk_max;
+  std::vector<long long> cand = {n0, (3 * n0 + 3) / 4, (n0 + 1) / 2, (n0 + 3) / 4};
+  std::sort(cand.begin(), cand.end());
+  cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+  hipEvent_t e0, e1;
+  HIPCK(c, hipEventCreate(&e0));
+  HIPCK(c, hipEventCreate(&e1));
+  long long best = n0;
+  float best_ms = 1e30f;
+  for (long long n : cand) {
+    set_gram_chunks(s, n);
+    float ms = 1e30f;
+    for (int rep = 0; rep < 2; rep++) {
+      HIPCK(c, hipEventRecord(e0, c->stream));
+      HIPCK(c, launches());
+      HIPCK(c, hipEventRecord(e1, c->stream));
+      HIPCK(c, hipEventSynchronize(e1));
+      float t = 0;
+      HIPCK(c, hipEventElapsedTime(&t, e0, e1));
+      ms = std::min(ms, t);
+    }
+    if (ms < best_ms) { best_ms = ms; best = n; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  set_gram_chunks(s, best);
+  {
+    std::lock_guard<std::mutex> lk(gram_wisdom().m);
+    gram_wisdom().w[key] = (int)best;
+  }
+  c->tunings++;
+  HIPCK(c, launches());
+  return ITP_OK;
+}
+
 template <class R>
 int gram_impl(itp_stateset* s) {
   itp_ctx* c = s->grid->ctx;
@@ -901,6 +989,12 @@ int gram_impl(itp_stateset* s) {
     return launch_gram<R>((const cplx<R>*)s->buf[s->cur], (const cplx<R>*)s->buf[s->cur], g.M, (int)s->N, s->nc,
                           s->part, s->nchunk, s->kchunk, g.dx, s->A[0], s->np, s->bad, c->stream);
   };
+  if (c->autotune && !s->gram_tuned) {
+    // the tuning's last launch is this call's overlap
+    int rc = tune_gram(s, (int)sizeof(R), launches);
+    po.stop();
+    return rc;
+  }
   HIPCK(c, c->graphs ? c->gcache.run(c->stream, {s->id, (uint64_t)s->cur, 1, 0}, launches) : launches());
   po.stop();
   return ITP_OK;
@@ -1166,6 +1260,50 @@ int itp_ctx_set_graphs(itp_ctx* c, int on) {
   }
   return ITP_OK;
 }
+
+int itp_ctx_set_autotune(itp_ctx* c, int on) {
+  if (!c) return ITP_E_INVALID;
+  c->autotune = on != 0;
+  return ITP_OK;
+}
+
+int itp_ctx_wisdom_export(const itp_ctx* c, char* out, size_t cap, size_t* len) {
+  if (!c || !len) return ITP_E_INVALID;
+  std::string t;
+  std::lock_guard<std::mutex> lk(gram_wisdom().m);
+  for (const auto& kv : gram_wisdom().w)
+    t += "gram " + std::to_string(kv.first[0]) + " " + std::to_string(kv.first[1]) + " " +
+         std::to_string(kv.first[2]) + " " + std::to_string(kv.second) + "\n";
+  *len = t.size();
+  if (out && cap > 0) {
+    const size_t n = std::min(cap - 1, t.size());
+    std::memcpy(out, t.data(), n);
+    out[n] = 0;
+  }
+  return ITP_OK;
+}
+
+int itp_ctx_wisdom_import(itp_ctx* c, const char* text) {
+  if (!c || !text) return ITP_E_INVALID;
+  std::map<std::array<long long, 3>, int> w;
+  const char* p = text;
+  while (*p) {
+    char kind[16] = {0};
+    long long a = 0, b = 0, d = 0;
+    int n = 0, used = 0;
+    if (std::sscanf(p, "%15s %lld %lld %lld %d%n", kind, &a, &b, &d, &n, &used) != 5 || std::strcmp(kind, "gram") ||
+        (a != 4 && a != 8) || b < 1 || d < 64 || d % 64 || n < 1)
+      return set_err(c, ITP_E_INVALID, "wisdom: malformed line");
+    w[{a, b, d}] = n;
+    p += used;
+    while (*p == '\n' || *p == ' ' || *p == '\r' || *p == '\t') p++;
+  }
+  std::lock_guard<std::mutex> lk(gram_wisdom().m);
+  for (const auto& kv : w) gram_wisdom().w[kv.first] = kv.second;
+  return ITP_OK;
+}
+
+int itp_ctx_tunings(const itp_ctx* c) { return c ? (int)c->tunings : 0; }
 
 int itp_ctx_set_eig_check_min(itp_ctx* c, int n) {
   if (!c || n < 1) return ITP_E_INVALID;
@@ -1512,6 +1650,7 @@ int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out
     s->kchunk32 = ((long long)M + nchunk - 1) / nchunk;
     s->kchunk32 = (s->kchunk32 + 511) / 512 * 512;
     s->nchunk32 = (int)(((long long)M + s->kchunk32 - 1) / s->kchunk32);
+    s->nchunk_max = s->nchunk;
     const size_t np2 = (size_t)s->np * s->np * 16;
     if ((rc = dalloc(c, (void**)&s->part, (size_t)s->nchunk * per_chunk))) return rc;
     for (int b = 0; b < 2; b++) {

@@ -35,6 +35,7 @@ struct Instance {
 struct itp_batch {
   int device = 0;
   int workers = 1;
+  bool autotune = false;
   std::vector<Instance> inst;
   double wall = 0;
 };
@@ -83,6 +84,12 @@ int itp_batch_create(int device, int workers, itp_batch** out) {
 
 void itp_batch_destroy(itp_batch* b) { delete b; }
 
+int itp_batch_set_autotune(itp_batch* b, int on) {
+  if (!b) return ITP_E_INVALID;
+  b->autotune = on != 0;
+  return ITP_OK;
+}
+
 int itp_batch_add(itp_batch* b, const itps_params* p, const double* V, size_t* index) {
   if (!b || !p || p->sx == 0 || p->sy == 0 || p->N == 0) return ITP_E_INVALID;
   try {
@@ -124,6 +131,7 @@ int itp_batch_run(itp_batch* b) {
   auto worker = [&]() {
     itp_ctx* ctx = nullptr;
     int rc = itp_ctx_create(b->device, &ctx);
+    if (!rc && b->autotune) rc = itp_ctx_set_autotune(ctx, 1);
     for (;;) {
       const size_t k = next.fetch_add(1);
       if (k >= n) break;

@@ -81,17 +81,24 @@ def test_convergence_strings(s, want):
 @pytest.mark.gpu
 def test_cli_run_matches_api(ctx, tmp_path):
     """A run through the command line equals the same run through the API bit
-    for bit, and its result file holds that run's energies in each save mode."""
+    for bit (both with launch-shape tuning, so both use the tuned shape), its
+    result file holds that run's energies in each save mode, and the wisdom
+    file it writes holds the tuned shape."""
     import itp2d_amd as I
     from itp2d_amd import datafile as D
     print("[gpu] command-line run", flush=True)
     argv = ["-s", "32", "-l", "10", "-n", "6", "-N", "8", "-d", "2", "-e", "0.1", "-p", "harmonic(1.5)",
-            "-T", "relchange(1e-10)", "-F", "relchange(1e-10)", "--maxsteps", "2000", "--rngseed", "11", "-q"]
+            "-T", "relchange(1e-10)", "-F", "relchange(1e-10)", "--maxsteps", "2000", "--rngseed", "11", "-q",
+            "--wisdomfile", str(tmp_path / "wisdom")]
     cfg, opts = cli.parse(argv)
     g = I.Grid(ctx, 32, 32, 10 / 32, 0)
     V = I.Potential.of_type(g, "harmonic", [1.5]).values()
-    s = I.ITPSystem(ctx, dict(cfg, precision=I.FP64), V)
-    assert s.run() == 0 and not s.error_flag
+    ctx.set_autotune(1)
+    try:
+        s = I.ITPSystem(ctx, dict(cfg, precision=I.FP64), V)
+        assert s.run() == 0 and not s.error_flag
+    finally:
+        ctx.set_autotune(0)
     E, sd, _ = s.sorted()
     for mode in ([], ["--save-only-energies"], ["--save-everything"]):
         p = str(tmp_path / f"run{len(mode)}{'e' if '--save-everything' in mode else ''}.h5")
@@ -104,6 +111,7 @@ def test_cli_run_matches_api(ctx, tmp_path):
     assert cli.main(argv + ["-o", p, "-f"]) == 0
     assert cli.main(argv + ["--save-nothing", "-o", str(tmp_path / "none.h5")]) == 0
     assert not (tmp_path / "none.h5").exists()
+    assert any(line.split()[:4] == ["gram", "8", "1024", "64"] for line in (tmp_path / "wisdom").read_text().splitlines())
 
 
 def test_signal_flags_installed_like_the_reference():
