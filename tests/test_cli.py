@@ -114,6 +114,32 @@ def test_cli_run_matches_api(ctx, tmp_path):
     assert any(line.split()[:4] == ["gram", "8", "1024", "64"] for line in (tmp_path / "wisdom").read_text().splitlines())
 
 
+@pytest.mark.gpu
+def test_cli_run_matches_oracle_golden(tmp_path):
+    """The c1 workload (64x64 periodic oscillator on [-8,8]^2, N = 20, 16
+    wanted, 2nd-order split, eps 0.1, relchange(1e-12) tests) through the
+    command line: the result file written by run_and_save holds the oracle
+    golden's step count (energy history rows) and sorted final energies to
+    1e-10 (tests/golden/itp_c1_ho64.npz)."""
+    import os
+    from itp2d_amd import datafile as D
+    from itp2d_amd import workloads as W
+    print("[gpu] command-line run against the c1 golden", flush=True)
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "itp_c1_ho64.npz"))
+    cfg, _ = W.config("c1_ho64")
+    p = str(tmp_path / "c1.h5")
+    argv = ["-s", "64", "-l", "16", "-n", "16", "-N", "20", "-d", "2", "-e", "0.1", "-D", "5", "-p", "harmonic",
+            "-T", "relchange(1e-12)", "-F", "relchange(1e-12)", "--maxsteps", "2000", "--mineps", "1e-9",
+            "--rngseed", str(cfg["seed"]), "-q", "--wisdomfile", str(tmp_path / "wisdom"), "-o", p]
+    assert cli.main(argv) == 0
+    N = cfg["N"]
+    assert D.read_doubles(p, "/energy_history").size == int(ref["steps"]) * N
+    E = D.read_doubles(p, "/final_energies")
+    nw = cfg["n_wanted"]
+    assert np.abs(E[:nw] - ref["E"][:nw]).max() / np.abs(ref["E"][:nw]).max() < 1e-10
+    assert D.read_doubles(p, "/final_energy_standard_deviations").size == N
+
+
 def test_signal_flags_installed_like_the_reference():
     """SIGUSR1 raises the save flag and SIGINT the abort flag, each with the
     reference's note on stderr (itp2d.cpp:40-66); run in a child process so
