@@ -128,6 +128,10 @@ GramWisdom& gram_wisdom() {
   return g;
 }
 
+// pinned bytes per context for downloads: the largest is the energies of
+// 4096 states (E, sigma, <H>, <H^2>: 48 bytes a state)
+constexpr size_t kPinBytes = 256 * 1024;
+
 uint64_t next_object_id() {
   static std::atomic<uint64_t> id{1};
   return id++;
@@ -171,7 +175,50 @@ struct itp_ctx {
   long long tunings = 0;
   long long oa_done = 0, oa_fallback = 0;
   long long cl_calls = 0, cl_clusters = 0, cl_largest = 0, cl_fallback = 0;
+  // pinned host buffer of the small per-call downloads (Downloads)
+  char* pin = nullptr;
+  size_t pin_cap = 0;
 };
+
+namespace {
+
+// The small device-to-host downloads of one call (energies, flags,
+// eigenvalues), staged through the context's pinned buffer: the copies queue
+// asynchronously on the stream and wait() synchronises the stream once, then
+// hands each download to its destination. Copies straight into pageable host
+// memory would each wait for the stream on their own. One Downloads at a time
+// per context (its adds and wait are not interleaved with other calls).
+struct Downloads {
+  struct Item {
+    void* dst;
+    size_t off, bytes;
+  };
+  static constexpr int kMax = 8;
+  itp_ctx* c;
+  Item items[kMax];
+  int n = 0;
+  size_t off = 0;
+  explicit Downloads(itp_ctx* ctx) : c(ctx) {}
+  hipError_t add(void* dst, const void* src, size_t bytes) {
+    const size_t at = (off + 15) / 16 * 16;
+    if (c->pin && n < kMax && at + bytes <= c->pin_cap) {
+      items[n++] = {dst, at, bytes};
+      off = at + bytes;
+      return hipMemcpyAsync(c->pin + at, src, bytes, hipMemcpyDeviceToHost, c->stream);
+    }
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream);
+  }
+  hipError_t wait() {
+    const hipError_t e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess)
+      for (int i = 0; i < n; i++) std::memcpy(items[i].dst, c->pin + items[i].off, items[i].bytes);
+    n = 0;
+    off = 0;
+    return e;
+  }
+};
+
+}  // namespace
 
 struct itp_grid {
   itp_ctx* ctx = nullptr;
@@ -586,11 +633,12 @@ int energies_impl(itp_hamiltonian* h, itp_stateset* s, double* E, double* sigma,
   HIPCK(c, launch_energy_reduce(p1, mean_only ? p1 : p2, nb, (int)s->N, g.dx, s->dE, s->dE + s->N, s->dmean,
                                 s->dmean + s->N, c->stream));
   ph.stop();
-  if (E) HIPCK(c, hipMemcpyAsync(E, s->dE, s->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  if (sigma) HIPCK(c, hipMemcpyAsync(sigma, s->dE + s->N, s->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  if (mean) HIPCK(c, hipMemcpyAsync(mean, s->dmean, s->N * 16, hipMemcpyDeviceToHost, c->stream));
-  if (msq) HIPCK(c, hipMemcpyAsync(msq, s->dmean + s->N, s->N * 16, hipMemcpyDeviceToHost, c->stream));
-  HIPCK(c, hipStreamSynchronize(c->stream));
+  Downloads dl(c);
+  if (E) HIPCK(c, dl.add(E, s->dE, s->N * sizeof(double)));
+  if (sigma) HIPCK(c, dl.add(sigma, s->dE + s->N, s->N * sizeof(double)));
+  if (mean) HIPCK(c, dl.add(mean, s->dmean, s->N * 16));
+  if (msq) HIPCK(c, dl.add(msq, s->dmean + s->N, s->N * 16));
+  HIPCK(c, dl.wait());
   return ITP_OK;
 }
 
@@ -658,11 +706,12 @@ struct EigRead {
 
 int read_eig(itp_stateset* s, EigRead& r, std::vector<double>& lam, bool oa) {
   itp_ctx* c = s->grid->ctx;
-  HIPCK(c, hipMemcpyAsync(r.info, s->info, sizeof(r.info), hipMemcpyDeviceToHost, c->stream));
-  if (oa) HIPCK(c, hipMemcpyAsync(r.state, s->oa_state, sizeof(r.state), hipMemcpyDeviceToHost, c->stream));
-  HIPCK(c, hipMemcpyAsync(&r.bad, s->bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  HIPCK(c, hipMemcpyAsync(lam.data(), s->lam, s->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPCK(c, hipStreamSynchronize(c->stream));
+  Downloads dl(c);
+  HIPCK(c, dl.add(r.info, s->info, sizeof(r.info)));
+  if (oa) HIPCK(c, dl.add(r.state, s->oa_state, sizeof(r.state)));
+  HIPCK(c, dl.add(&r.bad, s->bad, sizeof(int)));
+  HIPCK(c, dl.add(lam.data(), s->lam, s->N * sizeof(double)));
+  HIPCK(c, dl.wait());
   return ITP_OK;
 }
 
@@ -798,10 +847,13 @@ int eigen_stage(itp_stateset* s, size_t* bad_n, double* bad_lambda, bool scale, 
   EigRead rd;
   std::vector<double> lam(N);
   // a non-finite overlap entry (flagged by the Gram kernels) ends here
-  HIPCK(c, hipMemcpyAsync(&rd.bad, s->bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  HIPCK(c, hipMemcpyAsync(s->V[0], s->Vinit, np2, hipMemcpyDeviceToDevice, c->stream));
-  if (hybrid) HIPCK(c, hipMemcpyAsync(s->A[1], s->A[0], np2, hipMemcpyDeviceToDevice, c->stream));
-  HIPCK(c, hipStreamSynchronize(c->stream));
+  {
+    Downloads dl(c);
+    HIPCK(c, dl.add(&rd.bad, s->bad, sizeof(int)));
+    HIPCK(c, hipMemcpyAsync(s->V[0], s->Vinit, np2, hipMemcpyDeviceToDevice, c->stream));
+    if (hybrid) HIPCK(c, hipMemcpyAsync(s->A[1], s->A[0], np2, hipMemcpyDeviceToDevice, c->stream));
+    HIPCK(c, dl.wait());
+  }
   if (rd.bad) {
     c->eig_calls++;
     return set_err(c, ITP_E_EIGENSOLVER, "overlap matrix has non-finite entries");
@@ -850,11 +902,12 @@ int eigen_stage(itp_stateset* s, size_t* bad_n, double* bad_lambda, bool scale, 
       HIPCK(c, launch_block_jacobi(s->A[0], s->V[0], s->bjQ, s->bjD, s->bjident, s->bjslots, s->np, budget,
                                    c->eig_inner, 1e-14, handoff ? c->eig_handoff : 0.0, s->info, c->eig_grid,
                                    c->eig_spin_limit, c->stream));
-      HIPCK(c, hipMemcpyAsync(rd.info, s->info, sizeof(rd.info), hipMemcpyDeviceToHost, c->stream));
+      Downloads dl(c);
+      HIPCK(c, dl.add(rd.info, s->info, sizeof(rd.info)));
       // the grid barrier's error flag (after the per-check maxima and the arrival counter)
-      HIPCK(c, hipMemcpyAsync(&rd.barrier_err, (const char*)(s->bjslots + 2 * (budget + 1)) + sizeof(unsigned),
-                              sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
-      HIPCK(c, hipStreamSynchronize(c->stream));
+      HIPCK(c, dl.add(&rd.barrier_err, (const char*)(s->bjslots + 2 * (budget + 1)) + sizeof(unsigned),
+                      sizeof(unsigned)));
+      HIPCK(c, dl.wait());
     }
     if (rd.barrier_err || rd.info[1] < 0) {
       // a grid barrier gave up waiting: A and V are in an unknown state
@@ -1046,10 +1099,11 @@ int cholqr_impl(itp_stateset* s, int* used_eigen, size_t* bad_n, double* bad_lam
     std::memcpy(&tbits, &tol, sizeof(tbits));
     HIPCK(c, c->graphs ? c->gcache.run(c->stream, {s->id, tbits, 2, 0}, launches) : launches());
     std::vector<double> mm(2 + s->N, 0.0);
-    HIPCK(c, hipMemcpyAsync(&fail[0], s->chol_fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCK(c, hipMemcpyAsync(&fail[1], s->bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCK(c, hipMemcpyAsync(mm.data(), s->chol_mm, mm.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCK(c, hipStreamSynchronize(c->stream));
+    Downloads dl(c);
+    HIPCK(c, dl.add(&fail[0], s->chol_fail, sizeof(int)));
+    HIPCK(c, dl.add(&fail[1], s->bad, sizeof(int)));
+    HIPCK(c, dl.add(mm.data(), s->chol_mm, mm.size() * sizeof(double)));
+    HIPCK(c, dl.wait());
     const bool ok = !(fail[0] || fail[1]) && mm[1] > 0;
     s->chol_kappa = ok ? mm[0] / mm[1] : 0.0;
     if (ok) s->chol_diag.assign(mm.begin() + 2, mm.end());
@@ -1144,9 +1198,10 @@ int rr_impl(itp_hamiltonian* h, itp_stateset* s, double* E, double* sigma) {
   }
   s->cur ^= 1;
   HIPCK(c, launch_residual<R>(T, (const cplx<R>*)s->buf[s->cur], s->lam, g.M, N, g.dx, s->dE + s->N, c->stream));
-  if (E) HIPCK(c, hipMemcpyAsync(E, s->lam, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  if (sigma) HIPCK(c, hipMemcpyAsync(sigma, s->dE + s->N, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPCK(c, hipStreamSynchronize(c->stream));
+  Downloads dl(c);
+  if (E) HIPCK(c, dl.add(E, s->lam, N * sizeof(double)));
+  if (sigma) HIPCK(c, dl.add(sigma, s->dE + s->N, N * sizeof(double)));
+  HIPCK(c, dl.wait());
   return ITP_OK;
 }
 
@@ -1209,6 +1264,14 @@ int itp_ctx_create(int device, itp_ctx** out) {
   c->evpool.resize(2048);
   for (auto& e : c->evpool)
     if (hipEventCreate(&e) != hipSuccess) return ITP_E_HIP;
+  // the downloads' pinned buffer (without it, downloads go straight to the caller's memory)
+  void* pin = nullptr;
+  if (device_op([&] { return hipHostMalloc(&pin, kPinBytes, hipHostMallocDefault); }) == hipSuccess) {
+    c->pin = static_cast<char*>(pin);
+    c->pin_cap = kPinBytes;
+  } else {
+    (void)hipGetLastError();
+  }
   *out = c.release();
   return ITP_OK;
 }
@@ -1221,6 +1284,7 @@ void itp_ctx_destroy(itp_ctx* c) {
   if (c->ws) (void)dev_free(c->ws);
   for (auto& e : c->evpool)
     if (e) (void)hipEventDestroy(e);
+  if (c->pin) (void)device_op([&] { return hipHostFree(c->pin); });
   (void)hipStreamDestroy(c->stream);
   delete c;
 }

@@ -56,17 +56,18 @@ def test_tuned_overlap_matches_untuned(ctx, prec, tol):
 
 def test_wisdom_round_trip_and_import(ctx):
     say("autotune wisdom text")
+    # a shape no workload has (the wisdom is the process's: other tests add theirs)
     with pytest.raises(I.ITPError):
-        ctx.wisdom_import("gram 8 4096 64\n")  # a field short
+        ctx.wisdom_import("gram 8 12345 64\n")  # a field short
     with pytest.raises(I.ITPError):
-        ctx.wisdom_import("plan 8 4096 64 2\n")
+        ctx.wisdom_import("plan 8 12345 64 2\n")
     with pytest.raises(I.ITPError):
-        ctx.wisdom_import("gram 16 4096 64 2\n")  # 4 or 8 bytes per real
+        ctx.wisdom_import("gram 16 12345 64 2\n")  # 4 or 8 bytes per real
     with pytest.raises(I.ITPError):
-        ctx.wisdom_import("gram 8 4096 100 2\n")  # padded state counts are multiples of 64
+        ctx.wisdom_import("gram 8 12345 100 2\n")  # padded state counts are multiples of 64
     with pytest.raises(I.ITPError):
-        ctx.wisdom_import("gram 8 4096 64 2\ngram 8 4096 64 0\n")  # nothing of a bad text is added
-    assert (8, 4096, 64) not in _wisdom(ctx)
+        ctx.wisdom_import("gram 8 12345 64 2\ngram 8 12345 64 0\n")  # nothing of a bad text is added
+    assert (8, 12345, 64) not in _wisdom(ctx)
     # an imported shape is used as is: no measurement, the count it names
     sx, N = 64, 40
     ctx.wisdom_import(f"gram 8 {sx * sx} 64 1\n")
