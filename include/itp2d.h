@@ -60,6 +60,12 @@ typedef struct itp_hamiltonian itp_hamiltonian;
 /* ---- context: one device, one stream (replaces the process-wide OpenMP
  * setup, itpsystem.cpp:51) ---- */
 int itp_ctx_create(int device, itp_ctx** out);
+/* Objects created on a context may be destroyed before or after its handle;
+ * the context itself (stream, cached device memory) goes with the last of
+ * them. A destroyed object's device memory is kept by its context for the
+ * objects created after it (up to 4 GiB per context; reused memory reads as
+ * zero): no hipFree, which waits for every stream of the device, between the
+ * instances a batch worker runs one after another. */
 void itp_ctx_destroy(itp_ctx* ctx);
 const char* itp_last_error(const itp_ctx* ctx);
 int itp_ctx_synchronize(itp_ctx* ctx);

@@ -17,6 +17,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/itp2d.h"
@@ -48,7 +49,6 @@ hipError_t device_op(F&& f) {
 }
 hipError_t dev_malloc(void** p, size_t bytes) { return device_op([&] { return hipMalloc(p, bytes); }); }
 hipError_t dev_free(void* p) { return device_op([&] { return hipFree(p); }); }
-hipError_t dev_sync() { return device_op([&] { return hipDeviceSynchronize(); }); }
 hipError_t dev_memcpy(void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
   return device_op([&] { return hipMemcpy(dst, src, bytes, kind); });
 }
@@ -178,6 +178,20 @@ struct itp_ctx {
   // pinned host buffer of the small per-call downloads (Downloads)
   char* pin = nullptr;
   size_t pin_cap = 0;
+  // Device memory of the context's destroyed objects, kept for the objects
+  // created after them (a batch worker runs instance after instance): hipFree
+  // waits for the whole device, every other context's queued work included.
+  // Blocks by capacity; `sizes` holds every block the context allocated.
+  struct Pool {
+    std::mutex m;
+    std::multimap<size_t, void*> free;
+    std::unordered_map<void*, size_t> sizes;
+    size_t cached = 0;
+  } pool;
+  // the caller's handle plus one per live object created on the context: the
+  // last release frees the stream and the memory (objects may outlive the
+  // handle's itp_ctx_destroy)
+  std::atomic<int> refs{1};
 };
 
 namespace {
@@ -228,6 +242,7 @@ struct itp_grid {
 };
 
 struct itp_potential {
+  itp_ctx* ctx = nullptr;  // held: destroy needs no parent object
   itp_grid* grid = nullptr;
   bool null = true;
   std::vector<double> V;
@@ -236,6 +251,7 @@ struct itp_potential {
 
 struct itp_stateset {
   uint64_t id = next_object_id();  // graph keys
+  itp_ctx* ctx = nullptr;          // held: destroy needs no parent object
   itp_grid* grid = nullptr;
   size_t N = 0;
   int prec = ITP_FP64;
@@ -300,6 +316,7 @@ struct MemberTabs {
 struct itp_split {
   uint64_t id = next_object_id();  // graph keys: id, and gen (bumped when the tables are rebuilt)
   uint64_t gen = 0;
+  itp_ctx* ctx = nullptr;  // held: destroy needs no parent object
   itp_potential* pot = nullptr;
   int halforder = 1;
   double B = 0, eps = 0;
@@ -308,6 +325,7 @@ struct itp_split {
 };
 
 struct itp_hamiltonian {
+  itp_ctx* ctx = nullptr;  // held: destroy needs no parent object
   itp_potential* pot = nullptr;
   double B = 0;
   KinTabs host;
@@ -370,22 +388,86 @@ int guarded(itp_ctx* c, F f) {
   }
 }
 
+// Cached bytes per context at most; a cached block serves requests of at
+// least half its capacity (or all but 64 KiB of it).
+constexpr size_t kPoolCap = 4ull << 30;
+
 int dalloc(itp_ctx* c, void** p, size_t bytes) {
   if (bytes == 0) bytes = 16;
+  void* hit = nullptr;
+  size_t cap = 0;
+  {
+    std::lock_guard<std::mutex> lk(c->pool.m);
+    auto it = c->pool.free.lower_bound(bytes);
+    if (it != c->pool.free.end() && it->first <= std::max(2 * bytes, bytes + (64 << 10))) {
+      hit = it->second;
+      cap = it->first;
+      c->pool.cached -= cap;
+      c->pool.free.erase(it);
+    }
+  }
+  if (hit) {
+    // a reused block reads as zero, as fresh memory from the driver does; the
+    // wait orders the fill before host-synchronous copies into the block
+    hipError_t e = hipMemsetAsync(hit, 0, cap, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+      std::lock_guard<std::mutex> lk(c->pool.m);
+      c->pool.free.emplace(cap, hit);
+      c->pool.cached += cap;
+      *p = nullptr;
+      return set_err(c, ITP_E_HIP, std::string("clearing a reused block: ") + hipGetErrorString(e));
+    }
+    *p = hit;
+    return ITP_OK;
+  }
   hipError_t e = dev_malloc(p, bytes);
   if (e != hipSuccess) {
     *p = nullptr;
     (void)hipGetLastError();
     return set_err(c, ITP_E_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
   }
+  std::lock_guard<std::mutex> lk(c->pool.m);
+  c->pool.sizes[*p] = bytes;
   return ITP_OK;
+}
+
+// Returns a block of dalloc to the context's cache (after the context's
+// stream has finished every use of it), or frees it when the cache is full.
+void dfree(itp_ctx* c, void* p) {
+  if (!p) return;
+  (void)hipStreamSynchronize(c->stream);
+  {
+    std::lock_guard<std::mutex> lk(c->pool.m);
+    auto it = c->pool.sizes.find(p);
+    if (it != c->pool.sizes.end() && c->pool.cached + it->second <= kPoolCap) {
+      c->pool.free.emplace(it->second, p);
+      c->pool.cached += it->second;
+      return;
+    }
+    if (it != c->pool.sizes.end()) c->pool.sizes.erase(it);
+  }
+  (void)dev_free(p);
+}
+
+// Drops one reference of the context; the last one frees it.
+void ctx_release(itp_ctx* c) {
+  if (--c->refs > 0) return;
+  (void)hipStreamSynchronize(c->stream);
+  if (c->ws) (void)dev_free(c->ws);
+  for (auto& kv : c->pool.free) (void)dev_free(kv.second);
+  for (auto& e : c->evpool)
+    if (e) (void)hipEventDestroy(e);
+  if (c->pin) (void)device_op([&] { return hipHostFree(c->pin); });
+  (void)hipStreamDestroy(c->stream);
+  delete c;
 }
 
 int ensure_ws(itp_ctx* c, size_t bytes) {
   if (bytes <= c->ws_bytes) return ITP_OK;
   if (c->ws) {
     HIPCK(c, hipStreamSynchronize(c->stream));
-    HIPCK(c, dev_free(c->ws));
+    dfree(c, c->ws);
     c->ws = nullptr;
     c->ws_bytes = 0;
   }
@@ -497,9 +579,9 @@ int upload_tables32(itp_ctx* c, std::initializer_list<const std::vector<double>*
   return ITP_OK;
 }
 
-void free_member(MemberTabs* m) {
-  if (m && m->dev) { (void)dev_free(m->dev); m->dev = nullptr; }
-  if (m && m->dev32) { (void)dev_free(m->dev32); m->dev32 = nullptr; }
+void free_member(itp_ctx* c, MemberTabs* m) {
+  if (m && m->dev) { dfree(c, m->dev); m->dev = nullptr; }
+  if (m && m->dev32) { dfree(c, m->dev32); m->dev32 = nullptr; }
 }
 
 // FP32 copies of a member's tables (for FP32 sweep programs)
@@ -525,7 +607,7 @@ int build_split_tables(itp_split* sp) {
   const GridInfo& g = sp->pot->grid->g;
   const bool has_v = !sp->pot->null;
   HIPCK(c, hipStreamSynchronize(c->stream));
-  for (auto& m : sp->members) free_member(m.get());
+  for (auto& m : sp->members) free_member(c, m.get());
   sp->members.clear();
   sp->gen = next_object_id();
   for (int t = 1; t <= sp->halforder; t++) {
@@ -1218,9 +1300,18 @@ struct DevFields {
   cplx<double>* p = nullptr;
   cplx<double>* w = nullptr;
   ~DevFields() {
-    if (p) (void)dev_free(p);
-    if (w) (void)dev_free(w);
+    if (c) {
+      dfree(c, p);
+      dfree(c, w);
+    }
   }
+};
+
+// a temporary block of dalloc, returned when the scope ends
+struct DevHold {
+  itp_ctx* c;
+  void* p;
+  ~DevHold() { dfree(c, p); }
 };
 
 int hook_begin(itp_ctx* c, const double* host, size_t nelem, DevFields& df, bool need_work) {
@@ -1280,13 +1371,11 @@ void itp_ctx_destroy(itp_ctx* c) {
   if (!c) return;
   (void)hipStreamSynchronize(c->stream);
   c->gcache.clear();
-  if (c->graphs) graph_contexts()--;
-  if (c->ws) (void)dev_free(c->ws);
-  for (auto& e : c->evpool)
-    if (e) (void)hipEventDestroy(e);
-  if (c->pin) (void)device_op([&] { return hipHostFree(c->pin); });
-  (void)hipStreamDestroy(c->stream);
-  delete c;
+  if (c->graphs) {
+    c->graphs = false;
+    graph_contexts()--;
+  }
+  ctx_release(c);
 }
 
 const char* itp_last_error(const itp_ctx* c) { return c ? c->err.c_str() : "no context"; }
@@ -1486,6 +1575,7 @@ int itp_grid_create(itp_ctx* c, size_t sx, size_t sy, double dx, int boundary, i
     HIPCK(c, dev_memcpy(gr->d_tw, all.data(), all.size() * 16, hipMemcpyHostToDevice));
     gr->lg[0] = make_geom(gr->g, gr->d_tw, 16);
     gr->lg[1] = make_geom(gr->g, gr->d_tw, 8);
+    c->refs++;
     *out = gr.release();
     return ITP_OK;
   });
@@ -1493,9 +1583,10 @@ int itp_grid_create(itp_ctx* c, size_t sx, size_t sy, double dx, int boundary, i
 
 void itp_grid_destroy(itp_grid* g) {
   if (!g) return;
-  (void)dev_sync();  // no parent access: parents may already be gone
-  if (g->d_tw) (void)dev_free(g->d_tw);
+  itp_ctx* c = g->ctx;
+  if (g->d_tw) dfree(c, g->d_tw);
   delete g;
+  ctx_release(c);
 }
 
 int itp_grid_kvalues(const itp_grid* g, double* fkx, double* fky, double* dkx, double* dky) {
@@ -1567,6 +1658,8 @@ int itp_potential_create(itp_grid* g, const double* V, itp_potential** out) {
       if (rc) return rc;
       HIPCK(c, dev_memcpy(p->dV, V, g->g.M * sizeof(double), hipMemcpyHostToDevice));
     }
+    p->ctx = c;
+    c->refs++;
     *out = p.release();
     return ITP_OK;
   });
@@ -1598,6 +1691,8 @@ int itp_potential_create_type(itp_grid* g, const char* name, const double* param
     HIPCK(c, launch_potential(q, g->g.sx, g->g.sy, g->g.dx, p->dV, c->stream));
     HIPCK(c, hipMemcpyAsync(p->V.data(), p->dV, g->g.M * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCK(c, hipStreamSynchronize(c->stream));
+    p->ctx = c;
+    c->refs++;
     *out = p.release();
     return ITP_OK;
   });
@@ -1647,7 +1742,7 @@ int itp_potential_add_impurities(itp_potential* p, int kind, const double* data,
     if (e == hipSuccess)
       e = hipMemcpyAsync(p->V.data(), p->dV, gi.M * sizeof(double), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    (void)dev_free(dd);
+    dfree(c, dd);
     if (e != hipSuccess) return set_err(c, ITP_E_HIP, std::string("impurities: ") + hipGetErrorString(e));
     if (count) p->null = false;
     return ITP_OK;
@@ -1663,9 +1758,10 @@ int itp_potential_values(const itp_potential* p, double* V) {
 
 void itp_potential_destroy(itp_potential* p) {
   if (!p) return;
-  (void)dev_sync();  // no parent access: parents may already be gone
-  if (p->dV) (void)dev_free(p->dV);
+  itp_ctx* c = p->ctx;  // parents may already be gone
+  if (p->dV) dfree(c, p->dV);
   delete p;
+  ctx_release(c);
 }
 
 int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out) {
@@ -1760,6 +1856,8 @@ int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out
     if ((rc = dalloc(c, (void**)&s->epart, 2 * pstride * 16))) return rc;
     if ((rc = dalloc(c, (void**)&s->dE, 2 * N * sizeof(double)))) return rc;
     if ((rc = dalloc(c, (void**)&s->dmean, 2 * N * 16))) return rc;
+    s->ctx = c;
+    c->refs++;
     *out = s.release();
     return ITP_OK;
   });
@@ -1767,15 +1865,15 @@ int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out
 
 void itp_stateset_destroy(itp_stateset* s) {
   if (!s) return;
-  (void)dev_sync();  // no parent access: parents may already be gone
+  itp_ctx* c = s->ctx;  // parents may already be gone
   void* ptrs[] = {s->buf[0], s->buf[1], s->part, s->A[0], s->A[1], s->V[0], s->V[1], s->Vinit, s->bjQ, s->bjD,
                   s->bjslots, s->bjident, s->oa_work, s->oa_lam, s->oa_slots, s->oa_state, s->cl_hdr, s->cl_pairs,
                   s->cl_members, s->cl_offsets, s->Vsave, s->oa_part, s->C, s->C32,
                   s->lam, s->perm, s->info, s->bad, s->epart, s->dE, s->dmean, s->diag0, s->chol_fail, s->chol_mm, s->norms,
                   s->td_scratch};
-  for (void* p : ptrs)
-    if (p) (void)dev_free(p);
+  for (void* p : ptrs) dfree(c, p);
   delete s;
+  ctx_release(c);
 }
 
 int itp_stateset_upload(itp_stateset* s, const double* host) {
@@ -1828,9 +1926,11 @@ int itp_split_create(itp_potential* pot, int halforder, double eps, double B, it
     sp->coefs = mps_coefficients(sp->halforder);
     int rc = build_split_tables(sp.get());
     if (rc) {
-      for (auto& m : sp->members) free_member(m.get());
+      for (auto& m : sp->members) free_member(c, m.get());
       return rc;
     }
+    sp->ctx = c;
+    c->refs++;
     *out = sp.release();
     return ITP_OK;
   });
@@ -1838,9 +1938,10 @@ int itp_split_create(itp_potential* pot, int halforder, double eps, double B, it
 
 void itp_split_destroy(itp_split* sp) {
   if (!sp) return;
-  (void)dev_sync();  // no parent access: parents may already be gone
-  for (auto& m : sp->members) free_member(m.get());
+  itp_ctx* c = sp->ctx;  // parents may already be gone
+  for (auto& m : sp->members) free_member(c, m.get());
   delete sp;
+  ctx_release(c);
 }
 
 int itp_split_set_time_step(itp_split* sp, double eps) {
@@ -2000,6 +2101,8 @@ int itp_hamiltonian_create(itp_potential* pot, double B, itp_hamiltonian** out) 
     int rc = upload_tables(c, {&h->host.mxy, &h->host.mx, &h->host.mx2, &h->host.my}, &h->dev, p);
     if (rc) return rc;
     h->d = KinHDev{p[0], p[1], p[2], p[3]};
+    h->ctx = c;
+    c->refs++;
     *out = h.release();
     return ITP_OK;
   });
@@ -2007,9 +2110,10 @@ int itp_hamiltonian_create(itp_potential* pot, double B, itp_hamiltonian** out) 
 
 void itp_hamiltonian_destroy(itp_hamiltonian* h) {
   if (!h) return;
-  (void)dev_sync();  // no parent access: parents may already be gone
-  if (h->dev) (void)dev_free(h->dev);
+  itp_ctx* c = h->ctx;  // parents may already be gone
+  if (h->dev) dfree(c, h->dev);
   delete h;
+  ctx_release(c);
 }
 
 int itp_energies(itp_hamiltonian* h, itp_stateset* s, double* E, double* sigma) {
@@ -2041,7 +2145,7 @@ int itp_apply_expkinetic_host(itp_grid* g, double B, double e, double coeff, dou
     std::vector<const double*> p;
     int rc = upload_tables(c, {&t.mult, &t.xm, &t.xm2, &t.ym}, &dev, p);
     if (rc) return rc;
-    std::unique_ptr<double, void (*)(double*)> hold(dev, [](double* q) { (void)dev_free(q); });
+    DevHold hold{c, dev};
     DevFields df;
     const size_t nel = n * (size_t)g->g.M;
     if ((rc = hook_begin(c, fields, nel, df, true))) return rc;
@@ -2066,7 +2170,7 @@ int itp_apply_exppotential_host(itp_potential* pot, double e, double coeff, doub
     if (!pot->null) exppot_values(g->g, pot->V.data(), e, coeff, prefactor, vals);
     int rc = upload_tables(c, {&vals}, &dev, p);
     if (rc) return rc;
-    std::unique_ptr<double, void (*)(double*)> hold(dev, [](double* q) { (void)dev_free(q); });
+    DevHold hold{c, dev};
     DevFields df;
     const size_t nel = n * (size_t)g->g.M;
     if ((rc = hook_begin(c, fields, nel, df, false))) return rc;
@@ -2098,7 +2202,7 @@ int itp_apply_secondorder_host(itp_potential* pot, double B, double e, double pr
     std::vector<const double*> p;
     int rc = upload_tables(c, {&m.host.mult, &m.host.xm, &m.host.xm2, &m.host.ym, &m.vh, &m.vf, &m.vp}, &m.dev, p);
     if (rc) return rc;
-    std::unique_ptr<double, void (*)(double*)> hold(m.dev, [](double* q) { (void)dev_free(q); });
+    DevHold hold{c, m.dev};
     m.d.kin = KinDev{p[0], p[1], p[2], p[3]};
     m.d.vhalf = p[4];
     m.d.vfull = p[5];
@@ -2108,7 +2212,7 @@ int itp_apply_secondorder_host(itp_potential* pot, double B, double e, double pr
     if ((rc = hook_begin(c, fields, nel, df, true))) return rc;
     cplx<double>* out = nullptr;
     if ((rc = dalloc(c, (void**)&out, nel * 16))) return rc;
-    std::unique_ptr<cplx<double>, void (*)(cplx<double>*)> hold2(out, [](cplx<double>* q) { (void)dev_free(q); });
+    DevHold hold2{c, out};
     std::vector<Sweep<double>> prog;
     member_program<double>(branch_of(B, g->g.bt), m.d, has_v ? exponent : 1, has_v, df.p, df.w, out, false, prog);
     GpuBackend be;
@@ -2131,7 +2235,7 @@ int itp_apply_hamiltonian_host(itp_hamiltonian* h, double* fields, size_t n) {
     if (rc) return rc;
     cplx<double>* out = nullptr;
     if ((rc = dalloc(c, (void**)&out, nel * 16))) return rc;
-    std::unique_ptr<cplx<double>, void (*)(cplx<double>*)> hold(out, [](cplx<double>* q) { (void)dev_free(q); });
+    DevHold hold{c, out};
     std::vector<Sweep<double>> prog;
     hamiltonian_program<double>(g->g, g->lg[0], branch_of(h->B, g->g.bt), h->d, h->pot->null ? nullptr : h->pot->dV,
                                 df.p, df.w, out, nullptr, nullptr, prog);

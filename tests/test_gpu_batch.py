@@ -72,3 +72,39 @@ def test_batch_c4_subset():
         assert r["converged"] >= cfg["n_wanted"], (i, r)
         E, sd, f = b.sorted(i)
         assert np.all(np.diff(E) >= 0)
+
+
+def test_destroyed_objects_memory_is_reused_and_cleared():
+    """A destroyed set's device memory serves the next set of its context
+    (no hipFree, which would wait for every stream of the device): the new
+    set gets the old field buffer and reads as zeros."""
+    say("device memory reuse")
+    c = I.Context(0)
+    try:
+        g = I.Grid(c, 64, 64, 0.2)
+        a = I.StateSet(g, 40)
+        a.upload(I.init_states(W.SEED_BASE, 40, 64, 64, 0.2))
+        pa = I.lib().itp_stateset_device_ptr(a.h)
+        a.close()
+        b = I.StateSet(g, 40)
+        assert I.lib().itp_stateset_device_ptr(b.h) == pa
+        assert not np.any(b.download())
+        b.close()
+    finally:
+        c.close()
+
+
+def test_context_released_by_its_last_object():
+    """Objects may be destroyed after their context's handle (the reference's
+    objects have no such order): the context, its stream and its cached
+    memory go with the last object."""
+    import ctypes as C
+    say("context outlived by its objects")
+    L = I.lib()
+    ctx, g, s = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    assert L.itp_ctx_create(0, C.byref(ctx)) == 0
+    assert L.itp_grid_create(ctx, 32, 32, C.c_double(0.3), I.PERIODIC, C.byref(g)) == 0
+    assert L.itp_stateset_create(g, 8, I.FP64, C.byref(s)) == 0
+    L.itp_ctx_destroy(ctx)
+    L.itp_grid_destroy(g)
+    L.itp_stateset_destroy(s)
