@@ -52,7 +52,6 @@ hipError_t dev_free(void* p) { return device_op([&] { return hipFree(p); }); }
 hipError_t dev_memcpy(void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
   return device_op([&] { return hipMemcpy(dst, src, bytes, kind); });
 }
-hipError_t dev_memset(void* dst, int v, size_t bytes) { return device_op([&] { return hipMemset(dst, v, bytes); }); }
 
 // HIP graphs of fixed launch sequences on the context's stream, keyed by what
 // determines their launches and arguments. The first call with a key runs the
@@ -188,6 +187,9 @@ struct itp_ctx {
     std::unordered_map<void*, size_t> sizes;
     size_t cached = 0;
   } pool;
+  // fills and clears queued on the stream that host-synchronous copies must
+  // not overtake (cmemcpy waits for them)
+  bool fills_pending = false;
   // the caller's handle plus one per live object created on the context: the
   // last release frees the stream and the memory (objects may outlive the
   // handle's itp_ctx_destroy)
@@ -407,10 +409,10 @@ int dalloc(itp_ctx* c, void** p, size_t bytes) {
     }
   }
   if (hit) {
-    // a reused block reads as zero, as fresh memory from the driver does; the
-    // wait orders the fill before host-synchronous copies into the block
-    hipError_t e = hipMemsetAsync(hit, 0, cap, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    // a reused block reads as zero, as fresh memory from the driver does (the
+    // fill is ordered before host-synchronous copies by cmemcpy)
+    const hipError_t e = hipMemsetAsync(hit, 0, cap, c->stream);
+    c->fills_pending = true;
     if (e != hipSuccess) {
       std::lock_guard<std::mutex> lk(c->pool.m);
       c->pool.free.emplace(cap, hit);
@@ -430,6 +432,23 @@ int dalloc(itp_ctx* c, void** p, size_t bytes) {
   std::lock_guard<std::mutex> lk(c->pool.m);
   c->pool.sizes[*p] = bytes;
   return ITP_OK;
+}
+
+// Host-synchronous copy into or out of a context's memory, after the fills
+// and clears queued on its stream.
+hipError_t cmemcpy(itp_ctx* c, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  if (c->fills_pending) {
+    const hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return e;
+    c->fills_pending = false;
+  }
+  return dev_memcpy(dst, src, bytes, kind);
+}
+
+// Clear of a context's memory, queued on its stream.
+hipError_t cmemset(itp_ctx* c, void* dst, int v, size_t bytes) {
+  c->fills_pending = true;
+  return hipMemsetAsync(dst, v, bytes, c->stream);
 }
 
 // Returns a block of dalloc to the context's cache (after the context's
@@ -554,7 +573,7 @@ int upload_tables(itp_ctx* c, const std::vector<const std::vector<double>*>& tab
   ptrs.clear();
   for (auto t : tabs) {
     ptrs.push_back(t->empty() ? nullptr : *dev + off);
-    if (!t->empty()) HIPCK(c, dev_memcpy(*dev + off, t->data(), t->size() * sizeof(double), hipMemcpyHostToDevice));
+    if (!t->empty()) HIPCK(c, cmemcpy(c, *dev + off, t->data(), t->size() * sizeof(double), hipMemcpyHostToDevice));
     off += t->size();
   }
   return ITP_OK;
@@ -575,7 +594,7 @@ int upload_tables32(itp_ctx* c, std::initializer_list<const std::vector<double>*
     for (size_t i = 0; i < t->size(); i++) h[off + i] = float((*t)[i]);
     off += t->size();
   }
-  if (total) HIPCK(c, dev_memcpy(*dev, h.data(), total * sizeof(float), hipMemcpyHostToDevice));
+  if (total) HIPCK(c, cmemcpy(c, *dev, h.data(), total * sizeof(float), hipMemcpyHostToDevice));
   return ITP_OK;
 }
 
@@ -1322,14 +1341,14 @@ int hook_begin(itp_ctx* c, const double* host, size_t nelem, DevFields& df, bool
     rc = dalloc(c, (void**)&df.w, nelem * 16);
     if (rc) return rc;
   }
-  HIPCK(c, dev_memcpy(df.p, host, nelem * 16, hipMemcpyHostToDevice));
+  HIPCK(c, cmemcpy(c, df.p, host, nelem * 16, hipMemcpyHostToDevice));
   return ITP_OK;
 }
 
 int hook_end(itp_ctx* c, double* host, size_t nelem, DevFields& df, hipError_t err) {
   if (err != hipSuccess) return set_err(c, ITP_E_HIP, hipGetErrorString(err));
   HIPCK(c, hipStreamSynchronize(c->stream));
-  HIPCK(c, dev_memcpy(host, df.p, nelem * 16, hipMemcpyDeviceToHost));
+  HIPCK(c, cmemcpy(c, host, df.p, nelem * 16, hipMemcpyDeviceToHost));
   return ITP_OK;
 }
 
@@ -1572,7 +1591,7 @@ int itp_grid_create(itp_ctx* c, size_t sx, size_t sy, double dx, int boundary, i
     all.insert(all.end(), mky.begin(), mky.end());
     int rc = dalloc(c, (void**)&gr->d_tw, all.size() * 16);
     if (rc) return rc;
-    HIPCK(c, dev_memcpy(gr->d_tw, all.data(), all.size() * 16, hipMemcpyHostToDevice));
+    HIPCK(c, cmemcpy(c, gr->d_tw, all.data(), all.size() * 16, hipMemcpyHostToDevice));
     gr->lg[0] = make_geom(gr->g, gr->d_tw, 16);
     gr->lg[1] = make_geom(gr->g, gr->d_tw, 8);
     c->refs++;
@@ -1656,7 +1675,7 @@ int itp_potential_create(itp_grid* g, const double* V, itp_potential** out) {
       p->V.assign(V, V + g->g.M);
       int rc = dalloc(c, (void**)&p->dV, g->g.M * sizeof(double));
       if (rc) return rc;
-      HIPCK(c, dev_memcpy(p->dV, V, g->g.M * sizeof(double), hipMemcpyHostToDevice));
+      HIPCK(c, cmemcpy(c, p->dV, V, g->g.M * sizeof(double), hipMemcpyHostToDevice));
     }
     p->ctx = c;
     c->refs++;
@@ -1710,7 +1729,7 @@ int itp_potential_add_impurities(itp_potential* p, int kind, const double* data,
       int rc = dalloc(c, (void**)&p->dV, gi.M * sizeof(double));
       if (rc) return rc;
       p->V.assign(gi.M, 0.0);
-      HIPCK(c, dev_memcpy(p->dV, p->V.data(), gi.M * sizeof(double), hipMemcpyHostToDevice));
+      HIPCK(c, cmemcpy(c, p->dV, p->V.data(), gi.M * sizeof(double), hipMemcpyHostToDevice));
     }
     if (kind == kImpDelta) {
       // one grid point each (DeltaImpurities::add_noise, noise.cpp:80-88): on the host copy
@@ -1725,7 +1744,7 @@ int itp_potential_add_impurities(itp_potential* p, int kind, const double* data,
         at[n] = y * (long long)gi.sx + x;
       }
       for (size_t n = 0; n < count; n++) p->V[at[n]] += data[3 * n + 2];
-      HIPCK(c, dev_memcpy(p->dV, p->V.data(), gi.M * sizeof(double), hipMemcpyHostToDevice));
+      HIPCK(c, cmemcpy(c, p->dV, p->V.data(), gi.M * sizeof(double), hipMemcpyHostToDevice));
       if (count) p->null = false;
       return ITP_OK;
     }
@@ -1736,7 +1755,7 @@ int itp_potential_add_impurities(itp_potential* p, int kind, const double* data,
     double* dd = nullptr;
     int rc = dalloc(c, (void**)&dd, std::max<size_t>(1, count) * stride * sizeof(double));
     if (rc) return rc;
-    hipError_t e = dev_memcpy(dd, data, count * stride * sizeof(double), hipMemcpyHostToDevice);
+    hipError_t e = cmemcpy(c, dd, data, count * stride * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = launch_impurities(kind, dd, stride, (long long)count, g0, g1, gi.sx, gi.sy, gi.dx, p->dV,
                                                c->stream);
     if (e == hipSuccess)
@@ -1781,7 +1800,7 @@ int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out
     int rc;
     for (int b = 0; b < 2; b++) {
       if ((rc = dalloc(c, &s->buf[b], N * M * s->esz))) return rc;
-      HIPCK(c, dev_memset(s->buf[b], 0, N * M * s->esz));
+      HIPCK(c, cmemset(c, s->buf[b], 0, N * M * s->esz));
     }
     s->nc = (int)((N + 63) / 64 * 64);
     s->np = s->nc;  // the eigensolver works on the zero-padded nc x nc matrix
@@ -1816,7 +1835,7 @@ int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out
     for (int b = 0; b < 2; b++) {
       if ((rc = dalloc(c, (void**)&s->A[b], np2))) return rc;
       if ((rc = dalloc(c, (void**)&s->V[b], np2))) return rc;
-      HIPCK(c, dev_memset(s->A[b], 0, np2));
+      HIPCK(c, cmemset(c, s->A[b], 0, np2));
     }
     if ((rc = dalloc(c, (void**)&s->Vinit, np2))) return rc;
     {
@@ -1839,11 +1858,11 @@ int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out
     {
       std::vector<cplx<double>> I((size_t)s->np * s->np, mk<double>(0.0, 0.0));
       for (int i = 0; i < s->np; i++) I[(size_t)i * s->np + i] = mk<double>(1.0, 0.0);
-      HIPCK(c, dev_memcpy(s->Vinit, I.data(), np2, hipMemcpyHostToDevice));
+      HIPCK(c, cmemcpy(c, s->Vinit, I.data(), np2, hipMemcpyHostToDevice));
     }
     if ((rc = dalloc(c, (void**)&s->C, (size_t)s->nc * s->nc * 16))) return rc;
     if (precision == ITP_FP32 && (rc = dalloc(c, (void**)&s->C32, (size_t)s->nc * s->nc * 8))) return rc;
-    HIPCK(c, dev_memset(s->C, 0, (size_t)s->nc * s->nc * 16));
+    HIPCK(c, cmemset(c, s->C, 0, (size_t)s->nc * s->nc * 16));
     if ((rc = dalloc(c, (void**)&s->lam, N * sizeof(double)))) return rc;
     if ((rc = dalloc(c, (void**)&s->diag0, (size_t)s->np * sizeof(double)))) return rc;
     if ((rc = dalloc(c, (void**)&s->chol_fail, sizeof(int)))) return rc;
@@ -2016,7 +2035,7 @@ int itp_rayleigh_quotients(itp_hamiltonian* h, const itp_stateset* src, itp_stat
     int rc = energies_impl<double>(h, dst, E, nullptr, nullptr, nullptr, true);
     if (rc) return rc;
     std::vector<double> nrm(src->N);
-    HIPCK(c, dev_memcpy(nrm.data(), dst->norms, src->N * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCK(c, cmemcpy(c, nrm.data(), dst->norms, src->N * sizeof(double), hipMemcpyDeviceToHost));
     const double dx2 = g.dx * g.dx;
     for (size_t k = 0; k < src->N; k++) E[k] /= nrm[k] * dx2;
     return ITP_OK;
@@ -2265,13 +2284,13 @@ int itp_eigh_host(itp_ctx* c, size_t n, double* A, double* w) {
       for (size_t j = 0; j < n; j++)
         full[i * s->np + j] = (i <= j) ? a[j * n + i] : conj(a[i * n + j]);  // upper triangle, column-major
     for (size_t i = 0; i < n; i++) full[i * s->np + i].y = 0;
-    HIPCK(c, dev_memcpy(s->A[0], full.data(), full.size() * 16, hipMemcpyHostToDevice));
-    HIPCK(c, dev_memset(s->bad, 0, sizeof(int)));
+    HIPCK(c, cmemcpy(c, s->A[0], full.data(), full.size() * 16, hipMemcpyHostToDevice));
+    HIPCK(c, cmemset(c, s->bad, 0, sizeof(int)));
     rc = eigen_stage(s, nullptr, nullptr, false);
     if (rc && rc != ITP_E_NONPOSITIVE_EIG && rc != ITP_E_NONNORMAL_EIG) return rc;
     std::vector<cplx<double>> C((size_t)s->nc * s->nc);
-    HIPCK(c, dev_memcpy(C.data(), s->C, C.size() * 16, hipMemcpyDeviceToHost));
-    HIPCK(c, dev_memcpy(w, s->lam, n * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCK(c, cmemcpy(c, C.data(), s->C, C.size() * 16, hipMemcpyDeviceToHost));
+    HIPCK(c, cmemcpy(c, w, s->lam, n * sizeof(double), hipMemcpyDeviceToHost));
     cplx<double>* o = (cplx<double>*)A;
     for (size_t r = 0; r < n; r++)
       for (size_t cc = 0; cc < n; cc++) o[r * n + cc] = C[cc * s->nc + r];  // column r = eigenvector r
@@ -2297,7 +2316,7 @@ int itp_eigvalsh_host(itp_ctx* c, size_t n, const double* A, double* w) {
       for (size_t j = 0; j < n; j++)
         full[i * s->np + j] = (i <= j) ? a[j * n + i] : conj(a[i * n + j]);  // upper triangle, column-major
     for (size_t i = 0; i < n; i++) full[i * s->np + i].y = 0;
-    HIPCK(c, dev_memcpy(s->A[0], full.data(), full.size() * 16, hipMemcpyHostToDevice));
+    HIPCK(c, cmemcpy(c, s->A[0], full.data(), full.size() * 16, hipMemcpyHostToDevice));
     std::vector<double> ev;
     if ((rc = hermitian_eigenvalues(s, s->A[0], ev))) return rc;
     std::copy(ev.begin(), ev.end(), w);
