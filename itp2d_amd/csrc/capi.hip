@@ -131,6 +131,41 @@ GramWisdom& gram_wisdom() {
 // 4096 states (E, sigma, <H>, <H^2>: 48 bytes a state)
 constexpr size_t kPinBytes = 256 * 1024;
 
+// Pinned download buffers outlive their contexts: a destroyed context's
+// buffer goes back to this list for the next context instead of being
+// unpinned (hipHostFree) while other threads' contexts keep working; they are
+// released when the process ends. At most one buffer per context alive at once;
+// portable, so a buffer serves contexts on any device.
+struct PinnedBuffers {
+  std::mutex m;
+  std::vector<void*> free;
+};
+PinnedBuffers& pinned_buffers() {
+  static PinnedBuffers* p = new PinnedBuffers;  // never destroyed: contexts may outlive static destruction
+  return *p;
+}
+void* take_pinned() {
+  {
+    std::lock_guard<std::mutex> lk(pinned_buffers().m);
+    if (!pinned_buffers().free.empty()) {
+      void* p = pinned_buffers().free.back();
+      pinned_buffers().free.pop_back();
+      return p;
+    }
+  }
+  void* p = nullptr;
+  if (device_op([&] { return hipHostMalloc(&p, kPinBytes, hipHostMallocPortable); }) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  return p;
+}
+void give_pinned(void* p) {
+  if (!p) return;
+  std::lock_guard<std::mutex> lk(pinned_buffers().m);
+  pinned_buffers().free.push_back(p);
+}
+
 uint64_t next_object_id() {
   static std::atomic<uint64_t> id{1};
   return id++;
@@ -477,7 +512,7 @@ void ctx_release(itp_ctx* c) {
   for (auto& kv : c->pool.free) (void)dev_free(kv.second);
   for (auto& e : c->evpool)
     if (e) (void)hipEventDestroy(e);
-  if (c->pin) (void)device_op([&] { return hipHostFree(c->pin); });
+  give_pinned(c->pin);
   (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1375,12 +1410,9 @@ int itp_ctx_create(int device, itp_ctx** out) {
   for (auto& e : c->evpool)
     if (hipEventCreate(&e) != hipSuccess) return ITP_E_HIP;
   // the downloads' pinned buffer (without it, downloads go straight to the caller's memory)
-  void* pin = nullptr;
-  if (device_op([&] { return hipHostMalloc(&pin, kPinBytes, hipHostMallocDefault); }) == hipSuccess) {
+  if (void* pin = take_pinned()) {
     c->pin = static_cast<char*>(pin);
     c->pin_cap = kPinBytes;
-  } else {
-    (void)hipGetLastError();
   }
   *out = c.release();
   return ITP_OK;

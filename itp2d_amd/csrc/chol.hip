@@ -91,14 +91,16 @@ __device__ __forceinline__ void acc_store(const d4& cr, const d4& ci, double s, 
 // original diagonal.
 __device__ void factor_diag(cplx<double>* a, cplx<double>* lin, cplx<double>* inv, const CholArgs& p, int k) {
   __shared__ double piv[T];
+  __shared__ double ref0[T];  // the rows' original diagonal (read once, not per column)
   __shared__ int bad;
   const int tid = threadIdx.x;
   if (tid == 0) bad = 0;
+  if (tid < T) ref0[tid] = p.diag0[k * T + tid];
   for (int e = tid; e < T2; e += blockDim.x) lin[e] = mk<double>((e / T) == (e % T) ? 1.0 : 0.0, 0.0);
   __syncthreads();
   for (int c = 0; c < T; c++) {
     const double d = a[c * T + c].x;
-    const double ref = p.diag0[k * T + c];
+    const double ref = ref0[c];
     const bool ok = std::isfinite(d) && d > p.tol * ref && d > 0;
     if (tid == 0) {
       piv[c] = ok ? std::sqrt(d) : 1.0;
