@@ -45,8 +45,19 @@ template <class R, int N, int ORIENT, bool XS, class PG>
 __global__ __launch_bounds__(256, 2) void reg_sweep_kernel(SweepArgs<R> a) {
   extern __shared__ __align__(16) char lds_raw[];
   GpuRegExec<RegState<R, RegShape<N>::Q>> ex;
-  reg_sweep_program<R, N, ORIENT, XS, PG>(ex, a, reinterpret_cast<cplx<R>*>(lds_raw), (int)blockIdx.y,
-                                           (int)blockIdx.x);
+  int field = (int)blockIdx.y, blk = (int)blockIdx.x;
+  if constexpr (ORIENT == 1 && reg_lines_per_wg(N) * sizeof(cplx<R>) < 128) {
+    // A workgroup's columns are narrower than a 128-byte line: workgroups are
+    // dealt to the XCDs in turn, so neighbouring column groups would fetch the
+    // same lines into different L2s (4x the read traffic for 4-column groups).
+    // Each XCD takes whole fields instead, their column groups one after
+    // another, so the groups that share lines share them in one L2.
+    const long long b = (long long)blockIdx.y * gridDim.x + blockIdx.x;
+    long long outer = 0;
+    if (!xcd_tile(b, (int)gridDim.x, (long long)gridDim.y, blk, outer)) return;
+    field = (int)outer;
+  }
+  reg_sweep_program<R, N, ORIENT, XS, PG>(ex, a, reinterpret_cast<cplx<R>*>(lds_raw), field, blk);
 }
 
 namespace {
