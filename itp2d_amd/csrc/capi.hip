@@ -1815,6 +1815,24 @@ void itp_potential_destroy(itp_potential* p) {
   ctx_release(c);
 }
 
+}  // extern "C"
+
+namespace {
+
+// Device buffers of a set (every one of them may be null).
+void free_stateset_buffers(itp_ctx* c, itp_stateset* s) {
+  void* ptrs[] = {s->buf[0], s->buf[1], s->part, s->A[0], s->A[1], s->V[0], s->V[1], s->Vinit, s->bjQ, s->bjD,
+                  s->bjslots, s->bjident, s->oa_work, s->oa_lam, s->oa_slots, s->oa_state, s->cl_hdr, s->cl_pairs,
+                  s->cl_members, s->cl_offsets, s->Vsave, s->oa_part, s->C, s->C32,
+                  s->lam, s->perm, s->info, s->bad, s->epart, s->dE, s->dmean, s->diag0, s->chol_fail, s->chol_mm, s->norms,
+                  s->td_scratch};
+  for (void* p : ptrs) dfree(c, p);
+}
+
+}  // namespace
+
+extern "C" {
+
 int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out) {
   if (!g || !out || N < 1 || (precision != ITP_FP64 && precision != ITP_FP32)) return ITP_E_INVALID;
   itp_ctx* c = g->ctx;
@@ -1824,6 +1842,14 @@ int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out
   if (N > 65535) return set_err(c, ITP_E_UNSUPPORTED, "too many states");
   return guarded(c, [&]() -> int {
     std::unique_ptr<itp_stateset> s(new itp_stateset);
+    // a creation that fails part-way returns the buffers it took
+    struct Undo {
+      itp_ctx* c;
+      std::unique_ptr<itp_stateset>& s;
+      ~Undo() {
+        if (s) free_stateset_buffers(c, s.get());
+      }
+    } undo{c, s};
     s->grid = g;
     s->N = N;
     s->prec = precision;
@@ -1917,12 +1943,7 @@ int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out
 void itp_stateset_destroy(itp_stateset* s) {
   if (!s) return;
   itp_ctx* c = s->ctx;  // parents may already be gone
-  void* ptrs[] = {s->buf[0], s->buf[1], s->part, s->A[0], s->A[1], s->V[0], s->V[1], s->Vinit, s->bjQ, s->bjD,
-                  s->bjslots, s->bjident, s->oa_work, s->oa_lam, s->oa_slots, s->oa_state, s->cl_hdr, s->cl_pairs,
-                  s->cl_members, s->cl_offsets, s->Vsave, s->oa_part, s->C, s->C32,
-                  s->lam, s->perm, s->info, s->bad, s->epart, s->dE, s->dmean, s->diag0, s->chol_fail, s->chol_mm, s->norms,
-                  s->td_scratch};
-  for (void* p : ptrs) dfree(c, p);
+  free_stateset_buffers(c, s);
   delete s;
   ctx_release(c);
 }
