@@ -16,7 +16,10 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <exception>
+#include <functional>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -598,20 +601,49 @@ LineGeom make_geom(const GridInfo& g, const cplx<double>* d_tw, size_t esz) {
   return lg;
 }
 
+// Host tables in one device allocation, uploaded with one copy; ptrs[i] is
+// null for an empty table
 int upload_tables(itp_ctx* c, const std::vector<const std::vector<double>*>& tabs, double** dev,
                   std::vector<const double*>& ptrs) {
   size_t total = 0;
   for (auto t : tabs) total += t->size();
   int rc = dalloc(c, (void**)dev, total * sizeof(double));
   if (rc) return rc;
+  std::vector<double> h(total);
   size_t off = 0;
   ptrs.clear();
   for (auto t : tabs) {
     ptrs.push_back(t->empty() ? nullptr : *dev + off);
-    if (!t->empty()) HIPCK(c, cmemcpy(c, *dev + off, t->data(), t->size() * sizeof(double), hipMemcpyHostToDevice));
+    std::copy(t->begin(), t->end(), h.begin() + off);
     off += t->size();
   }
+  if (total) HIPCK(c, cmemcpy(c, *dev, h.data(), total * sizeof(double), hipMemcpyHostToDevice));
   return ITP_OK;
+}
+
+// Runs independent host jobs (table loops over the grid) on threads of their
+// own when the grid is large enough to pay for them; the first exception
+// thrown by a job is rethrown here.
+void run_jobs(std::vector<std::function<void()>>& jobs, long long M) {
+  if (jobs.size() < 2 || M < 16384) {
+    for (auto& j : jobs) j();
+    return;
+  }
+  std::vector<std::exception_ptr> err(jobs.size());
+  std::vector<std::thread> th;
+  th.reserve(jobs.size() - 1);
+  auto run = [&](size_t i) {
+    try {
+      jobs[i]();
+    } catch (...) {
+      err[i] = std::current_exception();
+    }
+  };
+  for (size_t i = 1; i < jobs.size(); i++) th.emplace_back(run, i);
+  run(0);
+  for (auto& t : th) t.join();
+  for (auto& e : err)
+    if (e) std::rethrow_exception(e);
 }
 
 // FP32 copies of host tables in one device allocation; ptrs[i] is null for an empty table
@@ -664,18 +696,28 @@ int build_split_tables(itp_split* sp) {
   for (auto& m : sp->members) free_member(c, m.get());
   sp->members.clear();
   sp->gen = next_object_id();
+  // every member's host tables first, the independent loops side by side
+  std::vector<std::unique_ptr<MemberTabs>> tabs;
+  std::vector<std::function<void()>> jobs;
+  const double* V = sp->pot->V.data();
+  const double B = sp->B;
   for (int t = 1; t <= sp->halforder; t++) {
-    std::unique_ptr<MemberTabs> m(new MemberTabs);
+    tabs.emplace_back(new MemberTabs);
+    MemberTabs* m = tabs.back().get();
     const double h = sp->eps / t;
     const double coef = sp->coefs[t - 1];
     if (has_v) {
-      expkin_tables(g, sp->B, h, -1.0, 1.0, m->host);
-      exppot_values(g, sp->pot->V.data(), h, -0.5, 1.0, m->vh);
-      if (t >= 2) exppot_values(g, sp->pot->V.data(), h, -1.0, 1.0, m->vf);
-      exppot_values(g, sp->pot->V.data(), h, -0.5, coef, m->vp);
+      jobs.push_back([&g, B, h, m] { expkin_tables(g, B, h, -1.0, 1.0, m->host); });
+      jobs.push_back([&g, V, h, m] { exppot_values(g, V, h, -0.5, 1.0, m->vh); });
+      if (t >= 2) jobs.push_back([&g, V, h, m] { exppot_values(g, V, h, -1.0, 1.0, m->vf); });
+      jobs.push_back([&g, V, h, coef, m] { exppot_values(g, V, h, -0.5, coef, m->vp); });
     } else {
-      expkin_tables(g, sp->B, h, -1.0 * t, coef, m->host);
+      jobs.push_back([&g, B, h, t, coef, m] { expkin_tables(g, B, h, -1.0 * t, coef, m->host); });
     }
+  }
+  run_jobs(jobs, g.M);
+  for (int t = 1; t <= sp->halforder; t++) {
+    std::unique_ptr<MemberTabs> m = std::move(tabs[t - 1]);
     std::vector<const double*> p;
     int rc = upload_tables(c, {&m->host.mult, &m->host.xm, &m->host.xm2, &m->host.ym, &m->vh, &m->vf, &m->vp},
                            &m->dev, p);
@@ -1913,11 +1955,7 @@ int itp_stateset_create(itp_grid* g, size_t N, int precision, itp_stateset** out
       if ((rc = dalloc(c, (void**)&s->Vsave, np2))) return rc;
       if (s->np < 1024 && (rc = dalloc(c, (void**)&s->oa_part, 4 * np2))) return rc;
     }
-    {
-      std::vector<cplx<double>> I((size_t)s->np * s->np, mk<double>(0.0, 0.0));
-      for (int i = 0; i < s->np; i++) I[(size_t)i * s->np + i] = mk<double>(1.0, 0.0);
-      HIPCK(c, cmemcpy(c, s->Vinit, I.data(), np2, hipMemcpyHostToDevice));
-    }
+    HIPCK(c, launch_identity(s->Vinit, s->np, c->stream));
     if ((rc = dalloc(c, (void**)&s->C, (size_t)s->nc * s->nc * 16))) return rc;
     if (precision == ITP_FP32 && (rc = dalloc(c, (void**)&s->C32, (size_t)s->nc * s->nc * 8))) return rc;
     HIPCK(c, cmemset(c, s->C, 0, (size_t)s->nc * s->nc * 16));

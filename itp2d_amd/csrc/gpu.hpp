@@ -23,6 +23,8 @@ template <class R, int N>
 hipError_t launch_reg_sweep_n(const SweepArgs<R>& a, int nblk, int nfields, hipStream_t st);
 // p[0..n) = 0 by a kernel (usable inside captured HIP graphs)
 hipError_t launch_zero_ints(int* p, int n, hipStream_t st);
+// A = I (np x np, row-major complex FP64)
+hipError_t launch_identity(cplx<double>* A, int np, hipStream_t st);
 hipError_t launch_energy_reduce(const cplx<double>* p1, const cplx<double>* p2, int nblk, int N, double dx,
                                 double* E, double* sigma, cplx<double>* mean, cplx<double>* msq, hipStream_t st);
 // V (sx x sy, device) = the potential kind q on the cell-centred grid

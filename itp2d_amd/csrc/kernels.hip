@@ -75,6 +75,19 @@ hipError_t launch_zero_ints(int* p, int n, hipStream_t st) {
   return hipGetLastError();
 }
 
+__global__ __launch_bounds__(256) void identity_kernel(cplx<double>* A, int np) {
+  const long long n2 = (long long)np * np;
+  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < n2; e += (long long)gridDim.x * 256)
+    A[e] = mk<double>(e / np == e % np ? 1.0 : 0.0, 0.0);
+}
+hipError_t launch_identity(cplx<double>* A, int np, hipStream_t st) {
+  if (np < 1) return hipErrorInvalidValue;
+  const long long n2 = (long long)np * np;
+  const unsigned blocks = (unsigned)std::min<long long>((n2 + 255) / 256, 4096);
+  hipLaunchKernelGGL(identity_kernel, dim3(blocks), dim3(256), 0, st, A, np);
+  return hipGetLastError();
+}
+
 template <class R>
 hipError_t launch_sweep(const SweepArgs<R>& a, int nblk, int nfields, hipStream_t st) {
   if (nblk <= 0 || nfields <= 0) return hipSuccess;
