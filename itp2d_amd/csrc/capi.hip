@@ -19,6 +19,7 @@
 #include <exception>
 #include <functional>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -639,8 +640,14 @@ void run_jobs(std::vector<std::function<void()>>& jobs, long long M) {
       err[i] = std::current_exception();
     }
   };
-  for (size_t i = 1; i < jobs.size(); i++) th.emplace_back(run, i);
+  size_t started = 1;
+  try {
+    for (; started < jobs.size(); started++) th.emplace_back(run, started);
+  } catch (const std::system_error&) {
+    // no more threads: the jobs not started run here
+  }
   run(0);
+  for (size_t i = started; i < jobs.size(); i++) run(i);
   for (auto& t : th) t.join();
   for (auto& e : err)
     if (e) std::rethrow_exception(e);

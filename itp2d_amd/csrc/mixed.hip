@@ -92,18 +92,31 @@ __global__ __launch_bounds__(256) void gram_bf16x3_kernel(const float* A, const 
       dim[a][b] = d4m{0, 0, 0, 0};
     }
   int sub = 0;
+  // the next step's reals are loaded while this step's products run (the
+  // loads of step k0 + 64 are issued before its MFMAs)
+  float4 nxa[4], nxb[4];
+  auto load_step = [&](long long k0) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int idx = tid + 256 * j;
+      const int row = idx >> 4, c4 = idx & 15;
+      const long long kk = k0 + 4 * c4;
+      nxa[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+      nxb[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (kk < k_end) {  // k_end - kk is a multiple of 4 (2 M and the chunk bounds are)
+        if (i0 + row < N) nxa[j] = *reinterpret_cast<const float4*>(A + (long long)(i0 + row) * K2 + kk);
+        if (j0 + row < N) nxb[j] = *reinterpret_cast<const float4*>(B + (long long)(j0 + row) * K2 + kk);
+      }
+    }
+  };
+  load_step(k_begin);
   for (long long k0 = k_begin; k0 < k_end; k0 += kG3KT) {
     // stage 64 rows x 64 reals of A and of B as three bf16 parts
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const int idx = tid + 256 * j;
       const int row = idx >> 4, c4 = idx & 15;
-      const long long kk = k0 + 4 * c4;
-      float4 va = make_float4(0.f, 0.f, 0.f, 0.f), vb = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (kk < k_end) {  // k_end - kk is a multiple of 4 (2 M and the chunk bounds are)
-        if (i0 + row < N) va = *reinterpret_cast<const float4*>(A + (long long)(i0 + row) * K2 + kk);
-        if (j0 + row < N) vb = *reinterpret_cast<const float4*>(B + (long long)(j0 + row) * K2 + kk);
-      }
+      const float4 va = nxa[j], vb = nxb[j];
       const float xa[4] = {va.x, va.y, va.z, va.w}, xb[4] = {vb.x, vb.y, vb.z, vb.w};
       bf4v ah, am, al, bh, bm, bl;
 #pragma unroll
@@ -122,6 +135,7 @@ __global__ __launch_bounds__(256) void gram_bf16x3_kernel(const float* A, const 
       *reinterpret_cast<bf4v*>(&Bs[2][row][4 * c4]) = bl;
     }
     __syncthreads();
+    if (k0 + kG3KT < k_end) load_step(k0 + kG3KT);
 #pragma unroll
     for (int ks = 0; ks < kG3KT / 32; ks++) {
       const int kcol = ks * 32 + 8 * (lane >> 4);
@@ -322,7 +336,10 @@ __global__ __launch_bounds__(256) void rotate_bf16x3_kernel(const cplx<float>* i
     }
   // upper triangular C: output states r0..r0+63 take inputs c < r0 + 64 only
   const int cend = upper ? (N < r0 + 64 ? N : r0 + 64) : N;
-  for (int c0 = 0; c0 < cend; c0 += 16) {
+  // the next step's coefficients and states are loaded while this step's
+  // products run (the loads of step c0 + 16 are issued before its MFMAs)
+  cplx<float> nxa[4], nxp[4];
+  auto load_step = [&](int c0) {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const int e = tid + 256 * q;
@@ -330,8 +347,18 @@ __global__ __launch_bounds__(256) void rotate_bf16x3_kernel(const cplx<float>* i
       const int c = c0 + cc;
       // C is zero-padded to nc x nc; states c >= N contribute nothing
       const cplx<float> cv = C[(long long)(c < nc ? c : nc - 1) * nc + r0 + x];
-      const cplx<float> a = (c < N) ? cv : mk<float>(0.f, 0.f);
-      const cplx<float> pv = (c < N && t0 + x < M) ? in[(long long)c * M + t0 + x] : mk<float>(0.f, 0.f);
+      nxa[q] = (c < N) ? cv : mk<float>(0.f, 0.f);
+      nxp[q] = (c < N && t0 + x < M) ? in[(long long)c * M + t0 + x] : mk<float>(0.f, 0.f);
+    }
+  };
+  load_step(0);
+  for (int c0 = 0; c0 < cend; c0 += 16) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int e = tid + 256 * q;
+      const int cc = e >> 6, x = e & 63;
+      const cplx<float> a = nxa[q];
+      const cplx<float> pv = nxp[q];
       // each (re, im) pair of a split part goes to LDS as one 4-byte store
       __bf16 s[4][3];
       split3(a.x, s[0][0], s[0][1], s[0][2]);
@@ -348,6 +375,7 @@ __global__ __launch_bounds__(256) void rotate_bf16x3_kernel(const cplx<float>* i
       *reinterpret_cast<bf2v*>(&Bs[2][x][2 * cc]) = bl;
     }
     __syncthreads();
+    if (c0 + 16 < cend) load_step(c0 + 16);
     const int kcol = 8 * (lane >> 4);
     bf8 a[3][2], br[3][2], bi[3][2];
 #pragma unroll
