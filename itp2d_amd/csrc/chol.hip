@@ -88,37 +88,72 @@ __device__ __forceinline__ void acc_store(const d4& cr, const d4& ci, double s, 
 // elimination runs on [S_kk | I]: the same row operations that turn S_kk into
 // R_kk turn I into L^-1 = R^-H, so the inverse needs no serial triangular
 // solve. Row r of the tile is global index k*T + r; the pivot test uses the
-// original diagonal.
+// original diagonal. The elimination keeps the tile in registers (thread t:
+// column t % 32 of rows t / 32 + 8 s) and passes only the pivot row through
+// LDS, double-buffered: one barrier per column. Needs 256 threads.
+__device__ __forceinline__ cplx<double> pick4(cplx<double> v0, cplx<double> v1, cplx<double> v2, cplx<double> v3,
+                                               int s) {
+  return s == 0 ? v0 : s == 1 ? v1 : s == 2 ? v2 : v3;
+}
 __device__ void factor_diag(cplx<double>* a, cplx<double>* lin, cplx<double>* inv, const CholArgs& p, int k) {
   __shared__ double piv[T];
   __shared__ double ref0[T];  // the rows' original diagonal (read once, not per column)
+  __shared__ cplx<double> rowA[2][T], rowL[2][T];
   __shared__ int bad;
   const int tid = threadIdx.x;
+  const int col = tid & (T - 1), grp = tid >> 5;
+  cplx<double> ra[4], rl[4];
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    const int r = grp + 8 * s;
+    ra[s] = a[r * T + col];
+    rl[s] = mk<double>(r == col ? 1.0 : 0.0, 0.0);
+  }
   if (tid == 0) bad = 0;
   if (tid < T) ref0[tid] = p.diag0[k * T + tid];
-  for (int e = tid; e < T2; e += blockDim.x) lin[e] = mk<double>((e / T) == (e % T) ? 1.0 : 0.0, 0.0);
+  if (grp == 0) {
+    rowA[0][col] = ra[0];
+    rowL[0][col] = rl[0];
+  }
   __syncthreads();
   for (int c = 0; c < T; c++) {
-    const double d = a[c * T + c].x;
+    const int b = c & 1;
+    const double d = rowA[b][c].x;
     const double ref = ref0[c];
     const bool ok = std::isfinite(d) && d > p.tol * ref && d > 0;
+    // the pivot's square root waits for the end (off the column chain)
     if (tid == 0) {
-      piv[c] = ok ? std::sqrt(d) : 1.0;
+      piv[c] = ok ? d : -1.0;
       if (!ok) bad = 1;
     }
     const double invd = ok ? 1.0 / d : 0.0;
     // rows i > c: S part a_ij -= conj(a_ci) a_cj / d (j >= i), identity part
-    // lin_ij -= conj(a_ci) lin_cj / d (j <= c); thread t takes column t % 32
-    // of rows c + 1 + t / 32, c + 9 + t / 32, ...
-    const int j = tid & (T - 1);
-    const cplx<double> sc = a[c * T + j], lc = lin[c * T + j];
-    for (int i = c + 1 + (tid >> 5); i < T; i += (int)blockDim.x >> 5) {
-      const cplx<double> m = conj(a[c * T + i]) * invd;
-      if (j >= i) a[i * T + j] = a[i * T + j] - m * sc;
-      else if (j <= c) lin[i * T + j] = lin[i * T + j] - m * lc;
+    // lin_ij -= conj(a_ci) lin_cj / d (j <= c); this thread's j is col
+    const cplx<double> sc = rowA[b][col], lc = rowL[b][col];
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      const int i = grp + 8 * s;
+      if (i > c) {
+        const cplx<double> m = conj(rowA[b][i]) * invd;
+        if (col >= i) ra[s] = ra[s] - m * sc;
+        else if (col <= c) rl[s] = rl[s] - m * lc;
+      }
+    }
+    // row c + 1 is final now: the next column's pivot row
+    if (c + 1 < T && ((c + 1) & 7) == grp) {
+      rowA[b ^ 1][col] = pick4(ra[0], ra[1], ra[2], ra[3], (c + 1) >> 3);
+      rowL[b ^ 1][col] = pick4(rl[0], rl[1], rl[2], rl[3], (c + 1) >> 3);
     }
     __syncthreads();
   }
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    const int r = grp + 8 * s;
+    a[r * T + col] = ra[s];
+    lin[r * T + col] = rl[s];
+  }
+  if (tid < T) piv[tid] = piv[tid] > 0 ? std::sqrt(piv[tid]) : 1.0;
+  __syncthreads();
   // R: rows scaled by 1 / sqrt(pivot), lower triangle zero; W = (L^-1)^H with
   // L^-1 the scaled identity part
   for (int e = tid; e < T2; e += blockDim.x) {
