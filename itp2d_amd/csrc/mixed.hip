@@ -339,11 +339,12 @@ __global__ __launch_bounds__(256) void rotate_bf16x3_kernel(const cplx<float>* i
   // the next step's coefficients and states are loaded while this step's
   // products run (the loads of step c0 + 16 are issued before its MFMAs)
   cplx<float> nxa[4], nxp[4];
+  // thread (w, x) = (tid / 64, tid % 64) stages inputs cc = 4 w + q of row x:
+  // global loads coalesced along x, and one 16-byte LDS store per split part
   auto load_step = [&](int c0) {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      const int e = tid + 256 * q;
-      const int cc = e >> 6, x = e & 63;
+      const int cc = 4 * (tid >> 6) + q, x = tid & 63;
       const int c = c0 + cc;
       // C is zero-padded to nc x nc; states c >= N contribute nothing
       const cplx<float> cv = C[(long long)(c < nc ? c : nc - 1) * nc + r0 + x];
@@ -353,26 +354,32 @@ __global__ __launch_bounds__(256) void rotate_bf16x3_kernel(const cplx<float>* i
   };
   load_step(0);
   for (int c0 = 0; c0 < cend; c0 += 16) {
+    {
+      // the (re, im) pairs of inputs 4 w .. 4 w + 3, per split part
+      bf8 ap[3], bp[3];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int e = tid + 256 * q;
-      const int cc = e >> 6, x = e & 63;
-      const cplx<float> a = nxa[q];
-      const cplx<float> pv = nxp[q];
-      // each (re, im) pair of a split part goes to LDS as one 4-byte store
-      __bf16 s[4][3];
-      split3(a.x, s[0][0], s[0][1], s[0][2]);
-      split3(a.y, s[1][0], s[1][1], s[1][2]);
-      split3(pv.x, s[2][0], s[2][1], s[2][2]);
-      split3(pv.y, s[3][0], s[3][1], s[3][2]);
-      const bf2v ah = {s[0][0], s[1][0]}, am = {s[0][1], s[1][1]}, al = {s[0][2], s[1][2]};
-      const bf2v bh = {s[2][0], s[3][0]}, bm = {s[2][1], s[3][1]}, bl = {s[2][2], s[3][2]};
-      *reinterpret_cast<bf2v*>(&As[0][x][2 * cc]) = ah;
-      *reinterpret_cast<bf2v*>(&As[1][x][2 * cc]) = am;
-      *reinterpret_cast<bf2v*>(&As[2][x][2 * cc]) = al;
-      *reinterpret_cast<bf2v*>(&Bs[0][x][2 * cc]) = bh;
-      *reinterpret_cast<bf2v*>(&Bs[1][x][2 * cc]) = bm;
-      *reinterpret_cast<bf2v*>(&Bs[2][x][2 * cc]) = bl;
+      for (int q = 0; q < 4; q++) {
+        const cplx<float> a = nxa[q];
+        const cplx<float> pv = nxp[q];
+        __bf16 s[4][3];
+        split3(a.x, s[0][0], s[0][1], s[0][2]);
+        split3(a.y, s[1][0], s[1][1], s[1][2]);
+        split3(pv.x, s[2][0], s[2][1], s[2][2]);
+        split3(pv.y, s[3][0], s[3][1], s[3][2]);
+#pragma unroll
+        for (int sp = 0; sp < 3; sp++) {
+          ap[sp][2 * q] = s[0][sp];
+          ap[sp][2 * q + 1] = s[1][sp];
+          bp[sp][2 * q] = s[2][sp];
+          bp[sp][2 * q + 1] = s[3][sp];
+        }
+      }
+      const int x = tid & 63, k8 = 8 * (tid >> 6);
+#pragma unroll
+      for (int sp = 0; sp < 3; sp++) {
+        *reinterpret_cast<bf8*>(&As[sp][x][k8]) = ap[sp];
+        *reinterpret_cast<bf8*>(&Bs[sp][x][k8]) = bp[sp];
+      }
     }
     __syncthreads();
     if (c0 + 16 < cend) load_step(c0 + 16);
