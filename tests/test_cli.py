@@ -140,6 +140,22 @@ def test_cli_run_matches_oracle_golden(tmp_path):
     assert D.read_doubles(p, "/final_energy_standard_deviations").size == N
 
 
+@pytest.mark.gpu
+def test_cli_wisdom_file_of_another_format_is_replaced(tmp_path, capsys):
+    """A wisdom file this engine did not write (FFTW's own format, as the
+    reference leaves it, itp2d.cpp:86-92) is not used: the run goes on with a
+    note and the file is written back with the tuned shapes."""
+    print("[gpu] command-line run with an FFTW wisdom file", flush=True)
+    w = tmp_path / "fftw_wisdom"
+    w.write_text("(fftw-3.3.10 fftw_wisdom #x4be12fff #x7b2df9b2 #xa9bdb5b5 #xc8a0e8ba\n)\n")
+    argv = ["-s", "32", "-l", "10", "-n", "4", "-N", "6", "-e", "0.1", "-p", "harmonic", "-T", "relchange(1e-8)",
+            "-F", "relchange(1e-8)", "--maxsteps", "500", "--rngseed", "3", "--save-nothing", "--wisdomfile", str(w)]
+    assert cli.main(argv) == 0
+    assert "not used" in capsys.readouterr().err
+    lines = w.read_text().splitlines()
+    assert lines and all(line.split()[0] == "gram" and len(line.split()) == 5 for line in lines)
+
+
 def test_signal_flags_installed_like_the_reference():
     """SIGUSR1 raises the save flag and SIGINT the abort flag, each with the
     reference's note on stderr (itp2d.cpp:40-66); run in a child process so
