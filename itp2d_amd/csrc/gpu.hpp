@@ -49,7 +49,7 @@ hipError_t launch_gram(const cplx<R>* psi, const cplx<R>* phi, long long M, int 
                        int nchunk, long long kchunk, double dx, cplx<double>* S, int np, int* bad, hipStream_t st,
                        int full = 0);
 hipError_t launch_gram_reduce(const cplx<double>* part, int nchunk, int nc, int N, int np, double dx, cplx<double>* S,
-                              int* bad, hipStream_t st, int full = 0);
+                              int* bad, hipStream_t st, int full = 0, int acc = 1);
 // mixed path (mixed.hip)
 hipError_t launch_gram_bf16x3(const cplx<float>* A, const cplx<float>* B, long long M, int N, int nc,
                               cplx<double>* part, int nchunk, long long kchunk, double dx, cplx<double>* S, int np,

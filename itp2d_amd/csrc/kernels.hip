@@ -342,13 +342,30 @@ __global__ __launch_bounds__(256) void gram_kernel(const cplx<R>* psi, const cpl
       }
 }
 
+// ACC partial sums over the chunks (ACC = 1: chunk order, the FP64 overlap's;
+// ACC = 4: four interleaved sums, more loads in flight, for the split-BF16x3
+// overlap)
+template <int ACC>
 __global__ void gram_reduce_kernel(const cplx<double>* part, int nchunk, int nc, int N, int np, double dx,
                                    cplx<double>* S, int* bad, int full) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
   if (j >= N || i > j) return;
   cplx<double> s = mk<double>(0.0, 0.0);
-  for (int z = 0; z < nchunk; z++) s = s + part[((long long)z * nc + i) * nc + j];
+  if constexpr (ACC == 1) {
+    for (int z = 0; z < nchunk; z++) s = s + part[((long long)z * nc + i) * nc + j];
+  } else {
+    cplx<double> t[ACC];
+#pragma unroll
+    for (int a = 0; a < ACC; a++) t[a] = mk<double>(0.0, 0.0);
+    int z = 0;
+    for (; z + ACC <= nchunk; z += ACC)
+#pragma unroll
+      for (int a = 0; a < ACC; a++) t[a] = t[a] + part[((long long)(z + a) * nc + i) * nc + j];
+    for (; z < nchunk; z++) t[0] = t[0] + part[((long long)z * nc + i) * nc + j];
+#pragma unroll
+    for (int a = 0; a < ACC; a++) s = s + t[a];
+  }
   if (full) {  // Hermitian part (G + G^H) / 2 of a matrix computed in full
     cplx<double> t = mk<double>(0.0, 0.0);
     for (int z = 0; z < nchunk; z++) t = t + part[((long long)z * nc + j) * nc + i];
@@ -381,9 +398,13 @@ hipError_t launch_gram(const cplx<R>* psi, const cplx<R>* phi, long long M, int 
 }
 
 hipError_t launch_gram_reduce(const cplx<double>* part, int nchunk, int nc, int N, int np, double dx, cplx<double>* S,
-                              int* bad, hipStream_t st, int full) {
-  hipLaunchKernelGGL(gram_reduce_kernel, dim3((N + 127) / 128, N), dim3(128), 0, st, part, nchunk, nc, N, np, dx, S,
-                     bad, full);
+                              int* bad, hipStream_t st, int full, int acc) {
+  if (acc == 4)
+    hipLaunchKernelGGL(gram_reduce_kernel<4>, dim3((N + 127) / 128, N), dim3(128), 0, st, part, nchunk, nc, N, np, dx,
+                       S, bad, full);
+  else
+    hipLaunchKernelGGL(gram_reduce_kernel<1>, dim3((N + 127) / 128, N), dim3(128), 0, st, part, nchunk, nc, N, np, dx,
+                       S, bad, full);
   return hipGetLastError();
 }
 template hipError_t launch_gram<double>(const cplx<double>*, const cplx<double>*, long long, int, int, cplx<double>*,

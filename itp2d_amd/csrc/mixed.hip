@@ -210,7 +210,7 @@ hipError_t launch_gram_bf16x3(const cplx<float>* A, const cplx<float>* B, long l
                      reinterpret_cast<const float*>(B), M, N, nb, kchunk, part, nc, full);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  return launch_gram_reduce(part, nchunk, nc, N, np, dx, S, bad, st, full);
+  return launch_gram_reduce(part, nchunk, nc, N, np, dx, S, bad, st, full, 4);
 }
 
 // ---------------------------------------------------------------------------
