@@ -46,14 +46,29 @@ struct CholArgs {
 
 __device__ __forceinline__ long long tile_off(int ti, int tj, int ld) { return (long long)ti * T * ld + (long long)tj * T; }
 
-__device__ void load_tile(const cplx<double>* G, int ld, int ti, int tj, cplx<double>* L) {
+// Tile copies between global memory and LDS by the kernels' 256 threads;
+// each thread's four loads are issued together (and the tiles a kernel loads
+// one after another overlap, the loads being independent of the LDS stores).
+constexpr int kTileThreads = 256;
+__device__ __forceinline__ void load_tile(const cplx<double>* G, int ld, int ti, int tj, cplx<double>* L) {
   const cplx<double>* g = G + tile_off(ti, tj, ld);
-  for (int e = threadIdx.x; e < T2; e += blockDim.x) L[e] = g[(long long)(e / T) * ld + (e % T)];
+  cplx<double> v[T2 / kTileThreads];
+#pragma unroll
+  for (int r = 0; r < T2 / kTileThreads; r++) {
+    const int e = threadIdx.x + kTileThreads * r;
+    v[r] = g[(long long)(e / T) * ld + (e % T)];
+  }
+#pragma unroll
+  for (int r = 0; r < T2 / kTileThreads; r++) L[threadIdx.x + kTileThreads * r] = v[r];
 }
 
-__device__ void store_tile(cplx<double>* G, int ld, int ti, int tj, const cplx<double>* L) {
+__device__ __forceinline__ void store_tile(cplx<double>* G, int ld, int ti, int tj, const cplx<double>* L) {
   cplx<double>* g = G + tile_off(ti, tj, ld);
-  for (int e = threadIdx.x; e < T2; e += blockDim.x) g[(long long)(e / T) * ld + (e % T)] = L[e];
+#pragma unroll
+  for (int r = 0; r < T2 / kTileThreads; r++) {
+    const int e = threadIdx.x + kTileThreads * r;
+    g[(long long)(e / T) * ld + (e % T)] = L[e];
+  }
 }
 
 // acc += op(A) B for LDS tiles (op = conjugate transpose if ah); each
