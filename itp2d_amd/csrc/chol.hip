@@ -133,23 +133,29 @@ __device__ void factor_diag(cplx<double>* a, cplx<double>* lin, cplx<double>* in
   __syncthreads();
   for (int c = 0; c < T; c++) {
     const int b = c & 1;
+    // every LDS read of the column first and unconditionally, so that they
+    // are in flight together (one wait instead of one per row)
     const double d = rowA[b][c].x;
     const double ref = ref0[c];
-    const bool ok = std::isfinite(d) && d > p.tol * ref && d > 0;
+    const cplx<double> sc = rowA[b][col], lc = rowL[b][col];
+    cplx<double> aci[4];
+#pragma unroll
+    for (int s = 0; s < 4; s++) aci[s] = rowA[b][grp + 8 * s];
+    const bool ok = std::isfinite(d) & (d > p.tol * ref) & (d > 0);
     // the pivot's square root waits for the end (off the column chain)
     if (tid == 0) {
       piv[c] = ok ? d : -1.0;
       if (!ok) bad = 1;
     }
-    const double invd = ok ? 1.0 / d : 0.0;
+    const double rd = 1.0 / d;
+    const double invd = ok ? rd : 0.0;
     // rows i > c: S part a_ij -= conj(a_ci) a_cj / d (j >= i), identity part
     // lin_ij -= conj(a_ci) lin_cj / d (j <= c); this thread's j is col
-    const cplx<double> sc = rowA[b][col], lc = rowL[b][col];
 #pragma unroll
     for (int s = 0; s < 4; s++) {
       const int i = grp + 8 * s;
       if (i > c) {
-        const cplx<double> m = conj(rowA[b][i]) * invd;
+        const cplx<double> m = conj(aci[s]) * invd;
         if (col >= i) ra[s] = ra[s] - m * sc;
         else if (col <= c) rl[s] = rl[s] - m * lc;
       }
@@ -167,7 +173,11 @@ __device__ void factor_diag(cplx<double>* a, cplx<double>* lin, cplx<double>* in
     a[r * T + col] = ra[s];
     lin[r * T + col] = rl[s];
   }
-  if (tid < T) piv[tid] = piv[tid] > 0 ? std::sqrt(piv[tid]) : 1.0;
+  if (tid < T) {
+    const double sq = piv[tid] > 0 ? std::sqrt(piv[tid]) : 1.0;
+    piv[tid] = sq;
+    ref0[tid] = 1.0 / sq;  // ref0 is free now: the reciprocal square roots
+  }
   __syncthreads();
   // R: rows scaled by 1 / sqrt(pivot), lower triangle zero; W = (L^-1)^H with
   // L^-1 the scaled identity part
@@ -175,8 +185,8 @@ __device__ void factor_diag(cplx<double>* a, cplx<double>* lin, cplx<double>* in
     const int i = e / T, j = e % T;
     if (j < i) a[e] = mk<double>(0.0, 0.0);
     else if (j == i) a[e] = mk<double>(piv[i], 0.0);
-    else a[e] = a[e] * (1.0 / piv[i]);
-    inv[e] = j < i ? mk<double>(0.0, 0.0) : conj(lin[j * T + i]) * (1.0 / piv[j]);
+    else a[e] = a[e] * ref0[i];
+    inv[e] = j < i ? mk<double>(0.0, 0.0) : conj(lin[j * T + i]) * ref0[j];
   }
   __syncthreads();
   if (tid == 0 && bad) atomicOr(p.fail, 1);
