@@ -55,8 +55,14 @@ __device__ __forceinline__ bf8 swapneg(bf8 b) {
   return r;
 }
 
-__global__ __launch_bounds__(256) void gram_bf16x3_kernel(const float* A, const float* B, long long M, int N, int nb,
-                                                          long long kchunk, cplx<double>* part, int nc, int full) {
+// Eight wavefronts per workgroup: wavefront (wi, wj) = (w / 4, w % 4) holds
+// rows wi*32 .. +31 of the I tile against columns wj*16 .. +15 of the J tile
+// (two 16 x 16 MFMA tiles), so that a wavefront's registers allow four
+// wavefronts per SIMD with two workgroups (the LDS limit) per CU.
+constexpr int kG3Threads = 512;
+__global__ __launch_bounds__(kG3Threads) void gram_bf16x3_kernel(const float* A, const float* B, long long M, int N,
+                                                                 int nb, long long kchunk, cplx<double>* part, int nc,
+                                                                 int full) {
   __shared__ __align__(16) __bf16 As[3][kG3Rows][kG3KT + kG3Pad];
   __shared__ __align__(16) __bf16 Bs[3][kG3Rows][kG3KT + kG3Pad];
   // tile pair (all of them if full, else I <= J) and grid chunk in the block
@@ -78,27 +84,27 @@ __global__ __launch_bounds__(256) void gram_bf16x3_kernel(const float* A, const 
   const long long k_begin = (long long)chunk * 2 * kchunk;
   const long long k_end = k_begin + 2 * kchunk < K2 ? k_begin + 2 * kchunk : K2;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int wi = w >> 1, wj = w & 1;
+  const int wi = w >> 2, wj = w & 3;
   const int i0 = I * kG3Rows, j0 = J * kG3Rows;
-  f4 re[2][2], im[2][2];
-  d4m dre[2][2], dim[2][2];
+  f4 re[2], im[2];
+  d4m dre[2], dim[2];
 #pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++) {
-      re[a][b] = f4{0, 0, 0, 0};
-      im[a][b] = f4{0, 0, 0, 0};
-      dre[a][b] = d4m{0, 0, 0, 0};
-      dim[a][b] = d4m{0, 0, 0, 0};
-    }
+  for (int x = 0; x < 2; x++) {
+    re[x] = f4{0, 0, 0, 0};
+    im[x] = f4{0, 0, 0, 0};
+    dre[x] = d4m{0, 0, 0, 0};
+    dim[x] = d4m{0, 0, 0, 0};
+  }
   int sub = 0;
   // the next step's reals are loaded while this step's products run (the
-  // loads of step k0 + 64 are issued before its MFMAs)
-  float4 nxa[4], nxb[4];
+  // loads of step k0 + 64 are issued before its MFMAs); item j of a thread is
+  // row (tid + 512 j) / 16, reals 4 ((tid + 512 j) % 16) .. + 3 of the step
+  constexpr int kItems = kG3Rows * kG3KT / 4 / kG3Threads;
+  float4 nxa[kItems], nxb[kItems];
   auto load_step = [&](long long k0) {
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int idx = tid + 256 * j;
+    for (int j = 0; j < kItems; j++) {
+      const int idx = tid + kG3Threads * j;
       const int row = idx >> 4, c4 = idx & 15;
       const long long kk = k0 + 4 * c4;
       nxa[j] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -113,8 +119,8 @@ __global__ __launch_bounds__(256) void gram_bf16x3_kernel(const float* A, const 
   for (long long k0 = k_begin; k0 < k_end; k0 += kG3KT) {
     // stage 64 rows x 64 reals of A and of B as three bf16 parts
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int idx = tid + 256 * j;
+    for (int j = 0; j < kItems; j++) {
+      const int idx = tid + kG3Threads * j;
       const int row = idx >> 4, c4 = idx & 15;
       const float4 va = nxa[j], vb = nxb[j];
       const float xa[4] = {va.x, va.y, va.z, va.w}, xb[4] = {vb.x, vb.y, vb.z, vb.w};
@@ -139,33 +145,31 @@ __global__ __launch_bounds__(256) void gram_bf16x3_kernel(const float* A, const 
 #pragma unroll
     for (int ks = 0; ks < kG3KT / 32; ks++) {
       const int kcol = ks * 32 + 8 * (lane >> 4);
-      bf8 a[3][2], b[3][2], bp[3][2];
+      bf8 a[3][2], b[3], bp[3];
 #pragma unroll
-      for (int s = 0; s < 3; s++)
+      for (int s = 0; s < 3; s++) {
 #pragma unroll
-        for (int x = 0; x < 2; x++) {
+        for (int x = 0; x < 2; x++)
           a[s][x] = *reinterpret_cast<const bf8*>(&As[s][wi * 32 + x * 16 + (lane & 15)][kcol]);
-          b[s][x] = *reinterpret_cast<const bf8*>(&Bs[s][wj * 32 + x * 16 + (lane & 15)][kcol]);
-          bp[s][x] = swapneg(b[s][x]);
-        }
+        b[s] = *reinterpret_cast<const bf8*>(&Bs[s][wj * 16 + (lane & 15)][kcol]);
+        bp[s] = swapneg(b[s]);
+      }
 #pragma unroll
-      for (int x = 0; x < 2; x++)
-#pragma unroll
-        for (int y = 0; y < 2; y++) {
-          // split products with p + q <= 4 (1-based): (1,1) (1,2) (2,1) (2,2) (1,3) (3,1)
-          re[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][x], b[0][y], re[x][y], 0, 0, 0);
-          re[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], b[2][y], re[x][y], 0, 0, 0);
-          re[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][x], b[1][y], re[x][y], 0, 0, 0);
-          re[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][x], b[0][y], re[x][y], 0, 0, 0);
-          re[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], b[1][y], re[x][y], 0, 0, 0);
-          re[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], b[0][y], re[x][y], 0, 0, 0);
-          im[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][x], bp[0][y], im[x][y], 0, 0, 0);
-          im[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], bp[2][y], im[x][y], 0, 0, 0);
-          im[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][x], bp[1][y], im[x][y], 0, 0, 0);
-          im[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][x], bp[0][y], im[x][y], 0, 0, 0);
-          im[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], bp[1][y], im[x][y], 0, 0, 0);
-          im[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], bp[0][y], im[x][y], 0, 0, 0);
-        }
+      for (int x = 0; x < 2; x++) {
+        // split products with p + q <= 4 (1-based): (1,1) (1,2) (2,1) (2,2) (1,3) (3,1)
+        re[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][x], b[0], re[x], 0, 0, 0);
+        re[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], b[2], re[x], 0, 0, 0);
+        re[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][x], b[1], re[x], 0, 0, 0);
+        re[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][x], b[0], re[x], 0, 0, 0);
+        re[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], b[1], re[x], 0, 0, 0);
+        re[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], b[0], re[x], 0, 0, 0);
+        im[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][x], bp[0], im[x], 0, 0, 0);
+        im[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], bp[2], im[x], 0, 0, 0);
+        im[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][x], bp[1], im[x], 0, 0, 0);
+        im[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][x], bp[0], im[x], 0, 0, 0);
+        im[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], bp[1], im[x], 0, 0, 0);
+        im[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], bp[0], im[x], 0, 0, 0);
+      }
     }
     __syncthreads();
     // FP32 partial sums span at most kG3Sub reals, then move to FP64
@@ -174,27 +178,22 @@ __global__ __launch_bounds__(256) void gram_bf16x3_kernel(const float* A, const 
 #pragma unroll
       for (int x = 0; x < 2; x++)
 #pragma unroll
-        for (int y = 0; y < 2; y++)
-#pragma unroll
-          for (int v = 0; v < 4; v++) {
-            dre[x][y][v] += (double)re[x][y][v];
-            dim[x][y][v] += (double)im[x][y][v];
-            re[x][y][v] = 0.f;
-            im[x][y][v] = 0.f;
-          }
+        for (int v = 0; v < 4; v++) {
+          dre[x][v] += (double)re[x][v];
+          dim[x][v] += (double)im[x][v];
+          re[x][v] = 0.f;
+          im[x][v] = 0.f;
+        }
     }
   }
 #pragma unroll
   for (int x = 0; x < 2; x++)
 #pragma unroll
-    for (int y = 0; y < 2; y++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int i = i0 + wi * 32 + x * 16 + 4 * (lane >> 4) + v;
-        const int j = j0 + wj * 32 + y * 16 + (lane & 15);
-        part[((long long)chunk * nc + i) * nc + j] =
-            mk<double>(dre[x][y][v] + (double)re[x][y][v], dim[x][y][v] + (double)im[x][y][v]);
-      }
+    for (int v = 0; v < 4; v++) {
+      const int i = i0 + wi * 32 + x * 16 + 4 * (lane >> 4) + v;
+      const int j = j0 + wj * 16 + (lane & 15);
+      part[((long long)chunk * nc + i) * nc + j] = mk<double>(dre[x][v] + (double)re[x][v], dim[x][v] + (double)im[x][v]);
+    }
 }
 
 hipError_t launch_gram_bf16x3(const cplx<float>* A, const cplx<float>* B, long long M, int N, int nc,
@@ -206,7 +205,7 @@ hipError_t launch_gram_bf16x3(const cplx<float>* A, const cplx<float>* B, long l
   if (nchunk != (M + kchunk - 1) / kchunk) return hipErrorInvalidValue;
   const long long blocks = xcd_grid(npairs, nchunk);
   if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(gram_bf16x3_kernel, dim3((unsigned)blocks), dim3(256), 0, st, reinterpret_cast<const float*>(A),
+  hipLaunchKernelGGL(gram_bf16x3_kernel, dim3((unsigned)blocks), dim3(kG3Threads), 0, st, reinterpret_cast<const float*>(A),
                      reinterpret_cast<const float*>(B), M, N, nb, kchunk, part, nc, full);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
