@@ -312,9 +312,14 @@ __device__ __forceinline__ bf8 swappairs(bf8 b) {
   return r;
 }
 
-__global__ __launch_bounds__(256) void rotate_bf16x3_kernel(const cplx<float>* in, cplx<float>* out,
-                                                            const cplx<float>* C, int nc, long long M, int N,
-                                                            int upper) {
+// Eight wavefronts per workgroup: wavefront (wr, wt) = (w / 4, w % 4) holds
+// output states wr*32 .. +31 against grid points wt*16 .. +15 (two 16 x 16
+// MFMA tiles), which keeps a wavefront's registers low enough for four
+// wavefronts per SIMD.
+constexpr int kR3Threads = 512;
+__global__ __launch_bounds__(kR3Threads) __attribute__((amdgpu_waves_per_eu(6))) void rotate_bf16x3_kernel(const cplx<float>* in, cplx<float>* out,
+                                                                   const cplx<float>* C, int nc, long long M, int N,
+                                                                   int upper) {
   __shared__ __align__(16) __bf16 As[3][64][32 + kR3Pad];  // [part][output state r][interleaved input k]
   __shared__ __align__(16) __bf16 Bs[3][64][32 + kR3Pad];  // [part][grid point t][interleaved input k]
   int rt;
@@ -324,26 +329,24 @@ __global__ __launch_bounds__(256) void rotate_bf16x3_kernel(const cplx<float>* i
   if (!xcd_tile(blockIdx.x, nc / 64, (M + 63) / 64, rt, tt)) return;
   const long long t0 = tt * 64;
   const int r0 = rt * 64;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wr = wave >> 1, wt = wave & 1;
-  f4 re[2][2], im[2][2];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wr = wave >> 2, wt = wave & 3;
+  f4 re[2], im[2];
 #pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++) {
-      re[a][b] = f4{0, 0, 0, 0};
-      im[a][b] = f4{0, 0, 0, 0};
-    }
+  for (int x = 0; x < 2; x++) {
+    re[x] = f4{0, 0, 0, 0};
+    im[x] = f4{0, 0, 0, 0};
+  }
   // upper triangular C: output states r0..r0+63 take inputs c < r0 + 64 only
   const int cend = upper ? (N < r0 + 64 ? N : r0 + 64) : N;
   // the next step's coefficients and states are loaded while this step's
   // products run (the loads of step c0 + 16 are issued before its MFMAs)
-  cplx<float> nxa[4], nxp[4];
-  // thread (w, x) = (tid / 64, tid % 64) stages inputs cc = 4 w + q of row x:
-  // global loads coalesced along x, and one 16-byte LDS store per split part
+  cplx<float> nxa[2], nxp[2];
+  // thread (w, x) = (tid / 64, tid % 64) stages inputs cc = 2 w + q of row x:
+  // global loads coalesced along x, and one 8-byte LDS store per split part
   auto load_step = [&](int c0) {
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int cc = 4 * (tid >> 6) + q, x = tid & 63;
+    for (int q = 0; q < 2; q++) {
+      const int cc = 2 * (tid >> 6) + q, x = tid & 63;
       const int c = c0 + cc;
       // C is zero-padded to nc x nc; states c >= N contribute nothing
       const cplx<float> cv = C[(long long)(c < nc ? c : nc - 1) * nc + r0 + x];
@@ -354,10 +357,10 @@ __global__ __launch_bounds__(256) void rotate_bf16x3_kernel(const cplx<float>* i
   load_step(0);
   for (int c0 = 0; c0 < cend; c0 += 16) {
     {
-      // the (re, im) pairs of inputs 4 w .. 4 w + 3, per split part
-      bf8 ap[3], bp[3];
+      // the (re, im) pairs of inputs 2 w, 2 w + 1, per split part
+      bf4v ap[3], bp[3];
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
+      for (int q = 0; q < 2; q++) {
         const cplx<float> a = nxa[q];
         const cplx<float> pv = nxp[q];
         __bf16 s[4][3];
@@ -373,63 +376,59 @@ __global__ __launch_bounds__(256) void rotate_bf16x3_kernel(const cplx<float>* i
           bp[sp][2 * q + 1] = s[3][sp];
         }
       }
-      const int x = tid & 63, k8 = 8 * (tid >> 6);
+      const int x = tid & 63, k4 = 4 * (tid >> 6);
 #pragma unroll
       for (int sp = 0; sp < 3; sp++) {
-        *reinterpret_cast<bf8*>(&As[sp][x][k8]) = ap[sp];
-        *reinterpret_cast<bf8*>(&Bs[sp][x][k8]) = bp[sp];
+        *reinterpret_cast<bf4v*>(&As[sp][x][k4]) = ap[sp];
+        *reinterpret_cast<bf4v*>(&Bs[sp][x][k4]) = bp[sp];
       }
     }
     __syncthreads();
     if (c0 + 16 < cend) load_step(c0 + 16);
     const int kcol = 8 * (lane >> 4);
-    bf8 a[3][2], br[3][2], bi[3][2];
+    bf8 a[3][2], br[3], bi[3];
 #pragma unroll
-    for (int sp = 0; sp < 3; sp++)
+    for (int sp = 0; sp < 3; sp++) {
 #pragma unroll
-      for (int x = 0; x < 2; x++) {
+      for (int x = 0; x < 2; x++)
         a[sp][x] = *reinterpret_cast<const bf8*>(&As[sp][wr * 32 + x * 16 + (lane & 15)][kcol]);
-        const bf8 b = *reinterpret_cast<const bf8*>(&Bs[sp][wt * 32 + x * 16 + (lane & 15)][kcol]);
-        br[sp][x] = negodd(b);
-        bi[sp][x] = swappairs(b);
-      }
+      const bf8 b = *reinterpret_cast<const bf8*>(&Bs[sp][wt * 16 + (lane & 15)][kcol]);
+      br[sp] = negodd(b);
+      bi[sp] = swappairs(b);
+    }
 #pragma unroll
-    for (int x = 0; x < 2; x++)
-#pragma unroll
-      for (int y = 0; y < 2; y++) {
-        // split products with p + q <= 4 (1-based), smallest first
-        re[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][x], br[0][y], re[x][y], 0, 0, 0);
-        re[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], br[2][y], re[x][y], 0, 0, 0);
-        re[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][x], br[1][y], re[x][y], 0, 0, 0);
-        re[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][x], br[0][y], re[x][y], 0, 0, 0);
-        re[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], br[1][y], re[x][y], 0, 0, 0);
-        re[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], br[0][y], re[x][y], 0, 0, 0);
-        im[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][x], bi[0][y], im[x][y], 0, 0, 0);
-        im[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], bi[2][y], im[x][y], 0, 0, 0);
-        im[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][x], bi[1][y], im[x][y], 0, 0, 0);
-        im[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][x], bi[0][y], im[x][y], 0, 0, 0);
-        im[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], bi[1][y], im[x][y], 0, 0, 0);
-        im[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], bi[0][y], im[x][y], 0, 0, 0);
-      }
+    for (int x = 0; x < 2; x++) {
+      // split products with p + q <= 4 (1-based), smallest first
+      re[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][x], br[0], re[x], 0, 0, 0);
+      re[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], br[2], re[x], 0, 0, 0);
+      re[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][x], br[1], re[x], 0, 0, 0);
+      re[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][x], br[0], re[x], 0, 0, 0);
+      re[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], br[1], re[x], 0, 0, 0);
+      re[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], br[0], re[x], 0, 0, 0);
+      im[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][x], bi[0], im[x], 0, 0, 0);
+      im[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], bi[2], im[x], 0, 0, 0);
+      im[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][x], bi[1], im[x], 0, 0, 0);
+      im[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][x], bi[0], im[x], 0, 0, 0);
+      im[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], bi[1], im[x], 0, 0, 0);
+      im[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], bi[0], im[x], 0, 0, 0);
+    }
     __syncthreads();
   }
 #pragma unroll
   for (int x = 0; x < 2; x++)
 #pragma unroll
-    for (int y = 0; y < 2; y++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int r = r0 + wr * 32 + x * 16 + 4 * (lane >> 4) + v;
-        const long long t = t0 + wt * 32 + y * 16 + (lane & 15);
-        if (r < N && t < M) out[(long long)r * M + t] = mk<float>(re[x][y][v], im[x][y][v]);
-      }
+    for (int v = 0; v < 4; v++) {
+      const int r = r0 + wr * 32 + x * 16 + 4 * (lane >> 4) + v;
+      const long long t = t0 + wt * 16 + (lane & 15);
+      if (r < N && t < M) out[(long long)r * M + t] = mk<float>(re[x][v], im[x][v]);
+    }
 }
 
 hipError_t launch_rotate_bf16x3(const cplx<float>* in, cplx<float>* out, const cplx<float>* C, int nc, long long M,
                                 int N, hipStream_t st, int upper) {
   const long long blocks = xcd_grid(nc / 64, (M + 63) / 64);
   if (blocks > 0x7fffffffLL || nc % 64) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(rotate_bf16x3_kernel, dim3((unsigned)blocks), dim3(256), 0, st, in, out, C, nc, M, N, upper);
+  hipLaunchKernelGGL(rotate_bf16x3_kernel, dim3((unsigned)blocks), dim3(kR3Threads), 0, st, in, out, C, nc, M, N, upper);
   return hipGetLastError();
 }
 
