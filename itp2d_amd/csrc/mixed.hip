@@ -43,6 +43,15 @@ __device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l)
   m = (__bf16)r1;
   l = (__bf16)(r1 - (float)m);
 }
+// the same for two values at once: each rounding is one packed conversion
+typedef float f2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void split3x2(f2v x, bf2v& h, bf2v& m, bf2v& l) {
+  h = __builtin_convertvector(x, bf2v);
+  const f2v r1 = x - __builtin_convertvector(h, f2v);
+  m = __builtin_convertvector(r1, bf2v);
+  l = __builtin_convertvector(r1 - __builtin_convertvector(m, f2v), bf2v);
+}
+__device__ __forceinline__ bf4v cat2(bf2v a, bf2v b) { return __builtin_shufflevector(a, b, 0, 1, 2, 3); }
 
 // (b0, b1, b2, ...) -> (b1, -b0, b3, -b2, ...): the operand of the imaginary part
 __device__ __forceinline__ bf8 swapneg(bf8 b) {
@@ -123,22 +132,17 @@ __global__ __launch_bounds__(kG3Threads) void gram_bf16x3_kernel(const float* A,
       const int idx = tid + kG3Threads * j;
       const int row = idx >> 4, c4 = idx & 15;
       const float4 va = nxa[j], vb = nxb[j];
-      const float xa[4] = {va.x, va.y, va.z, va.w}, xb[4] = {vb.x, vb.y, vb.z, vb.w};
-      bf4v ah, am, al, bh, bm, bl;
-#pragma unroll
-      for (int e = 0; e < 4; e++) {
-        __bf16 h, m, l;
-        split3(xa[e], h, m, l);
-        ah[e] = h; am[e] = m; al[e] = l;
-        split3(xb[e], h, m, l);
-        bh[e] = h; bm[e] = m; bl[e] = l;
-      }
-      *reinterpret_cast<bf4v*>(&As[0][row][4 * c4]) = ah;
-      *reinterpret_cast<bf4v*>(&As[1][row][4 * c4]) = am;
-      *reinterpret_cast<bf4v*>(&As[2][row][4 * c4]) = al;
-      *reinterpret_cast<bf4v*>(&Bs[0][row][4 * c4]) = bh;
-      *reinterpret_cast<bf4v*>(&Bs[1][row][4 * c4]) = bm;
-      *reinterpret_cast<bf4v*>(&Bs[2][row][4 * c4]) = bl;
+      bf2v h0, m0, l0, h1, m1, l1;
+      split3x2(f2v{va.x, va.y}, h0, m0, l0);
+      split3x2(f2v{va.z, va.w}, h1, m1, l1);
+      *reinterpret_cast<bf4v*>(&As[0][row][4 * c4]) = cat2(h0, h1);
+      *reinterpret_cast<bf4v*>(&As[1][row][4 * c4]) = cat2(m0, m1);
+      *reinterpret_cast<bf4v*>(&As[2][row][4 * c4]) = cat2(l0, l1);
+      split3x2(f2v{vb.x, vb.y}, h0, m0, l0);
+      split3x2(f2v{vb.z, vb.w}, h1, m1, l1);
+      *reinterpret_cast<bf4v*>(&Bs[0][row][4 * c4]) = cat2(h0, h1);
+      *reinterpret_cast<bf4v*>(&Bs[1][row][4 * c4]) = cat2(m0, m1);
+      *reinterpret_cast<bf4v*>(&Bs[2][row][4 * c4]) = cat2(l0, l1);
     }
     __syncthreads();
     if (k0 + kG3KT < k_end) load_step(k0 + kG3KT);
@@ -358,29 +362,17 @@ __global__ __launch_bounds__(kR3Threads) __attribute__((amdgpu_waves_per_eu(6)))
   for (int c0 = 0; c0 < cend; c0 += 16) {
     {
       // the (re, im) pairs of inputs 2 w, 2 w + 1, per split part
-      bf4v ap[3], bp[3];
+      bf2v a[2][3], p[2][3];
 #pragma unroll
       for (int q = 0; q < 2; q++) {
-        const cplx<float> a = nxa[q];
-        const cplx<float> pv = nxp[q];
-        __bf16 s[4][3];
-        split3(a.x, s[0][0], s[0][1], s[0][2]);
-        split3(a.y, s[1][0], s[1][1], s[1][2]);
-        split3(pv.x, s[2][0], s[2][1], s[2][2]);
-        split3(pv.y, s[3][0], s[3][1], s[3][2]);
-#pragma unroll
-        for (int sp = 0; sp < 3; sp++) {
-          ap[sp][2 * q] = s[0][sp];
-          ap[sp][2 * q + 1] = s[1][sp];
-          bp[sp][2 * q] = s[2][sp];
-          bp[sp][2 * q + 1] = s[3][sp];
-        }
+        split3x2(f2v{nxa[q].x, nxa[q].y}, a[q][0], a[q][1], a[q][2]);
+        split3x2(f2v{nxp[q].x, nxp[q].y}, p[q][0], p[q][1], p[q][2]);
       }
       const int x = tid & 63, k4 = 4 * (tid >> 6);
 #pragma unroll
       for (int sp = 0; sp < 3; sp++) {
-        *reinterpret_cast<bf4v*>(&As[sp][x][k4]) = ap[sp];
-        *reinterpret_cast<bf4v*>(&Bs[sp][x][k4]) = bp[sp];
+        *reinterpret_cast<bf4v*>(&As[sp][x][k4]) = cat2(a[0][sp], a[1][sp]);
+        *reinterpret_cast<bf4v*>(&Bs[sp][x][k4]) = cat2(p[0][sp], p[1][sp]);
       }
     }
     __syncthreads();
