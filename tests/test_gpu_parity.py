@@ -630,6 +630,42 @@ def test_rayleigh_ritz_fp64(ctx, bt, B):
     assert np.abs(G - np.eye(N)).max() < 1e-12
 
 
+@pytest.mark.parametrize("bt,B", BRANCHES)
+def test_rayleigh_ritz_fp32(ctx, bt, B):
+    """Rayleigh-Ritz on a complex-FP32 set (split-BF16x3 H-matrix, in full with
+    its Hermitian part on the magnetic Dirichlet branch) against the oracle's
+    H-matrix on the span of the same FP32-rounded states: Ritz values and
+    orthonormality at FP32 level, residual norms against the oracle's H applied
+    to the returned states."""
+    say(f"rayleigh-ritz fp32 bt={bt} B={B}")
+    sx, N = 48, 12
+    dx = 12.0 / sx
+    g = I.Grid(ctx, sx, sx, dx, bt)
+    V = W.harmonic(sx, sx, dx)
+    pot = I.Potential(g, V)
+    H = I.Hamiltonian(pot, B)
+    f = rand_states(N, sx, sx, dx, 43).astype(np.complex64).astype(np.complex128)
+    S = I.StateSet(g, N, I.FP32)
+    S.upload(f)
+    E, sd = H.rayleigh_ritz(S)
+    out = S.download()
+    Q, _ = np.linalg.qr(f.reshape(N, -1).T)
+    Qf = (Q.T / dx).reshape(N, sx, sx)
+    HQ = O.apply_hamiltonian(Qf, dx, bt, B, V)
+    Hm = (Qf.reshape(N, -1).conj() @ HQ.reshape(N, -1).T) * dx * dx
+    El = np.linalg.eigvalsh(0.5 * (Hm + Hm.conj().T))
+    scale = np.abs(El).max()
+    say(f"  E err {np.abs(E - El).max() / scale:.2e}")
+    assert np.abs(E - El).max() < 1e-6 * scale
+    Ho = O.apply_hamiltonian(out, dx, bt, B, V)
+    r = np.sqrt((np.abs(Ho - E[:, None, None] * out) ** 2).sum(axis=(1, 2))) * dx
+    say(f"  sigma err {np.abs(sd - r).max() / scale:.2e}")
+    assert np.abs(sd - r).max() < 1e-6 * scale
+    G = (out.reshape(N, -1).conj() @ out.reshape(N, -1).T) * dx * dx
+    say(f"  orthonormality {np.abs(G - np.eye(N)).max():.2e}")
+    assert np.abs(G - np.eye(N)).max() < 1e-6
+
+
 @pytest.mark.parametrize("name", ["c1_ho64", "c2_small"])
 def test_mixed_itp_matches_fp64(ctx, name):
     """The mixed path (FP32 ITP, FP64 Rayleigh-Ritz pass) against the FP64 golden
