@@ -318,12 +318,12 @@ __device__ __forceinline__ bf8 swappairs(bf8 b) {
 
 // Eight wavefronts per workgroup: wavefront (wr, wt) = (w / 4, w % 4) holds
 // output states wr*32 .. +31 against grid points wt*16 .. +15 (two 16 x 16
-// MFMA tiles), which keeps a wavefront's registers low enough for four
-// wavefronts per SIMD.
+// MFMA tiles). The register bound of six wavefronts per SIMD (at most 85
+// VGPRs; 78 without spills) lets three workgroups share a CU (30 KiB of LDS
+// each); at 88 VGPRs only two fit, which measured slower.
 constexpr int kR3Threads = 512;
-__global__ __launch_bounds__(kR3Threads) __attribute__((amdgpu_waves_per_eu(6))) void rotate_bf16x3_kernel(const cplx<float>* in, cplx<float>* out,
-                                                                   const cplx<float>* C, int nc, long long M, int N,
-                                                                   int upper) {
+__global__ __launch_bounds__(kR3Threads) __attribute__((amdgpu_waves_per_eu(6))) void rotate_bf16x3_kernel(
+    const cplx<float>* in, cplx<float>* out, const cplx<float>* C, int nc, long long M, int N, int upper) {
   __shared__ __align__(16) __bf16 As[3][64][32 + kR3Pad];  // [part][output state r][interleaved input k]
   __shared__ __align__(16) __bf16 Bs[3][64][32 + kR3Pad];  // [part][grid point t][interleaved input k]
   int rt;
