@@ -55,8 +55,9 @@ def test_cholesky_fp64_matches_numpy(ctx, N, sx):
     assert loss(out) < 10 * loss(ref) + 1e-13
 
 
-@pytest.mark.parametrize("N,sx", [(40, 64), (500, 64)])
+@pytest.mark.parametrize("N,sx", [(40, 64), (500, 64), (70, 20)])
 def test_cholesky_fp32_orthonormal_same_span(ctx, N, sx):
+    # 20 x 20: the rotation's last tile of 64 grid points is partial (M = 400)
     say(f"cholesky fp32 N={N}")
     dx = 12.0 / sx
     g = I.Grid(ctx, sx, sx, dx, I.PERIODIC)
